@@ -1,0 +1,447 @@
+// C ABI of tilenufft: plan lifetime, launch tables and entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tn_internal.h"
+
+namespace tn {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+void clear_error() { g_error.clear(); }
+
+const char *last_error() { return g_error.c_str(); }
+
+void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t samp_off,
+               int64_t w_off, std::vector<Job> &jobs) {
+    const tn_plan_params &p = plan->p;
+    const int64_t gvol = (int64_t)p.grid[0] * p.grid[1] * p.grid[2];
+    for (int f = 0; f < p.nframes; ++f) {
+        Job j{};
+        for (int d = 0; d < 3; ++d) {
+            const int zyx = 2 - d;
+            j.G[d] = p.grid[zyx];
+            j.W[d] = p.width[zyx];
+            j.shift[d] = p.shift[zyx];
+            j.T[d] = plan->tile[zyx];
+            j.NT[d] = plan->ntiles[zyx];
+            j.beta[d] = p.beta[zyx];
+        }
+        j.inv_scale = p.inv_scale;
+        j.ndim = p.ndim;
+        j.nbatch = (int32_t)nbatch;
+        j.npts = p.npts;
+        j.coord = plan->d_coord + (int64_t)f * p.npts * p.ndim;
+        j.entries = plan->d_entries;
+        j.bin_off = plan->d_bin_off + (int64_t)f * plan->ntiles_total;
+        // deepmr layout (batch, nframes, ...): frame f of coil b
+        j.grid_off = grid_off + (int64_t)f * gvol;
+        j.grid_bstride = (int64_t)p.nframes * gvol;
+        j.samp_off = samp_off + (int64_t)f * p.npts;
+        j.samp_bstride = (int64_t)p.nframes * p.npts;
+        j.w_off = w_off < 0 ? -1 : w_off + (int64_t)f * p.npts;
+        jobs.push_back(j);
+    }
+}
+
+namespace {
+
+int chunk_for(int requested, int64_t nbatch, int fallback) {
+    int c = requested > 0 ? requested : fallback;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(c, nbatch));
+}
+
+}  // namespace
+
+void free_tables(LaunchTables &t) {
+    if (t.d_jobs) (void)hipFree(t.d_jobs);
+    if (t.d_interp) (void)hipFree(t.d_interp);
+    if (t.d_grid) (void)hipFree(t.d_grid);
+    t = LaunchTables{};
+}
+
+static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &interp,
+                         const std::vector<Item> &grid, LaunchTables &t) {
+    t.n_interp = (int64_t)interp.size();
+    t.n_grid = (int64_t)grid.size();
+    if (t.n_interp >= INT32_MAX || t.n_grid >= INT32_MAX)
+        return fail(TN_E_RANGE, "work list too long");
+    TN_HIP(hipMalloc(&t.d_jobs, sizeof(Job) * std::max<size_t>(jobs.size(), 1)));
+    TN_HIP(hipMalloc(&t.d_interp, sizeof(Item) * std::max<size_t>(interp.size(), 1)));
+    TN_HIP(hipMalloc(&t.d_grid, sizeof(Item) * std::max<size_t>(grid.size(), 1)));
+    if (!jobs.empty())
+        TN_HIP(hipMemcpy(t.d_jobs, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice));
+    if (!interp.empty())
+        TN_HIP(hipMemcpy(t.d_interp, interp.data(), sizeof(Item) * interp.size(),
+                         hipMemcpyHostToDevice));
+    if (!grid.empty())
+        TN_HIP(hipMemcpy(t.d_grid, grid.data(), sizeof(Item) * grid.size(), hipMemcpyHostToDevice));
+    return TN_OK;
+}
+
+// Work lists. Interpolation items are (job, sample block, coil range);
+// gridding items are (job, tile, coil range). `plan_of[j]` is the plan job j
+// came from (for its chunk preferences).
+static int build_tables_for(const std::vector<Job> &jobs, const std::vector<const tn_plan *> &plan_of,
+                            LaunchTables &t) {
+    t.kind_interp = interp_kind(jobs);
+    t.kind_grid = t.kind_interp;
+    int64_t max_cells = 1;
+    for (const Job &j : jobs) max_cells = std::max<int64_t>(max_cells, (int64_t)j.T[0] * j.T[1] * j.T[2]);
+    const size_t per_coil = (size_t)max_cells * 2 * sizeof(float);
+    int lds_coils = (int)std::max<size_t>(1, kGridLdsBudget / per_coil);
+    if (per_coil > 160 * 1024) return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes)", per_coil);
+
+    int max_gc = 1;
+    std::vector<Item> interp, grid;
+    for (size_t ji = 0; ji < jobs.size(); ++ji) {
+        const Job &j = jobs[ji];
+        const tn_plan *pl = plan_of[ji];
+        const int ic = chunk_for(pl->p.interp_chunk, j.nbatch, j.nbatch >= 8 ? (int)((j.nbatch + 1) / 2) : j.nbatch);
+        const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, lds_coils), lds_coils);
+        max_gc = std::max(max_gc, gc);
+        const int64_t nblk = (j.npts + kInterpBlockPts - 1) / kInterpBlockPts;
+        for (int64_t b = 0; b < nblk; ++b)
+            for (int c0 = 0; c0 < j.nbatch; c0 += ic)
+                interp.push_back(make_int4((int)ji, (int)b, c0, std::min(j.nbatch, c0 + ic)));
+        const int64_t ntiles = (int64_t)j.NT[0] * j.NT[1] * j.NT[2];
+        for (int64_t tl = 0; tl < ntiles; ++tl)
+            for (int c0 = 0; c0 < j.nbatch; c0 += gc)
+                grid.push_back(make_int4((int)ji, (int)tl, c0, std::min(j.nbatch, c0 + gc)));
+    }
+    t.grid_coils = max_gc;
+    t.grid_lds = per_coil * (size_t)max_gc;
+    return upload_tables(jobs, interp, grid, t);
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+namespace {
+
+int validate(const tn_plan_params *p) {
+    if (!p) return fail(TN_E_ARG, "params is NULL");
+    if (p->ndim != 2 && p->ndim != 3) return fail(TN_E_ARG, "ndim must be 2 or 3 (got %d)", p->ndim);
+    if (p->nframes < 1) return fail(TN_E_ARG, "nframes must be >= 1");
+    if (p->npts < 0 || p->npts >= INT32_MAX) return fail(TN_E_RANGE, "npts out of range");
+    for (int d = 0; d < 3; ++d) {
+        const bool used = d >= 3 - p->ndim;
+        if (!used) {
+            if (p->grid[d] != 1 || p->shape[d] != 1 || p->width[d] != 1)
+                return fail(TN_E_ARG, "unused axis %d must have grid = shape = width = 1", d);
+            continue;
+        }
+        if (p->shape[d] < 1 || p->grid[d] < p->shape[d])
+            return fail(TN_E_ARG, "axis %d: need 1 <= shape (%d) <= grid (%d)", d, p->shape[d], p->grid[d]);
+        if (p->width[d] < 1 || p->width[d] > 8 || p->width[d] > p->grid[d])
+            return fail(TN_E_ARG, "axis %d: width %d unsupported (1..8, <= grid)", d, p->width[d]);
+        if (p->shift[d] < 0 || p->shift[d] >= p->grid[d])
+            return fail(TN_E_ARG, "axis %d: shift %d outside [0, grid)", d, p->shift[d]);
+        if (p->tile[d] < 0) return fail(TN_E_ARG, "axis %d: negative tile", d);
+    }
+    if ((int64_t)p->grid[0] * p->grid[1] * p->grid[2] >= INT32_MAX)
+        return fail(TN_E_RANGE, "grid too large");
+    return TN_OK;
+}
+
+void choose_tiles(tn_plan *pl) {
+    const tn_plan_params &p = pl->p;
+    int def[3];
+    if (p.ndim == 2) {
+        def[0] = 1; def[1] = 32; def[2] = 32;
+    } else {
+        def[0] = 16; def[1] = 16; def[2] = 16;
+    }
+    pl->ntiles_total = 1;
+    for (int d = 0; d < 3; ++d) {
+        int t = p.tile[d] > 0 ? p.tile[d] : def[d];
+        t = std::min(t, p.grid[d]);
+        pl->tile[d] = t;
+        pl->ntiles[d] = (p.grid[d] + t - 1) / t;
+        pl->ntiles_total *= pl->ntiles[d];
+    }
+}
+
+void destroy(tn_plan *pl) {
+    if (!pl) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(pl->device);
+    for (auto &kv : pl->cache) free_tables(kv.second);
+    if (pl->d_coord) (void)hipFree(pl->d_coord);
+    if (pl->d_entries) (void)hipFree(pl->d_entries);
+    if (pl->d_bin_off) (void)hipFree(pl->d_bin_off);
+    for (int d = 0; d < 3; ++d)
+        if (pl->d_apod[d]) (void)hipFree(pl->d_apod[d]);
+    (void)hipSetDevice(cur);
+    delete pl;
+}
+
+tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc) {
+    auto it = plan->cache.find(nbatch);
+    if (it != plan->cache.end()) {
+        *rc = TN_OK;
+        return &it->second;
+    }
+    std::vector<Job> jobs;
+    make_jobs(plan, nbatch, 0, 0, 0, jobs);
+    std::vector<const tn_plan *> owners(jobs.size(), plan);
+    LaunchTables t;
+    *rc = build_tables_for(jobs, owners, t);
+    if (*rc != TN_OK) {
+        free_tables(t);
+        return nullptr;
+    }
+    return &(plan->cache[nbatch] = t);
+}
+
+// Checks that the current device is the plan's device.
+int check_device(int device) {
+    int cur = -1;
+    TN_HIP(hipGetDevice(&cur));
+    if (cur != device)
+        return fail(TN_E_ARG, "plan lives on device %d but device %d is current", device, cur);
+    return TN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tn_last_error(void) { return tn::last_error(); }
+
+const char *tn_version(void) { return "tilenufft 0.1 (gfx950)"; }
+
+int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stream stream,
+                   tn_plan **out) {
+    clear_error();
+    if (!out) return fail(TN_E_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = validate(params);
+    if (rc) return rc;
+    if (!coord_dev && params->npts > 0) return fail(TN_E_ARG, "coord is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    tn_plan *pl = new tn_plan();
+    pl->p = *params;
+    for (int d = 0; d < 3; ++d) pl->p.apod[d] = nullptr;
+    if (hipGetDevice(&pl->device) != hipSuccess) {
+        delete pl;
+        return fail(TN_E_HIP, "hipGetDevice failed");
+    }
+    choose_tiles(pl);
+    const int64_t ncoord = (int64_t)params->nframes * params->npts * params->ndim;
+    auto cleanup = [&](int code) {
+        destroy(pl);
+        return code;
+    };
+    if (hipMalloc(&pl->d_coord, sizeof(float) * std::max<int64_t>(ncoord, 1)) != hipSuccess)
+        return cleanup(fail(TN_E_NOMEM, "hipMalloc of %lld coordinates failed", (long long)ncoord));
+    if (ncoord > 0 &&
+        hipMemcpyAsync(pl->d_coord, coord_dev, sizeof(float) * ncoord, hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+        return cleanup(fail(TN_E_HIP, "coordinate copy failed"));
+    for (int d = 0; d < 3; ++d) {
+        if (!params->apod[d] || params->width[d] == 1) continue;
+        const size_t n = (size_t)params->shape[d];
+        if (hipMalloc(&pl->d_apod[d], sizeof(float) * n) != hipSuccess)
+            return cleanup(fail(TN_E_NOMEM, "hipMalloc of apodization failed"));
+        if (hipMemcpy(pl->d_apod[d], params->apod[d], sizeof(float) * n, hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return cleanup(fail(TN_E_HIP, "apodization upload failed"));
+    }
+    rc = build_bins(pl, s);
+    if (rc) return cleanup(rc);
+    *out = pl;
+    return TN_OK;
+}
+
+int tn_plan_destroy(tn_plan *plan) {
+    clear_error();
+    destroy(plan);
+    return TN_OK;
+}
+
+int64_t tn_plan_device_bytes(const tn_plan *plan) {
+    if (!plan) return 0;
+    const tn_plan_params &p = plan->p;
+    int64_t b = sizeof(float) * (int64_t)p.nframes * p.npts * p.ndim;
+    b += sizeof(float4) * plan->nentries;
+    b += sizeof(int32_t) * (plan->ntiles_total * p.nframes + 1);
+    for (int d = 0; d < 3; ++d)
+        if (plan->d_apod[d]) b += sizeof(float) * p.shape[d];
+    for (const auto &kv : plan->cache)
+        b += sizeof(Job) * p.nframes + sizeof(Item) * (kv.second.n_interp + kv.second.n_grid);
+    return b;
+}
+
+int64_t tn_plan_entries(const tn_plan *plan) { return plan ? plan->nentries : -1; }
+
+int64_t tn_plan_footprint_cells(const tn_plan *plan_c) {
+    clear_error();
+    tn_plan *plan = const_cast<tn_plan *>(plan_c);
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (plan->footprint < 0) {
+        int rc = check_device(plan->device);
+        if (rc) return rc;
+        rc = count_footprint(plan, nullptr);
+        if (rc) return rc;
+    }
+    return plan->footprint;
+}
+
+int tn_plan_tile(const tn_plan *plan, int32_t tile_out[3]) {
+    clear_error();
+    if (!plan || !tile_out) return fail(TN_E_ARG, "NULL argument");
+    for (int d = 0; d < 3; ++d) tile_out[d] = plan->tile[d];
+    return TN_OK;
+}
+
+int tn_plan_prepare(tn_plan *plan, int64_t nbatch) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nbatch < 1 || nbatch >= INT32_MAX) return fail(TN_E_ARG, "nbatch must be >= 1");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    plan_tables(plan, nbatch, &rc);
+    return rc;
+}
+
+int tn_interp(tn_plan *plan, const void *grid, void *samp, int64_t nbatch, const float *weight,
+              tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nbatch < 1 || nbatch >= INT32_MAX) return fail(TN_E_ARG, "nbatch must be >= 1");
+    if ((!grid || !samp) && plan->p.npts > 0) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    LaunchTables *t = plan_tables(plan, nbatch, &rc);
+    if (!t) return rc;
+    return launch_interp(*t, (const float2 *)grid, (float2 *)samp, weight, (hipStream_t)stream);
+}
+
+int tn_grid(tn_plan *plan, const void *samp, void *grid, int64_t nbatch, const float *weight,
+            tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nbatch < 1 || nbatch >= INT32_MAX) return fail(TN_E_ARG, "nbatch must be >= 1");
+    if (!grid || (!samp && plan->p.npts > 0)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    LaunchTables *t = plan_tables(plan, nbatch, &rc);
+    if (!t) return rc;
+    return launch_grid(*t, (const float2 *)samp, (float2 *)grid, weight, (hipStream_t)stream);
+}
+
+int tn_apod_pad(tn_plan *plan, const void *image, void *grid, int64_t nimg, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nimg < 0) return fail(TN_E_ARG, "nimg must be >= 0");
+    if (nimg > 0 && (!image || !grid)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    return launch_apod_pad(plan, (const float2 *)image, (float2 *)grid, nimg, (hipStream_t)stream);
+}
+
+int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nimg < 0) return fail(TN_E_ARG, "nimg must be >= 0");
+    if (nimg > 0 && (!image || !grid)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    return launch_crop_apod(plan, (const float2 *)grid, (float2 *)image, nimg, (hipStream_t)stream);
+}
+
+int tn_batch_create(tn_plan *const *plans, const int64_t *nbatch, const int64_t *grid_offset,
+                    const int64_t *samp_offset, const int64_t *weight_offset, int32_t njobs,
+                    tn_batch **out) {
+    clear_error();
+    if (!out || !plans || !nbatch || !grid_offset || !samp_offset)
+        return fail(TN_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (njobs < 1) return fail(TN_E_ARG, "njobs must be >= 1");
+    std::vector<Job> jobs;
+    std::vector<const tn_plan *> owners;
+    int device = plans[0] ? plans[0]->device : -1;
+    int ndim = plans[0] ? plans[0]->p.ndim : 0;
+    for (int k = 0; k < njobs; ++k) {
+        const tn_plan *pl = plans[k];
+        if (!pl) return fail(TN_E_ARG, "plan %d is NULL", k);
+        if (pl->device != device) return fail(TN_E_ARG, "plans live on different devices");
+        if (pl->p.ndim != ndim) return fail(TN_E_ARG, "plans mix 2D and 3D");
+        if (nbatch[k] < 1 || nbatch[k] >= INT32_MAX) return fail(TN_E_ARG, "nbatch[%d] < 1", k);
+        if (grid_offset[k] < 0 || samp_offset[k] < 0) return fail(TN_E_ARG, "negative offset");
+        const size_t before = jobs.size();
+        make_jobs(pl, nbatch[k], grid_offset[k], samp_offset[k],
+                  weight_offset ? weight_offset[k] : -1, jobs);
+        owners.resize(jobs.size(), pl);
+        (void)before;
+    }
+    int rc = check_device(device);
+    if (rc) return rc;
+    tn_batch *b = new tn_batch();
+    b->device = device;
+    b->plans.assign(plans, plans + njobs);
+    rc = build_tables_for(jobs, owners, b->tables);
+    if (rc) {
+        free_tables(b->tables);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return TN_OK;
+}
+
+int tn_batch_destroy(tn_batch *batch) {
+    clear_error();
+    if (!batch) return TN_OK;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(batch->device);
+    free_tables(batch->tables);
+    (void)hipSetDevice(cur);
+    delete batch;
+    return TN_OK;
+}
+
+int tn_batch_interp(tn_batch *batch, const void *grid_base, void *samp_base,
+                    const float *weight_base, tn_stream stream) {
+    clear_error();
+    if (!batch || !grid_base || !samp_base) return fail(TN_E_ARG, "NULL argument");
+    int rc = check_device(batch->device);
+    if (rc) return rc;
+    return launch_interp(batch->tables, (const float2 *)grid_base, (float2 *)samp_base, weight_base,
+                         (hipStream_t)stream);
+}
+
+int tn_batch_grid(tn_batch *batch, const void *samp_base, void *grid_base, const float *weight_base,
+                  tn_stream stream) {
+    clear_error();
+    if (!batch || !grid_base || !samp_base) return fail(TN_E_ARG, "NULL argument");
+    int rc = check_device(batch->device);
+    if (rc) return rc;
+    return launch_grid(batch->tables, (const float2 *)samp_base, (float2 *)grid_base, weight_base,
+                       (hipStream_t)stream);
+}
+
+}  // extern "C"

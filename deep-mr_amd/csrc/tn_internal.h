@@ -1,0 +1,118 @@
+// Internal types shared by the tilenufft HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "tilenufft.h"
+
+namespace tn {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kWave = 64;
+constexpr int kMaxWidth = 16;      // largest kernel width supported per axis
+constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
+constexpr size_t kGridLdsBudget = 64 * 1024;  // LDS per gridding workgroup
+
+// One trajectory (frame) bound to its data placement for a launch. All
+// per-axis arrays are (x, y, z) so that axis d matches coordinate column d.
+struct alignas(16) Job {
+    int32_t G[3];       // oversampled grid (x, y, z); z = 1 in 2D
+    int32_t W[3];       // kernel width; 1 = Cartesian axis
+    int32_t shift[3];   // output index = (tap + shift) mod G
+    int32_t T[3];       // gridding tile
+    int32_t NT[3];      // tiles per axis
+    float beta[3];
+    float inv_scale;
+    int32_t ndim;
+    int32_t nbatch;     // coils (batch) of this job
+    int32_t pad0;
+    int64_t npts;
+    const float *coord;       // (npts, ndim) grid units
+    const float4 *entries;    // (x, y, z, bits(sample)) sorted by tile
+    const int32_t *bin_off;   // (ntiles + 1) offsets into entries
+    int64_t grid_off;         // complex elements from the grid base
+    int64_t grid_bstride;     // complex elements between coils
+    int64_t samp_off;
+    int64_t samp_bstride;
+    int64_t w_off;            // offset into the weight base (-1: no weight)
+};
+
+// Work item: {job, unit, first coil, end coil}; unit = sample block
+// (interpolation) or tile (gridding).
+using Item = int4;
+
+struct LaunchTables {
+    Job *d_jobs = nullptr;
+    Item *d_interp = nullptr;
+    Item *d_grid = nullptr;
+    int64_t n_interp = 0;
+    int64_t n_grid = 0;
+    int grid_coils = 1;        // coils per gridding workgroup (LDS bound)
+    size_t grid_lds = 0;       // dynamic LDS bytes for the gridding kernel
+    int kind_interp = 0;       // kernel variant selector
+    int kind_grid = 0;
+};
+
+}  // namespace tn
+
+struct tn_plan {
+    int device = 0;
+    tn_plan_params p{};        // (z, y, x) as given; apod pointers cleared
+    int32_t tile[3] = {1, 1, 1};   // (z, y, x)
+    int32_t ntiles[3] = {1, 1, 1};
+    int64_t ntiles_total = 1;
+    float *d_coord = nullptr;
+    float4 *d_entries = nullptr;
+    int32_t *d_bin_off = nullptr;
+    float *d_apod[3] = {nullptr, nullptr, nullptr};
+    int64_t nentries = 0;
+    int64_t footprint = -1;
+    std::map<int64_t, tn::LaunchTables> cache;
+};
+
+struct tn_batch {
+    int device = 0;
+    tn::LaunchTables tables;
+    std::vector<tn_plan *> plans;
+};
+
+namespace tn {
+
+// Error reporting (thread-local message + return code).
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+// Fills job records for every frame of `plan` with the given placement.
+void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off,
+               int64_t samp_off, int64_t w_off, std::vector<Job> &jobs);
+void free_tables(LaunchTables &t);
+
+// Kernel launchers (defined in the kernel translation units).
+int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp,
+                  const float *weight, hipStream_t stream);
+int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid,
+                const float *weight, hipStream_t stream);
+int launch_apod_pad(const tn_plan *plan, const float2 *image, float2 *grid,
+                    int64_t nimg, hipStream_t stream);
+int launch_crop_apod(const tn_plan *plan, const float2 *grid, float2 *image,
+                     int64_t nimg, hipStream_t stream);
+int build_bins(tn_plan *plan, hipStream_t stream);
+int count_footprint(tn_plan *plan, hipStream_t stream);
+
+// Variant selection shared by the table builder and the launchers.
+int interp_kind(const std::vector<Job> &jobs);
+
+}  // namespace tn
+
+#define TN_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::tn::fail(TN_E_HIP, "%s failed: %s (%s:%d)", #call,       \
+                              hipGetErrorString(e_), __FILE__, __LINE__);     \
+    } while (0)

@@ -1,0 +1,301 @@
+// Plan construction on the device: tile binning of every sample.
+//
+// Each sample is entered into every gridding tile its kernel footprint
+// touches (owner-computes: a tile later accumulates only the taps that fall
+// inside it, so every tile is written exactly once without global atomics).
+// Entries are keyed by (frame, tile) and sorted with a stable radix sort, so
+// inside a tile the samples keep ascending sample order: the gridding
+// accumulation order is fixed and the adjoint is deterministic.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tn_internal.h"
+#include "tn_kb.h"
+
+namespace tn {
+namespace {
+
+struct AxisGeom {
+    int G, W, shift, T;
+};
+
+struct BinGeom {
+    int ndim;
+    int64_t npts;
+    int nframes;
+    AxisGeom ax[3];      // (x, y, z)
+    int NT[3];           // tiles per axis (x, y, z)
+    int64_t ntiles;      // product of NT
+};
+
+// Distinct tiles touched along one axis by a footprint starting at `base`.
+// A footprint that wraps around the grid edge can come back into a tile it
+// already touched (when the last tile is narrower than W - 1), so the check
+// is against every tile collected so far, not only the previous one.
+__device__ __forceinline__ int axis_tiles(const AxisGeom &a, int base, int (&tl)[kMaxWidth]) {
+    int n = 0;
+    for (int k = 0; k < a.W; ++k) {
+        const int t = wrap_add(base, k, a.G) / a.T;
+        bool seen = false;
+        for (int j = 0; j < n; ++j) seen |= (tl[j] == t);
+        if (!seen) tl[n++] = t;
+    }
+    return n;
+}
+
+__device__ __forceinline__ int axis_base(const AxisGeom &a, float g) {
+    if (a.W == 1) return wrap_mod((int)g + a.shift, a.G);
+    const float x0 = ceilf(g - 0.5f * (float)a.W);
+    return wrap_mod((int)x0 + a.shift, a.G);
+}
+
+__global__ void bin_count_kernel(BinGeom geo, const float *__restrict__ coord,
+                                 int32_t *__restrict__ counts) {
+    const int64_t total = geo.npts * geo.nframes;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int c = 1;
+        for (int d = 0; d < geo.ndim; ++d) {
+            int tl[kMaxWidth];
+            const int base = axis_base(geo.ax[d], coord[i * geo.ndim + d]);
+            c *= axis_tiles(geo.ax[d], base, tl);
+        }
+        counts[i] = c;
+    }
+}
+
+__global__ void bin_emit_kernel(BinGeom geo, const float *__restrict__ coord,
+                                const int32_t *__restrict__ offsets,
+                                uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int64_t total = geo.npts * geo.nframes;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frame = i / geo.npts;
+        const int32_t pt = (int32_t)(i - frame * geo.npts);
+        int tl[3][kMaxWidth];
+        int nt[3] = {1, 1, 1};
+        for (int d = 0; d < 3; ++d) tl[d][0] = 0;
+        for (int d = 0; d < geo.ndim; ++d) {
+            const int base = axis_base(geo.ax[d], coord[i * geo.ndim + d]);
+            nt[d] = axis_tiles(geo.ax[d], base, tl[d]);
+        }
+        int32_t pos = offsets[i];
+        for (int a = 0; a < nt[2]; ++a)
+            for (int b = 0; b < nt[1]; ++b)
+                for (int c = 0; c < nt[0]; ++c) {
+                    const int64_t tile =
+                        ((int64_t)tl[2][a] * geo.NT[1] + tl[1][b]) * geo.NT[0] + tl[0][c];
+                    keys[pos] = (uint32_t)(frame * geo.ntiles + tile);
+                    vals[pos] = pt;
+                    ++pos;
+                }
+    }
+}
+
+// bin_off[k] = first entry whose key >= k, for k in [0, nbins]; e == n
+// stands for a virtual key nbins.
+__global__ void bin_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n, int64_t nbins,
+                                   int32_t *__restrict__ bin_off) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = e < n ? (int64_t)keys[e] : nbins;
+        const int64_t kp = e > 0 ? (int64_t)keys[e - 1] : -1;
+        for (int64_t b = kp + 1; b <= k; ++b) bin_off[b] = (int32_t)e;
+    }
+}
+
+__global__ void bin_gather_kernel(BinGeom geo, const float *__restrict__ coord,
+                                  const uint32_t *__restrict__ keys,
+                                  const int32_t *__restrict__ vals, int64_t n,
+                                  float4 *__restrict__ entries) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frame = keys[e] / geo.ntiles;
+        const int32_t pt = vals[e];
+        const float *c = coord + (frame * geo.npts + pt) * geo.ndim;
+        float4 v;
+        v.x = c[0];
+        v.y = geo.ndim > 1 ? c[1] : 0.0f;
+        v.z = geo.ndim > 2 ? c[2] : 0.0f;
+        v.w = __int_as_float(pt);
+        entries[e] = v;
+    }
+}
+
+// Marks every grid cell touched by a footprint (one frame).
+__global__ void footprint_mark_kernel(BinGeom geo, const float *__restrict__ coord,
+                                      uint8_t *__restrict__ mask) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < geo.npts;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int base[3] = {0, 0, 0};
+        int w[3] = {1, 1, 1};
+        for (int d = 0; d < geo.ndim; ++d) {
+            base[d] = axis_base(geo.ax[d], coord[i * geo.ndim + d]);
+            w[d] = geo.ax[d].W;
+        }
+        const int Gx = geo.ax[0].G, Gy = geo.ndim > 1 ? geo.ax[1].G : 1;
+        const int Gz = geo.ndim > 2 ? geo.ax[2].G : 1;
+        for (int a = 0; a < w[2]; ++a) {
+            const int z = geo.ndim > 2 ? wrap_add(base[2], a, Gz) : 0;
+            for (int b = 0; b < w[1]; ++b) {
+                const int y = geo.ndim > 1 ? wrap_add(base[1], b, Gy) : 0;
+                for (int c = 0; c < w[0]; ++c) {
+                    const int x = wrap_add(base[0], c, Gx);
+                    mask[((int64_t)z * Gy + y) * Gx + x] = 1;
+                }
+            }
+        }
+    }
+}
+
+__global__ void count_mask_kernel(const uint8_t *__restrict__ mask, int64_t n,
+                                  unsigned long long *__restrict__ out) {
+    unsigned long long local = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        local += mask[i];
+    __shared__ unsigned long long red[kBlock];
+    red[threadIdx.x] = local;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+BinGeom make_geom(const tn_plan *plan) {
+    BinGeom g{};
+    g.ndim = plan->p.ndim;
+    g.npts = plan->p.npts;
+    g.nframes = plan->p.nframes;
+    for (int d = 0; d < 3; ++d) {
+        const int zyx = 2 - d;
+        g.ax[d].G = plan->p.grid[zyx];
+        g.ax[d].W = plan->p.width[zyx];
+        g.ax[d].shift = plan->p.shift[zyx];
+        g.ax[d].T = plan->tile[zyx];
+        g.NT[d] = plan->ntiles[zyx];
+    }
+    g.ntiles = plan->ntiles_total;
+    return g;
+}
+
+unsigned grid_blocks(int64_t n) {
+    const int64_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+// RAII holder for temporaries of the plan build.
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+int build_bins(tn_plan *plan, hipStream_t stream) {
+    const BinGeom geo = make_geom(plan);
+    const int64_t total = geo.npts * geo.nframes;
+    const int64_t nbins = geo.nframes * geo.ntiles;
+    if (nbins >= (int64_t)UINT32_MAX) return fail(TN_E_RANGE, "too many tiles (%lld)", (long long)nbins);
+
+    TN_HIP(hipMalloc(&plan->d_bin_off, sizeof(int32_t) * (nbins + 1)));
+    if (total == 0) {
+        TN_HIP(hipMemsetAsync(plan->d_bin_off, 0, sizeof(int32_t) * (nbins + 1), stream));
+        plan->nentries = 0;
+        return TN_OK;
+    }
+
+    DevBuf counts, offsets, scan_tmp;
+    TN_HIP(hipMalloc(&counts.p, sizeof(int32_t) * total));
+    TN_HIP(hipMalloc(&offsets.p, sizeof(int32_t) * (total + 1)));
+    hipLaunchKernelGGL(bin_count_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
+                       plan->d_coord, (int32_t *)counts.p);
+    TN_HIP(hipGetLastError());
+
+    // exclusive scan of the per-sample copy counts (total + 1 entries: the
+    // last one is the number of entries)
+    size_t tmp_bytes = 0;
+    TN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int32_t *)counts.p,
+                                            (int32_t *)offsets.p, (int)total, stream));
+    TN_HIP(hipMalloc(&scan_tmp.p, tmp_bytes));
+    TN_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tmp_bytes, (int32_t *)counts.p,
+                                            (int32_t *)offsets.p, (int)total, stream));
+    int32_t last_off = 0, last_cnt = 0;
+    TN_HIP(hipMemcpyAsync(&last_off, (int32_t *)offsets.p + (total - 1), sizeof(int32_t),
+                          hipMemcpyDeviceToHost, stream));
+    TN_HIP(hipMemcpyAsync(&last_cnt, (int32_t *)counts.p + (total - 1), sizeof(int32_t),
+                          hipMemcpyDeviceToHost, stream));
+    TN_HIP(hipStreamSynchronize(stream));
+    const int64_t n = (int64_t)last_off + last_cnt;
+    if (n >= INT32_MAX) return fail(TN_E_RANGE, "too many tile entries (%lld)", (long long)n);
+    plan->nentries = n;
+
+    DevBuf keys, vals, keys_sorted, vals_sorted, sort_tmp;
+    TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * n));
+    TN_HIP(hipMalloc(&vals.p, sizeof(int32_t) * n));
+    TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
+    TN_HIP(hipMalloc(&vals_sorted.p, sizeof(int32_t) * n));
+    hipLaunchKernelGGL(bin_emit_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
+                       plan->d_coord, (const int32_t *)offsets.p, (uint32_t *)keys.p,
+                       (int32_t *)vals.p);
+    TN_HIP(hipGetLastError());
+
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)nbins) ++end_bit;
+    size_t sort_bytes = 0;
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)keys.p,
+                                              (uint32_t *)keys_sorted.p, (int32_t *)vals.p,
+                                              (int32_t *)vals_sorted.p, (int)n, 0, end_bit,
+                                              stream));
+    TN_HIP(hipMalloc(&sort_tmp.p, sort_bytes));
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, sort_bytes, (uint32_t *)keys.p,
+                                              (uint32_t *)keys_sorted.p, (int32_t *)vals.p,
+                                              (int32_t *)vals_sorted.p, (int)n, 0, end_bit,
+                                              stream));
+
+    hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(n + 1)), dim3(kBlock), 0, stream,
+                       (const uint32_t *)keys_sorted.p, n, nbins, plan->d_bin_off);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipMalloc(&plan->d_entries, sizeof(float4) * std::max<int64_t>(n, 1)));
+    hipLaunchKernelGGL(bin_gather_kernel, dim3(grid_blocks(n)), dim3(kBlock), 0, stream, geo,
+                       plan->d_coord, (const uint32_t *)keys_sorted.p,
+                       (const int32_t *)vals_sorted.p, n, plan->d_entries);
+    TN_HIP(hipGetLastError());
+    // temporaries are freed on scope exit; make sure the device is done
+    TN_HIP(hipStreamSynchronize(stream));
+    return TN_OK;
+}
+
+int count_footprint(tn_plan *plan, hipStream_t stream) {
+    const BinGeom geo = make_geom(plan);
+    const int64_t gvol = (int64_t)plan->p.grid[0] * plan->p.grid[1] * plan->p.grid[2];
+    DevBuf mask, counter;
+    TN_HIP(hipMalloc(&mask.p, gvol));
+    TN_HIP(hipMalloc(&counter.p, sizeof(unsigned long long)));
+    TN_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), stream));
+    for (int f = 0; f < geo.nframes; ++f) {
+        TN_HIP(hipMemsetAsync(mask.p, 0, gvol, stream));
+        hipLaunchKernelGGL(footprint_mark_kernel, dim3(grid_blocks(geo.npts)), dim3(kBlock), 0,
+                           stream, geo, plan->d_coord + (int64_t)f * geo.npts * geo.ndim,
+                           (uint8_t *)mask.p);
+        TN_HIP(hipGetLastError());
+        hipLaunchKernelGGL(count_mask_kernel, dim3(grid_blocks(gvol)), dim3(kBlock), 0, stream,
+                           (const uint8_t *)mask.p, gvol, (unsigned long long *)counter.p);
+        TN_HIP(hipGetLastError());
+    }
+    unsigned long long cells = 0;
+    TN_HIP(hipMemcpyAsync(&cells, counter.p, sizeof(cells), hipMemcpyDeviceToHost, stream));
+    TN_HIP(hipStreamSynchronize(stream));
+    plan->footprint = (int64_t)cells;
+    return TN_OK;
+}
+
+}  // namespace tn

@@ -1,0 +1,51 @@
+"""Drop-in for the NUFFT part of ``deepmr.fft`` (src/deepmr/fft/__init__.py:145-286)."""
+
+__all__ = [
+    "fft",
+    "ifft",
+    "nufft",
+    "nufft_adj",
+    "plan_nufft",
+    "apply_nufft",
+    "apply_nufft_adj",
+    "NUFFTPlan",
+    "set_autograd_mode",
+    "get_autograd_mode",
+]
+
+from .fft import fft, ifft  # noqa: F401
+from .nufft import (  # noqa: F401
+    NUFFTPlan,
+    apply_nufft,
+    apply_nufft_adj,
+    get_autograd_mode,
+    plan_nufft,
+    set_autograd_mode,
+)
+
+
+def nufft(image, coord, shape=None, basis_adjoint=None, device="cpu", threadsperblock=128,
+          width=4, oversamp=1.25):
+    """N-dimensional NUFFT (deepmr ``nufft``, src/deepmr/fft/__init__.py:145-218).
+
+    image (..., ncontrasts, ny, nx) or (..., ncontrasts, nz, ny, nx);
+    coord (ncontrasts, nviews, nsamples, ndim) in [-N/2, N/2), x first.
+    Returns k-space (..., ncontrasts, nviews, nsamples) on the image's device.
+    Plans on every call, like deepmr; reuse ``plan_nufft`` + ``apply_nufft``
+    to amortise planning.
+    """
+    ndim = coord.shape[-1]
+    if shape is None:
+        shape = image.shape[-ndim:]
+    plan = plan_nufft(coord, shape, width, oversamp, device)
+    return apply_nufft(image, plan, basis_adjoint, threadsperblock=threadsperblock)
+
+
+def nufft_adj(kspace, coord, shape, basis=None, device="cpu", threadsperblock=128, width=4,
+              oversamp=1.25):
+    """Adjoint NUFFT (deepmr ``nufft_adj``, src/deepmr/fft/__init__.py:221-286).
+
+    Same scaling as deepmr: the exact adjoint divided by prod(os_shape).
+    """
+    plan = plan_nufft(coord, shape, width, oversamp, device)
+    return apply_nufft_adj(kspace, plan, basis, threadsperblock=threadsperblock)

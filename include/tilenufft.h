@@ -1,0 +1,153 @@
+/*
+ * tilenufft — MI355X (gfx950) Kaiser-Bessel NUFFT kernels behind a C ABI.
+ *
+ * Drop-in boundary for deepmr's NUFFT hot path. deepmr binds its kernels
+ * from Python through Numba (src/deepmr/_utils/backend.py:9-61, zero-copy
+ * __cuda_array_interface__); this library replaces those kernels and is bound
+ * through ctypes (see deep-mr_amd/tilenufft/_lib.py). Every entry point
+ * takes plain pointers and sizes, returns 0 on success or a negative
+ * TN_E_* code, and records a thread-local message readable with
+ * tn_last_error(). No exception crosses the ABI.
+ *
+ * Conventions
+ *   - Complex data are complex64 (interleaved float re, im).
+ *   - Grids are (z, y, x) row-major, x fastest; 2D grids use grid[0] == 1.
+ *   - Coordinates are grid units after deepmr's _scale_coord
+ *     (src/deepmr/fft/nufft.py:411-420), (npts, ndim) per frame, x first.
+ *   - All device buffers passed to apply calls are owned by the caller; the
+ *     apply calls allocate nothing and launch only on `stream`, so they can
+ *     be captured into a HIP graph once the plan's launch tables exist
+ *     (tn_plan_prepare).
+ *   - A plan is immutable after creation except for its launch-table cache,
+ *     which tn_plan_prepare fills; calls on one plan from several host
+ *     threads must be serialised by the caller.
+ */
+#ifndef TILENUFFT_H
+#define TILENUFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tn_plan tn_plan;
+typedef struct tn_batch tn_batch;
+typedef void *tn_stream; /* hipStream_t */
+
+enum {
+    TN_OK = 0,
+    TN_E_ARG = -1,     /* invalid argument / shape                    */
+    TN_E_HIP = -2,     /* HIP runtime error (message has the details)  */
+    TN_E_NOMEM = -3,   /* device allocation failed                     */
+    TN_E_RANGE = -4,   /* a size exceeds the supported range           */
+};
+
+/* Plan parameters. All per-axis arrays are (z, y, x); 2D uses index 0 = z
+ * with grid[0] = shape[0] = width[0] = 1. */
+typedef struct tn_plan_params {
+    int32_t ndim;       /* 2 or 3                                               */
+    int32_t nframes;    /* trajectories (deepmr's ncontrasts axis)              */
+    int64_t npts;       /* samples per frame (nviews * nsamples)                */
+    int32_t shape[3];   /* image size N                                         */
+    int32_t grid[3];    /* oversampled grid G = ceil(osf * N)                   */
+    int32_t width[3];   /* kernel width; 1 marks a Cartesian axis               */
+    float beta[3];      /* Kaiser-Bessel beta                                   */
+    int32_t shift[3];   /* output index = (tap index + shift) mod G; the fused
+                           NUFFT uses G - G/2 (folds the centring shifts), the
+                           plain interpolator 0                                 */
+    int32_t tile[3];    /* gridding tile; 0 = automatic                         */
+    int32_t interp_chunk; /* coils per interpolation work item; 0 = automatic  */
+    int32_t grid_chunk;   /* coils per gridding workgroup; 0 = automatic (LDS) */
+    float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
+    const float *apod[3]; /* host apodization vectors of length shape[d], or
+                             NULL for ones (src/deepmr/fft/nufft.py:423-447)   */
+} tn_plan_params;
+
+/* Builds a plan on the current HIP device: copies the coordinates
+ * (device pointer, nframes*npts*ndim floats), bins every sample into the
+ * gridding tiles its kernel footprint touches (stable, sorted by sample
+ * index inside each tile) and uploads the apodization vectors.
+ * Replaces plan_interpolator / _prepare_interpolator
+ * (src/deepmr/fft/_interp/plan.py:14-143, :251-269); the per-axis tap
+ * tables are not stored: taps are evaluated in registers at apply time. */
+int tn_plan_create(const tn_plan_params *params, const float *coord_dev,
+                   tn_stream stream, tn_plan **out);
+int tn_plan_destroy(tn_plan *plan);
+
+/* Device bytes held by the plan (coordinates, bins, tables). */
+int64_t tn_plan_device_bytes(const tn_plan *plan);
+/* Number of (tile, sample) entries after duplication across tile edges. */
+int64_t tn_plan_entries(const tn_plan *plan);
+/* Size of the union of the kernel footprints (grid cells touched by at
+ * least one sample), summed over frames. Used for algorithmic-byte counts. */
+int64_t tn_plan_footprint_cells(const tn_plan *plan);
+/* Effective gridding tile (z, y, x). */
+int tn_plan_tile(const tn_plan *plan, int32_t tile_out[3]);
+
+/* Builds (once) the device launch tables for `nbatch` coils so that later
+ * apply calls with that nbatch allocate nothing (needed before graph
+ * capture). Apply calls call it implicitly. */
+int tn_plan_prepare(tn_plan *plan, int64_t nbatch);
+
+/* Interpolation, grid -> samples.
+ *   grid: (nbatch, nframes, *G) complex64, natural order when shift == 0
+ *   samp: (nbatch, nframes, npts) complex64, every element written
+ *   weight: NULL or (nframes, npts) float, multiplied into the output
+ * out[b,f,p] = inv_scale * weight[f,p] * sum_taps KB * grid[b,f,taps].
+ * Replaces _do_interpolation_cuda2/3 + the /scale of apply_interpolation
+ * (src/deepmr/fft/_interp/interp.py:682-742, :120). */
+int tn_interp(tn_plan *plan, const void *grid, void *samp, int64_t nbatch,
+              const float *weight, tn_stream stream);
+
+/* Gridding, samples -> grid (the transpose of tn_interp).
+ *   samp: (nbatch, nframes, npts) complex64
+ *   grid: (nbatch, nframes, *G) complex64, every element written (zeros
+ *         where no sample contributes); tile-owned, deterministic.
+ * Replaces _do_gridding_cuda2/3 + the /scale of apply_gridding
+ * (src/deepmr/fft/_interp/grid.py:1073-1166, :125). */
+int tn_grid(tn_plan *plan, const void *samp, void *grid, int64_t nbatch,
+            const float *weight, tn_stream stream);
+
+/* Fused apodize + centred zero-pad + ifftshift: image (nimg, *N) ->
+ * grid (nimg, *G) ready for an unnormalised forward FFT. Replaces
+ * _apodize + resize + the ifftshift of fft.fft
+ * (src/deepmr/fft/nufft.py:499-505, src/deepmr/_signal/resize.py:11-102,
+ * src/deepmr/fft/fft.py:76-80). */
+int tn_apod_pad(tn_plan *plan, const void *image, void *grid, int64_t nimg,
+                tn_stream stream);
+
+/* Fused fftshift + centred crop + apodize: grid (nimg, *G) after an
+ * inverse FFT -> image (nimg, *N). Replaces fft.ifft's shifts + resize +
+ * _apodize (src/deepmr/fft/nufft.py:587-593). */
+int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg,
+                 tn_stream stream);
+
+/* ---- batched launches over several plans (ragged work queue) ----------
+ * A batch binds `njobs` (plan, nbatch, grid offset, sample offset) tuples to
+ * single launches: one interpolation launch and one gridding launch cover
+ * every frame, coil and tile of every plan in the batch. Offsets are in
+ * complex elements from the base pointers given at apply time; plan k's
+ * data is laid out as in tn_interp / tn_grid starting at its offset.
+ * No counterpart in deepmr (it launches once per call,
+ * src/deepmr/fft/_interp/interp.py:682-742). */
+int tn_batch_create(tn_plan *const *plans, const int64_t *nbatch,
+                    const int64_t *grid_offset, const int64_t *samp_offset,
+                    const int64_t *weight_offset, int32_t njobs, tn_batch **out);
+int tn_batch_destroy(tn_batch *batch);
+int tn_batch_interp(tn_batch *batch, const void *grid_base, void *samp_base,
+                    const float *weight_base, tn_stream stream);
+int tn_batch_grid(tn_batch *batch, const void *samp_base, void *grid_base,
+                  const float *weight_base, tn_stream stream);
+
+/* Thread-local description of the last error (empty string if none). */
+const char *tn_last_error(void);
+/* Library version string. */
+const char *tn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TILENUFFT_H */

@@ -1,0 +1,210 @@
+"""GPU parity of the NUFFT path against the oracle (deepmr restatement).
+
+Bar: relative L2 <= 1e-5 for complex64 against the oracle on the same seeded
+inputs; bitwise run-to-run determinism of the adjoint; dot-product test with
+deepmr's 1/prod(os_shape) adjoint scaling; error vs the FP64 NDFT within 1%
+of the oracle's own error.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import tilenufft
+from oracle import ndft
+from oracle import nufft_oracle as orc
+from tilenufft import testing
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+DEV = "cuda:0"
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def cnormal(shape, seed):
+    return testing.complex_normal(shape, seed=seed).numpy()
+
+
+def run_pair(x, coord, shape, width, osf, weight=None):
+    plan = tilenufft.fft.plan_nufft(coord, shape, width, osf, device=DEV)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan, weight=weight)
+    k = cnormal(y.shape, 99)
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan, weight=weight)
+    return plan, y.cpu().numpy(), k, xa.cpu().numpy()
+
+
+CASES_2D = [
+    # (N, coils, nviews, nsamples, osf)
+    (64, 4, 32, 128, 2.0),
+    (64, 3, 24, 96, 1.25),
+    (48, 2, 20, 100, 1.5),
+]
+
+
+@pytest.mark.parametrize("n, ncoil, nviews, nsamp, osf", CASES_2D)
+def test_nufft2d_matches_oracle(n, ncoil, nviews, nsamp, osf):
+    print(f"2D radial N={n} coils={ncoil} {nviews}x{nsamp} osf={osf}", flush=True)
+    coord = testing.golden_angle_radial(nviews, nsamp, n)
+    x = cnormal((ncoil, n, n), 1)
+    plan, y, k, xa = run_pair(x, coord, n, 4, osf)
+    p_ref = orc.plan_nufft(coord, n, 4, osf)
+    y_ref = orc.apply_nufft(x, p_ref)
+    xa_ref = orc.apply_nufft_adj(k, p_ref)
+    assert y.shape == y_ref.shape and xa.shape == xa_ref.shape
+    assert rel(y, y_ref) < RTOL
+    assert rel(xa, xa_ref) < RTOL
+
+
+def test_nufft2d_anisotropic_odd():
+    # odd and unequal axes catch x/y swaps and centring off-by-ones
+    shape = (30, 45)  # (ny, nx)
+    print("2D anisotropic shape=(30,45) osf=1.5", flush=True)
+    rng = np.random.default_rng(3)
+    coord = np.stack([rng.uniform(-22.5, 22.5, (7, 50)), rng.uniform(-15, 15, (7, 50))], -1).astype(np.float32)
+    x = cnormal((2, *shape), 2)
+    plan, y, k, xa = run_pair(x, coord, shape, 4, 1.5)
+    p_ref = orc.plan_nufft(coord, shape, 4, 1.5)
+    assert plan.os_shape == (45, 68)
+    assert rel(y, orc.apply_nufft(x, p_ref)) < RTOL
+    assert rel(xa, orc.apply_nufft_adj(k, p_ref)) < RTOL
+
+
+@pytest.mark.parametrize("osf", [1.5, 2.0])
+def test_nufft3d_matches_oracle(osf):
+    n = 16
+    print(f"3D kooshball N={n} osf={osf}", flush=True)
+    coord = testing.kooshball(60, 32, n)
+    x = cnormal((2, n, n, n), 4)
+    plan, y, k, xa = run_pair(x, coord, n, 4, osf)
+    p_ref = orc.plan_nufft(coord, n, 4, osf)
+    assert rel(y, orc.apply_nufft(x, p_ref)) < RTOL
+    assert rel(xa, orc.apply_nufft_adj(k, p_ref)) < RTOL
+
+
+@pytest.mark.parametrize("ndim, n, tile", [(2, 20, (14, 14)), (2, 20, (5, 3)), (3, 12, None),
+                                           (3, 12, (7, 5, 16))])
+def test_narrow_edge_tiles(ndim, n, tile):
+    # grids whose last tile is narrower than the kernel: footprints wrap back
+    # into a tile they already touched (regression: such samples were binned
+    # twice into that tile)
+    print(f"narrow edge tiles ndim={ndim} N={n} tile={tile}", flush=True)
+    coord = testing.golden_angle_radial(16, 40, n) if ndim == 2 else testing.kooshball(40, 24, n)
+    x = cnormal((2,) + (n,) * ndim, 21)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 1.5, device=DEV, tile=tile)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    k = cnormal(y.shape, 22)
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    p_ref = orc.plan_nufft(coord, n, 4, 1.5)
+    assert rel(y, orc.apply_nufft(x, p_ref)) < RTOL
+    assert rel(xa, orc.apply_nufft_adj(k, p_ref)) < RTOL
+
+
+def test_multiframe_and_weight():
+    # 3 frames (contrasts) with their own trajectories, batch (2 slices, 3 coils)
+    n, nf = 32, 3
+    print("2D multiframe nf=3 batch=(2,3) with weight", flush=True)
+    coord = np.stack([testing.golden_angle_radial(10, 64, n, offset=0.3 * f) for f in range(nf)])
+    x = cnormal((2, 3, nf, n, n), 5)
+    w = np.random.default_rng(6).uniform(0.5, 1.5, coord.shape[:-1]).astype(np.float32)
+    plan, y, k, xa = run_pair(x, coord, n, 4, 2.0, weight=torch.from_numpy(w))
+    p_ref = orc.plan_nufft(coord, n, 4, 2.0)
+    assert y.shape == (2, 3, nf, 10, 64)
+    assert rel(y, orc.apply_nufft(x, p_ref, weight=w)) < RTOL
+    assert rel(xa, orc.apply_nufft_adj(k, p_ref, weight=w)) < RTOL
+
+
+def test_dot_product_2d_and_3d():
+    for ndim, n in ((2, 40), (3, 12)):
+        print(f"dot test ndim={ndim} N={n}", flush=True)
+        coord = testing.golden_angle_radial(12, 80, n) if ndim == 2 else testing.kooshball(40, 24, n)
+        x = cnormal((2,) + (n,) * ndim, 7)
+        plan, y, k, xa = run_pair(x, coord, n, 4, 1.5)
+        lhs = np.vdot(y.astype(np.complex128), k.astype(np.complex128))
+        rhs = np.prod(plan.os_shape) * np.vdot(x.astype(np.complex128), xa.astype(np.complex128))
+        assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_adjoint_bitwise_deterministic():
+    n = 64
+    print("determinism 2D N=64 8 coils dense radial", flush=True)
+    coord = testing.golden_angle_radial(64, 256, n)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV)
+    k = torch.from_numpy(cnormal((8, 64, 256), 8)).to(DEV)
+    a = tilenufft.fft.apply_nufft_adj(k, plan)
+    b = tilenufft.fft.apply_nufft_adj(k, plan)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_ndft_error_matches_oracle(ndim):
+    n = 24 if ndim == 2 else 10
+    print(f"NDFT error ratio ndim={ndim}", flush=True)
+    coord = testing.golden_angle_radial(20, 48, n) if ndim == 2 else testing.kooshball(30, 20, n)
+    x = cnormal((n,) * ndim, 9)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    y_ref = orc.nufft(x, coord, n, 4, 2.0)
+    exact = ndft.ndft(x, coord, (n,) * ndim).reshape(y.shape)
+    e_gpu, e_ref = rel(y, exact), rel(y_ref, exact)
+    assert abs(e_gpu / e_ref - 1) < 0.01
+    # adjoint
+    k = cnormal(y.shape, 10)
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    xa_ref = orc.nufft_adj(k, coord, n, 4, 2.0)
+    exact_a = ndft.ndft_adjoint(k.reshape(-1), coord, (n,) * ndim) / np.prod(plan.os_shape)
+    assert abs(rel(xa, exact_a) / rel(xa_ref, exact_a) - 1) < 0.01
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("ncontrasts", [1, 2])
+def test_cartesian_known_answers(ndim, ncontrasts):
+    """Restates deepmr's tests/fft/test_nufft.py: integer coordinates take the
+    width-1 path; a centred delta maps to all ones and back."""
+    npix, ncoils = 4, 2
+    print(f"Cartesian known answer ndim={ndim} ncontrasts={ncontrasts}", flush=True)
+    nodes = np.arange(npix) - npix // 2
+    mesh = np.meshgrid(*([nodes] * ndim), indexing="ij")
+    coord = np.stack([m.ravel() for m in mesh], -1).astype(np.float32)[None]
+    if ncontrasts > 1:
+        coord = np.repeat(coord[None], ncontrasts, 0)
+    lead = (ncoils,) if ncontrasts == 1 else (ncoils, ncontrasts)
+    img = torch.zeros((*lead, *(npix,) * ndim), dtype=torch.complex64)
+    img[(Ellipsis,) + (npix // 2,) * ndim] = 1.0
+    k = tilenufft.fft.nufft(img, coord=torch.from_numpy(coord), device=DEV, width=12)
+    assert k.device.type == "cpu"
+    np.testing.assert_allclose(k.numpy(), np.ones((*lead, 1, npix**ndim)), rtol=1e-4, atol=1e-4)
+    back = tilenufft.fft.nufft_adj(torch.ones((*lead, 1, npix**ndim), dtype=torch.complex64),
+                                   coord=torch.from_numpy(coord), shape=[npix] * ndim, device=DEV,
+                                   width=12)
+    np.testing.assert_allclose(back.numpy(), img.numpy(), rtol=1e-4, atol=1e-4)
+
+
+def test_plain_interpolator_matches_oracle():
+    n = 40
+    print("plain interpolator/gridding 2D", flush=True)
+    rng = np.random.default_rng(11)
+    coord = rng.uniform(0, n, (2, 5, 30, 2)).astype(np.float32)  # 2 frames, grid units
+    interp = tilenufft._interp.plan_interpolator(coord, n, width=3, beta=2.5, device=DEV)
+    oi = orc.plan_interpolator(coord, [n, n], [3, 3], [np.float32(2.5)] * 2)
+    g = cnormal((3, 2, n, n), 12)
+    s = tilenufft._interp.apply_interpolation(torch.from_numpy(g).to(DEV), interp).cpu().numpy()
+    assert rel(s, orc.apply_interpolation(g, oi)) < RTOL
+    ks = cnormal((3, 2, 5, 30), 13)
+    gg = tilenufft._interp.apply_gridding(torch.from_numpy(ks).to(DEV), interp).cpu().numpy()
+    assert rel(gg, orc.apply_gridding(ks, oi)) < RTOL
+
+
+def test_numpy_in_numpy_out_and_real_input():
+    n = 16
+    coord = testing.golden_angle_radial(8, 32, n)
+    x = np.random.default_rng(0).standard_normal((n, n)).astype(np.float32)
+    y = tilenufft.fft.nufft(x, coord, device=DEV, oversamp=2.0)
+    assert isinstance(y, np.ndarray) and y.dtype == np.complex64
+    assert rel(y, orc.nufft(x, coord, n, 4, 2.0)) < RTOL

@@ -1,0 +1,148 @@
+"""Pins for the oracle's full transforms (CPU only).
+
+- resize: the reference's known answers (tests/signal/test_resize.py:9-91).
+- Cartesian width-1 path and FFT conventions: the reference's known answers
+  (tests/fft/test_nufft.py, centred delta <-> all ones).
+- Kaiser-Bessel path: error against the FP64 direct NDFT (the closed form the
+  NUFFT approximates), on anisotropic shapes so swapped axes fail, and the
+  dot-product identity <A x, y> = prod(os_shape) <x, nufft_adj(y)>.
+"""
+
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import ndft
+from oracle import nufft_oracle as orc
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def cn(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+# --- resize known answers (tests/signal/test_resize.py) ---------------------
+def test_resize_known_answers():
+    x = np.array([[0, 0, 0], [0, 1, 0], [0, 0, 0]])
+    np.testing.assert_array_equal(orc.resize(x, [5, 5]), np.pad(x, 1))
+    np.testing.assert_array_equal(orc.resize(x, [4, 4]),
+                                  [[0, 0, 0, 0], [0, 0, 0, 0], [0, 0, 1, 0], [0, 0, 0, 0]])
+    np.testing.assert_array_equal(orc.resize(x[None], [5, 5])[0], np.pad(x, 1))
+    y = np.pad(x, 1)
+    np.testing.assert_array_equal(orc.resize(y, [3, 3]), x)
+    z = np.array([[0, 0, 0, 0], [0, 0, 0, 0], [0, 0, 1, 0], [0, 0, 0, 0]])
+    np.testing.assert_array_equal(orc.resize(z, [3, 3]), x)
+    np.testing.assert_array_equal(orc.resize(z[None], [3, 3])[0], x)
+
+
+# --- Cartesian known answers (tests/fft/test_nufft.py) ------------------------
+def _cart_coord(ndim, ncontrasts, npix):
+    nodes = np.arange(npix) - npix // 2
+    mesh = np.meshgrid(*([nodes] * ndim), indexing="ij")
+    coord = np.stack([m.ravel() for m in mesh], -1).astype(np.float32)[None]
+    if ncontrasts > 1:
+        coord = np.repeat(coord[None], ncontrasts, 0)
+    return coord
+
+
+@pytest.mark.parametrize("ndim, ncontrasts, ncoils, nslices",
+                         list(itertools.product([2, 3], [1, 2], [1, 2], [1, 2])))
+def test_cartesian_delta_ones(ndim, ncontrasts, ncoils, nslices):
+    npix = 4
+    coord = _cart_coord(ndim, ncontrasts, npix)
+    lead = (nslices, ncoils) if ndim == 2 else (ncoils,)
+    if ncontrasts > 1:
+        lead = lead + (ncontrasts,)
+    img = np.zeros((*lead, *(npix,) * ndim), np.complex64)
+    img[(Ellipsis,) + (npix // 2,) * ndim] = 1
+    k = orc.nufft(img, coord, None, width=12)
+    np.testing.assert_allclose(k, np.ones((*lead, 1, npix**ndim)), rtol=1e-4, atol=1e-4)
+    back = orc.nufft_adj(np.ones((*lead, 1, npix**ndim), np.complex64), coord, [npix] * ndim, width=12)
+    np.testing.assert_allclose(back, img, rtol=1e-4, atol=1e-4)
+
+
+# --- Kaiser-Bessel path vs the FP64 NDFT ---------------------------------------
+@pytest.mark.parametrize("osf, bound", [(2.0, 2e-3), (1.5, 8e-3), (1.25, 2e-2)])
+def test_nufft2d_vs_ndft_anisotropic(osf, bound):
+    rng = np.random.default_rng(1)
+    shape = (20, 28)  # (ny, nx)
+    coord = np.stack([rng.uniform(-14, 14, 300), rng.uniform(-10, 10, 300)], -1).astype(np.float32)
+    x = cn(rng, shape)
+    y = orc.nufft(x, coord, shape, 4, osf)
+    exact = ndft.ndft(x, coord, shape).reshape(y.shape)
+    assert rel(y, exact) < bound
+    k = cn(rng, y.shape)
+    plan = orc.plan_nufft(coord, shape, 4, osf)
+    xa = orc.apply_nufft_adj(k, plan)
+    exact_a = ndft.ndft_adjoint(k.reshape(-1), coord, shape) / np.prod(plan["os_shape"])
+    assert rel(xa, exact_a) < bound
+    # swapping x/y in the coordinates must break the match
+    swapped = orc.nufft(x, coord[:, ::-1].copy(), shape, 4, osf)
+    assert rel(swapped, exact) > 0.1
+
+
+@pytest.mark.parametrize("osf, bound", [(2.0, 3e-3), (1.5, 1e-2)])
+def test_nufft3d_vs_ndft(osf, bound):
+    rng = np.random.default_rng(2)
+    shape = (8, 10, 12)
+    coord = np.stack([rng.uniform(-6, 6, 200), rng.uniform(-5, 5, 200), rng.uniform(-4, 4, 200)],
+                     -1).astype(np.float32)
+    x = cn(rng, shape)
+    y = orc.nufft(x, coord, shape, 4, osf)
+    assert rel(y, ndft.ndft(x, coord, shape).reshape(y.shape)) < bound
+
+
+def test_dot_product_identity():
+    rng = np.random.default_rng(3)
+    for ndim, shape in ((2, (18, 22)), (3, (8, 8, 10))):
+        coord = rng.uniform(-4, 4, (2, 50, ndim)).astype(np.float32)  # 2 frames
+        plan = orc.plan_nufft(coord, shape, 4, 1.5)
+        x = cn(rng, (3, 2, *shape))
+        y = orc.apply_nufft(x, plan)
+        k = cn(rng, y.shape)
+        xa = orc.apply_nufft_adj(k, plan)
+        lhs = np.vdot(y.astype(np.complex128), k)
+        rhs = np.prod(plan["os_shape"]) * np.vdot(x.astype(np.complex128), xa)
+        assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_interp_grid_are_transposes():
+    rng = np.random.default_rng(4)
+    coord = rng.uniform(0, 30, (3, 1, 40, 2)).astype(np.float32)
+    interp = orc.plan_interpolator(coord, [30, 30], [4, 4], [np.float32(6.0)] * 2)
+    g = cn(rng, (3, 2, 30, 30))
+    s = cn(rng, (3, 2, 40))
+    lhs = np.vdot(orc.interpolate_raw(g, interp).astype(np.complex128), s)
+    rhs = np.vdot(g.astype(np.complex128), orc.grid_raw(s, interp))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-6
+
+
+def test_weight_applies_after_interp_and_before_grid():
+    rng = np.random.default_rng(5)
+    coord = rng.uniform(-8, 8, (30, 2)).astype(np.float32)
+    plan = orc.plan_nufft(coord, 16, 4, 2.0)
+    x = cn(rng, (16, 16))
+    w = rng.uniform(0.2, 2.0, 30).astype(np.float32)
+    np.testing.assert_allclose(orc.apply_nufft(x, plan, weight=w), w * orc.apply_nufft(x, plan),
+                               rtol=1e-6)
+    k = cn(rng, (1, 30))
+    np.testing.assert_allclose(orc.apply_nufft_adj(k, plan, weight=w),
+                               orc.apply_nufft_adj(w * k, plan), rtol=1e-6, atol=1e-7)
+
+
+def test_lowrank_with_identity_basis_equals_plain():
+    rng = np.random.default_rng(6)
+    nf = 3
+    coord = rng.uniform(-6, 6, (nf, 1, 25, 2)).astype(np.float32)
+    plan = orc.plan_nufft(coord, 12, 4, 2.0)
+    x = cn(rng, (2, nf, 12, 12))
+    eye = np.eye(nf, dtype=np.complex64)
+    np.testing.assert_allclose(orc.apply_nufft(x, plan, basis_adjoint=eye), orc.apply_nufft(x, plan),
+                               rtol=1e-5, atol=1e-6)
+    k = cn(rng, (2, nf, 1, 25))
+    np.testing.assert_allclose(orc.apply_nufft_adj(k, plan, basis=eye), orc.apply_nufft_adj(k, plan),
+                               rtol=1e-5, atol=1e-6)
