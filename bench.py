@@ -1,0 +1,247 @@
+"""NUFFT benchmark on one MI355X.
+
+Workload (default): batched 2D multicoil spiral — 8 frames x 16 coils, 256x256
+complex64 images, one variable-density spiral per frame with 24..48
+interleaves x 2048 samples (uneven sample counts, golden-angle rotation per
+frame), Gaussian-bump coil maps, oversampling 2.0, Kaiser-Bessel width 4.
+Tile / chunk sizes come from the autotuner; the forward+adjoint pair of all
+frames and coils is one HIP graph. One step = forward + adjoint of every
+sample of every frame and coil, inputs resident in HBM.
+
+Prints one JSON line with samples/s, the roofline of the dominant kernel
+(interpolation or gridding; achieved = algorithmic bytes / average kernel
+time from HIP events on the launch stream) and a CPU baseline (the oracle's
+C/OpenMP restatement of deepmr's Numba CPU loops) on a bounded sample.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "deep-mr_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "NUFFT samples/s fwd+adj, % HBM roofline; served recon p50/p99 vs plain"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_spiral_workload(device, nframes=8, ncoils=16, n=256, nsamp=2048, osf=2.0, width=4, seed=0):
+    import tilenufft
+    from tilenufft import testing
+
+    rng = np.random.default_rng(seed)
+    inter = rng.integers(24, 49, size=nframes)
+    coords = [testing.vd_spiral(int(inter[f]), nsamp, n, rotation=f * testing.GOLDEN_ANGLE)
+              for f in range(nframes)]
+    plans = [tilenufft.fft.plan_nufft(c, n, width, osf, device=device) for c in coords]
+    smaps = torch.from_numpy(testing.gaussian_coil_maps(ncoils, (n, n), seed=seed)).to(device)
+    gen = torch.Generator().manual_seed(seed)
+    frames = testing.complex_normal((nframes, n, n), generator=gen).to(device)
+    return dict(coords=coords, plans=plans, smaps=smaps, frames=frames, interleaves=inter.tolist(),
+                n=n, ncoils=ncoils, osf=osf, width=width, nsamp=nsamp)
+
+
+def time_kernel(fn, reps, device):
+    """Average duration of fn() from HIP events recorded on the launch stream."""
+    stream = torch.cuda.current_stream(device)
+    fn()
+    torch.cuda.synchronize(device)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for s, e in ev:
+        s.record(stream)
+        fn()
+        e.record(stream)
+    torch.cuda.synchronize(device)
+    return float(np.mean([s.elapsed_time(e) for s, e in ev]))
+
+
+def cpu_baseline(wl, batch, frame=0, reps=5, warm=1, budget_s=30.0):
+    """Oracle (C/OpenMP restatement of deepmr's CPU loops) on one frame of the
+    workload: forward + adjoint of all coils; returns samples/s and the
+    relative L2 of the GPU result for that frame against it."""
+    from oracle import nufft_oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lib = orc.load_loops(fast=True)
+    lib.orc_set_threads(threads)
+    torch.set_num_threads(threads)
+    orc.use_fast_loops(True)
+    orc.set_fft_backend("torch")
+    try:
+        n, ncoils = wl["n"], wl["ncoils"]
+        coord = wl["coords"][frame]
+        t0 = time.perf_counter()
+        plan = orc.plan_nufft(coord, n, wl["width"], wl["osf"])
+        t_plan = time.perf_counter() - t0
+        x = (wl["smaps"] * wl["frames"][frame]).cpu().numpy()
+        times = []
+        y = xa = None
+        for r in range(warm + reps):
+            t0 = time.perf_counter()
+            y = orc.apply_nufft(x, plan)
+            xa = orc.apply_nufft_adj(y, plan)
+            dt = time.perf_counter() - t0
+            if r >= warm:
+                times.append(dt)
+            if sum(times) > budget_s:
+                break
+        med = float(np.median(times))
+        nsamples = ncoils * coord.shape[0] * coord.shape[1]
+        gpu_y = batch.kspace_view(frame).reshape(ncoils, -1).cpu().numpy()
+        err = float(np.linalg.norm(gpu_y - y.reshape(ncoils, -1)) / np.linalg.norm(y))
+        return dict(value=nsamples / med, unit="samples/s", cores=threads, kind="port",
+                    sample=f"frame {frame} of the workload ({ncoils} coils x {coord.shape[0]}x"
+                           f"{coord.shape[1]} samples) forward+adjoint, median of {len(times)} after "
+                           f"{warm} warm-up, plan {t_plan:.2f} s excluded",
+                    ms_per_step=med * 1e3), err
+    finally:
+        orc.use_fast_loops(False)
+        orc.set_fft_backend("numpy")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-graph", action="store_true", help="replay eagerly instead of a HIP graph")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the NUFFT path "
+              "shards by frame/coil/request without communication, see DESIGN.md", flush=True)
+        sys.exit(2)
+
+    import tilenufft
+    from tilenufft.batch import BatchedNUFFT
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("bench: building the spiral workload (8 frames x 16 coils, 256^2, osf 2)")
+    wl = make_spiral_workload(dev)
+    plans, ncoils = wl["plans"], wl["ncoils"]
+    tune = None
+    if not args.no_autotune:
+        log("bench: autotuning tile / chunk sizes")
+        tune = tilenufft.autotune.tune_batch(plans, ncoils)
+        tune = {k: v for k, v in tune.items() if k != "trials"}
+        log(f"bench: autotune -> {tune}")
+    batch = BatchedNUFFT(plans, ncoils)
+    for f in range(len(plans)):
+        batch.image_view(f)[:, 0].copy_(wl["smaps"] * wl["frames"][f])
+
+    step = (lambda: (batch.forward(), batch.adjoint()))
+    graph = None
+    if not args.no_graph:
+        log("bench: capturing forward+adjoint into a HIP graph")
+        graph = batch.capture()
+        run = graph.replay
+    else:
+        run = step
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize(dev)
+    log(f"bench: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize(dev)
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed / args.steps * 1e3
+
+    # per-kernel durations on the launch stream (the forward output in the
+    # k-space arena is kept for the parity check against the CPU baseline)
+    kspace_saved = batch.kspace.clone()
+    kgrid = torch.fft.fftn(batch.grid, dim=(-2, -1))
+    t_interp = time_kernel(lambda: batch.interp(kgrid), args.kernel_reps, dev)
+    batch.kspace.copy_(kspace_saved)
+    gout = torch.empty_like(batch.grid)
+    t_grid = time_kernel(lambda: batch.gridding(gout), args.kernel_reps, dev)
+    t_copy = time_kernel(lambda: batch.image_adj.copy_(batch.image), args.kernel_reps, dev)
+    batch.kspace.copy_(kspace_saved)
+    b_interp, b_grid = batch.algorithmic_bytes()
+    gbs_interp = b_interp / (t_interp * 1e-3) / 1e9
+    gbs_grid = b_grid / (t_grid * 1e-3) / 1e9
+    dominant = "grid" if t_grid >= t_interp else "interp"
+    ach = gbs_grid if dominant == "grid" else gbs_interp
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_spiral2d.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dominant)
+
+    total_samples = batch.nsamples
+    value = total_samples / (ms_step * 1e-3)
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle restatement of deepmr's CPU loops) on frame 0")
+        cpu, parity = cpu_baseline(wl, batch)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched 2D multicoil VD spiral, 8 frames x 16 coils, 256x256, osf 2.0, "
+                        "KB width 4, 24-48 interleaves x 2048 samples per frame, fwd+adj of all "
+                        "frames/coils per step" + (" in one HIP graph" if graph is not None else ""),
+            "frames": len(plans),
+            "coils": ncoils,
+            "interleaves": wl["interleaves"],
+            "samples_per_step": total_samples,
+            "autotune": tune,
+            "hip_graph": graph is not None,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": ach,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "kernels": {
+            "interp_ms": t_interp,
+            "grid_ms": t_grid,
+            "interp_alg_bytes": b_interp,
+            "grid_alg_bytes": b_grid,
+            "interp_GBs": gbs_interp,
+            "grid_GBs": gbs_grid,
+            "interp_frac": gbs_interp / HBM_PEAK_GBS,
+            "grid_frac": gbs_grid / HBM_PEAK_GBS,
+            "image_copy_ms": t_copy,
+        },
+        "cpu_baseline": cpu,
+        "parity_rel_l2_frame0_vs_oracle": parity,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -8,7 +8,7 @@ Modules mirror deepmr's layout for the NUFFT path:
 The compute path is the native library libtilenufft.so (HIP, gfx950).
 """
 
-from . import _interp, _signal, fft, linops, optim  # noqa: F401
+from . import _interp, _signal, autotune, batch, fft, linops, optim, recon, serve, testing  # noqa: F401
 from ._lib import version  # noqa: F401
 from ._signal import resize  # noqa: F401
 

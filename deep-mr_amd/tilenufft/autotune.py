@@ -1,0 +1,129 @@
+"""Autotuner for the launch parameters of the gridding / interpolation kernels.
+
+Tunable per plan: the gridding tile (bin size), the coils per gridding
+workgroup and the coils per interpolation work item. Candidates are timed on
+the actual batch with HIP events; the choice is cached on disk keyed by the
+geometry (ndim, N, G, W, coils, samples per trajectory rounded) and the
+device name.
+"""
+
+__all__ = ["tune_batch", "default_candidates"]
+
+import json
+import os
+
+import numpy as np
+import torch
+
+from .batch import BatchedNUFFT
+from .fft.nufft import retune
+
+
+def default_candidates(ndim):
+    if ndim == 2:
+        tiles = [(16, 16), (32, 32), (32, 64), (64, 32), (64, 64), (16, 64)]
+    else:
+        tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
+    return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16])
+
+
+def _cache_path():
+    root = os.environ.get("TILENUFFT_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "tilenufft"))
+    return os.path.join(root, "autotune.json")
+
+
+def _key(plans, ncoils):
+    p = plans[0]
+    m = int(np.median([q.npts * q.nframes for q in plans]))
+    bucket = int(2 ** round(np.log2(max(m, 1))))
+    dev = torch.cuda.get_device_name(p.interpolator.device)
+    return f"{dev}|d{p.ndim}|N{p.shape}|G{p.os_shape}|W{p.width}|C{max(ncoils)}|M{bucket}|P{len(plans)}"
+
+
+def _time(fn, reps, device):
+    torch.cuda.synchronize(device)
+    fn()
+    start = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    end = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    for s, e in zip(start, end):
+        s.record()
+        fn()
+        e.record()
+    torch.cuda.synchronize(device)
+    return float(np.median([s.elapsed_time(e) for s, e in zip(start, end)]))
+
+
+def _apply(plans, tile, ic, gc):
+    for p in plans:
+        retune(p, tile=tile, interp_chunk=ic, grid_chunk=gc)
+
+
+def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=False):
+    """Pick tile / chunk sizes for ``plans`` (rebuilt in place) and return them.
+
+    Returns dict(tile, grid_chunk, interp_chunk, grid_ms, interp_ms).
+    """
+    ncoils = [int(ncoils)] * len(plans) if np.isscalar(ncoils) else list(ncoils)
+    key = _key(plans, ncoils)
+    path = _cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            with open(path) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+    if use_cache and key in cache:
+        best = cache[key]
+        _apply(plans, tuple(best["tile"]), best["interp_chunk"], best["grid_chunk"])
+        return dict(best, cached=True)
+
+    dev = plans[0].interpolator.device
+    cand = candidates or default_candidates(plans[0].ndim)
+    results = []
+    with torch.cuda.device(dev):
+        # 1) tile, automatic chunks
+        best_tile, best_t = None, float("inf")
+        for tile in cand["tiles"]:
+            _apply(plans, tile, 0, 0)
+            b = BatchedNUFFT(plans, ncoils)
+            t = _time(lambda: b.gridding(b.grid), reps, dev)
+            results.append(("tile", tile, t))
+            if verbose:
+                print(f"autotune tile {tile}: grid {t:.3f} ms", flush=True)
+            if t < best_t:
+                best_tile, best_t = tile, t
+        # 2) coils per gridding workgroup
+        best_gc, best_gt = 0, best_t
+        for gc in cand["grid_chunks"]:
+            if gc == 0:
+                continue
+            _apply(plans, best_tile, 0, gc)
+            b = BatchedNUFFT(plans, ncoils)
+            t = _time(lambda: b.gridding(b.grid), reps, dev)
+            results.append(("grid_chunk", gc, t))
+            if t < best_gt:
+                best_gc, best_gt = gc, t
+        # 3) coils per interpolation item
+        best_ic, best_it = 0, float("inf")
+        for ic in [0] + list(cand["interp_chunks"]):
+            _apply(plans, best_tile, ic, best_gc)
+            b = BatchedNUFFT(plans, ncoils)
+            t = _time(lambda: b.interp(b.grid), reps, dev)
+            results.append(("interp_chunk", ic, t))
+            if verbose:
+                print(f"autotune interp_chunk {ic}: interp {t:.3f} ms", flush=True)
+            if t < best_it:
+                best_ic, best_it = ic, t
+        _apply(plans, best_tile, best_ic, best_gc)
+    best = dict(tile=list(best_tile), grid_chunk=best_gc, interp_chunk=best_ic, grid_ms=best_gt,
+                interp_ms=best_it)
+    if use_cache:
+        cache[key] = best
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(path, "w") as f:
+                json.dump(cache, f, indent=1)
+        except OSError:
+            pass
+    return dict(best, cached=False, trials=results)

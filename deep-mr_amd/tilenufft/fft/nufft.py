@@ -22,6 +22,7 @@ __all__ = [
     "ApplyNUFFTAdjoint",
     "set_autograd_mode",
     "get_autograd_mode",
+    "retune",
 ]
 
 import math
@@ -59,6 +60,8 @@ class NUFFTPlan:
     scale: float = 1.0
     coord: np.ndarray = field(default=None, repr=False)  # scaled, (nframes, npts, ndim)
     tile: tuple = None
+    interp_chunk: int = 0
+    grid_chunk: int = 0
 
     def to(self, device):
         """Move the device plan to another HIP device ("cpu" keeps it where it is)."""
@@ -73,7 +76,7 @@ class NUFFTPlan:
         return int(np.prod(self.ishape))
 
 
-def _build_native(plan, device, shift, tile=None, interp_chunk=0, grid_chunk=0):
+def _build_native(plan, device, shift):
     ndim = plan.ndim
     sh = [g - g // 2 for g in plan.os_shape] if shift else [0] * ndim
     apod = [
@@ -82,11 +85,23 @@ def _build_native(plan, device, shift, tile=None, interp_chunk=0, grid_chunk=0):
         else None
         for d in range(ndim)
     ]
-    tile = tile if tile is not None else plan.tile
     return _core.NativePlan(
         plan.coord, plan.shape, plan.os_shape, plan.width, plan.beta, 1.0 / plan.scale, sh,
-        device, apod=apod, tile=tile, interp_chunk=interp_chunk, grid_chunk=grid_chunk,
+        device, apod=apod, tile=plan.tile, interp_chunk=plan.interp_chunk,
+        grid_chunk=plan.grid_chunk,
     )
+
+
+def retune(plan, tile=None, interp_chunk=None, grid_chunk=None):
+    """Rebuild the device plan of ``plan`` with other launch parameters."""
+    if tile is not None:
+        plan.tile = tuple(int(t) for t in tile)
+    if interp_chunk is not None:
+        plan.interp_chunk = int(interp_chunk)
+    if grid_chunk is not None:
+        plan.grid_chunk = int(grid_chunk)
+    plan.interpolator = _build_native(plan, plan.interpolator.device, shift=True)
+    return plan
 
 
 def _expand_coord(coord):
@@ -99,13 +114,15 @@ def _expand_coord(coord):
     return coord.reshape(nframes, int(np.prod(ishape)), coord.shape[-1]), nframes, ishape
 
 
-def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None):
+def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, interp_chunk=0,
+               grid_chunk=0):
     """Precompute a NUFFT (deepmr ``plan_nufft``, src/deepmr/fft/nufft.py:27-132).
 
     Parameters follow deepmr: ``coord`` (ncontrasts, nviews, nsamples, ndim)
     in [-N/2, N/2) with x first; ``shape``, ``width``, ``oversamp`` scalar or
     (z, y, x). ``device`` selects the HIP device ("cpu" = current HIP device).
-    ``tile`` optionally overrides the gridding tile (z, y, x).
+    ``tile`` (z, y, x), ``interp_chunk`` and ``grid_chunk`` (coils per work
+    item) optionally override the launch parameters (0 / None = automatic).
     """
     coord = torch.as_tensor(coord).detach().cpu().to(torch.float32).numpy().copy()
     ndim = coord.shape[-1]
@@ -145,6 +162,8 @@ def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None):
         scale=scale,
         coord=flat,
         tile=None if tile is None else tuple(int(t) for t in tile),
+        interp_chunk=int(interp_chunk),
+        grid_chunk=int(grid_chunk),
     )
     plan.interpolator = _build_native(plan, dev, shift=True)
     return plan

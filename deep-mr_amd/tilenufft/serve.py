@@ -1,0 +1,496 @@
+"""Continuous-batching CG-SENSE reconstruction server on one GPU.
+
+Requests (2D non-Cartesian multicoil k-space, coil maps, trajectory, number
+of CG iterations) arrive over time. The server advances every active request
+by one CG-SENSE iteration per step, and all active requests share the
+launches of that step:
+
+  coil expansion -> per-geometry apod+pad -> per-geometry batched rocFFT ->
+  ONE interpolation launch over every active request (ragged work queue) ->
+  ONE gridding launch over every active request -> per-geometry batched
+  inverse rocFFT -> per-geometry crop+apod -> coil combination -> CG update
+
+Requests are admitted at step boundaries while a slot is free and the HBM
+budget covers their resident state (plan / sorted trajectory, coil maps,
+k-space, CG vectors and their share of the step arenas); finished requests
+leave and free their budget, so new arrivals join while others are mid-solve.
+Plans are kept resident and reused for repeated trajectories (``traj_key``).
+
+The CG recursion is deepmr's CGStep (src/deepmr/optim/cg.py:194-212) on
+E^H E x = E^H y with E = NUFFT * SENSE (src/deepmr/recon/alg/linop.py:108-148),
+without density compensation.
+
+The per-request interpolation and gridding results do not depend on which
+other requests share a launch (each work item is one request's sample block
+or tile); only the batched FFT may pick another algorithm, so a result alone
+and in a mix agree to float32 round-off.
+"""
+
+__all__ = ["ReconRequest", "ReconServer", "run_plain", "estimate_request_bytes", "latency_stats",
+           "make_stream"]
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _core, fft as _fft
+from .batch import BatchedNUFFT
+
+
+@dataclass
+class ReconRequest:
+    """One 2D CG-SENSE reconstruction.
+
+    coord : (nviews, nsamples, 2) float32 in [-N/2, N/2)
+    kspace: (ncoils, nviews, nsamples) complex64
+    smaps : (ncoils, N, N) complex64
+    """
+
+    rid: int
+    coord: np.ndarray
+    kspace: torch.Tensor
+    smaps: torch.Tensor
+    shape: int
+    niter: int = 10
+    arrival: float = 0.0
+    traj_key: object = None
+    # filled by the server
+    result: torch.Tensor = None
+    error: str = None
+    t_admit: float = None
+    t_done: float = None
+    cancelled: bool = False
+
+    @property
+    def ncoils(self):
+        return int(self.kspace.shape[0])
+
+    @property
+    def npts(self):
+        return int(np.prod(self.coord.shape[:-1]))
+
+    def cancel(self):
+        self.cancelled = True
+
+
+def estimate_request_bytes(n, ncoils, npts, oversamp=1.25, dup=1.35):
+    """Device bytes a request holds while active."""
+    g = math.ceil(np.float32(oversamp) * n)
+    plan = 8 * npts + int(16 * dup * npts) + 4 * 512
+    state = 8 * (ncoils * n * n  # coil maps
+                 + ncoils * npts  # measured k-space
+                 + 5 * n * n)  # x, r, p, AHAp, b
+    arenas = 8 * (ncoils * n * n + 2 * ncoils * g * g + ncoils * npts)
+    return plan + state + arenas
+
+
+class _Active:
+    """Device-resident state of one admitted request."""
+
+    def __init__(self, req, plan, device):
+        self.req = req
+        self.plan = plan
+        self.smaps = req.smaps.to(device, torch.complex64).contiguous()
+        self.y = req.kspace.to(device, torch.complex64).reshape(req.ncoils, -1).contiguous()
+        n = req.shape
+        self.x = torch.zeros((n, n), dtype=torch.complex64, device=device)
+        self.r = None
+        self.p = None
+        self.rs_old = None
+        self.it = -1  # -1: E^H y pending, then 0..niter-1
+        self.bytes = 0
+
+
+class ReconServer:
+    """Continuous batching of CG-SENSE requests on one HIP device."""
+
+    def __init__(self, device=None, hbm_budget=8 << 30, max_active=32, width=4, oversamp=1.25,
+                 plan_cache=64):
+        self.device = _core.compute_device(device)
+        self.budget = int(hbm_budget)
+        self.max_active = int(max_active)
+        self.width = width
+        self.oversamp = oversamp
+        self.queue = collections.deque()
+        self.active = []
+        self.done = []
+        self.reserved = 0
+        self._plans = collections.OrderedDict()  # traj_key -> NUFFTPlan (resident)
+        self._plan_cache = plan_cache
+        self._layout_key = None
+        self._layout = None
+        self.steps = 0
+        self.t0 = time.perf_counter()
+
+    # ---- admission -----------------------------------------------------------
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def submit(self, req):
+        self.queue.append(req)
+
+    def _plan_for(self, req):
+        key = req.traj_key
+        if key is not None and key in self._plans:
+            self._plans.move_to_end(key)
+            return self._plans[key]
+        plan = _fft.plan_nufft(req.coord, req.shape, self.width, self.oversamp, device=self.device)
+        if plan.nframes != 1 or plan.ndim != 2:
+            raise ValueError("server requests are single-frame 2D")
+        if key is not None:
+            self._plans[key] = plan
+            while len(self._plans) > self._plan_cache:
+                self._plans.popitem(last=False)
+        return plan
+
+    def _validate(self, req):
+        if req.coord.ndim < 2 or req.coord.shape[-1] != 2:
+            raise ValueError("coord must be (..., 2)")
+        if tuple(req.smaps.shape) != (req.ncoils, req.shape, req.shape):
+            raise ValueError(f"smaps shape {tuple(req.smaps.shape)} != ({req.ncoils}, {req.shape}, {req.shape})")
+        if req.kspace.numel() != req.ncoils * req.npts:
+            raise ValueError("k-space does not match the trajectory")
+        if not np.isfinite(req.coord).all():
+            raise ValueError("coordinates contain NaN or Inf")
+
+    def _admit(self):
+        admitted = False
+        while self.queue and len(self.active) < self.max_active:
+            req = self.queue[0]
+            if req.cancelled:
+                self.queue.popleft()
+                req.error = "cancelled"
+                req.t_done = self.now()
+                self.done.append(req)
+                continue
+            need = estimate_request_bytes(req.shape, req.ncoils, req.npts, self.oversamp)
+            if need > self.budget:
+                self.queue.popleft()
+                req.error = f"request needs {need} bytes, more than the HBM budget {self.budget}"
+                req.t_done = self.now()
+                self.done.append(req)
+                continue
+            if self.reserved + need > self.budget:
+                break  # wait for memory to free up (FIFO)
+            self.queue.popleft()
+            try:
+                self._validate(req)
+                plan = self._plan_for(req)
+                st = _Active(req, plan, self.device)
+            except Exception as exc:  # a malformed request fails alone
+                req.error = f"{type(exc).__name__}: {exc}"
+                req.t_done = self.now()
+                self.done.append(req)
+                continue
+            st.bytes = need
+            self.reserved += need
+            req.t_admit = self.now()
+            self.active.append(st)
+            admitted = True
+        return admitted
+
+    # ---- one step ------------------------------------------------------------
+    def _build_layout(self):
+        """Arenas and batched launch tables for the current active set."""
+        act = sorted(self.active, key=lambda s: (s.req.shape, s.req.rid))
+        groups = collections.OrderedDict()
+        for s in act:
+            groups.setdefault(s.req.shape, []).append(s)
+        dev = self.device
+        img_rows = sum(s.req.ncoils for s in act)
+        # per-geometry contiguous slices of the arenas
+        gsizes = {n: math.ceil(np.float32(self.oversamp) * n) for n in groups}
+        grid_elems = sum(s.req.ncoils * gsizes[s.req.shape] ** 2 for s in act)
+        ksp_elems = sum(s.req.ncoils * s.req.npts for s in act)
+        lay = dict(order=act, groups=groups, gsizes=gsizes)
+        lay["gridA"] = torch.empty(grid_elems, dtype=torch.complex64, device=dev)
+        lay["gridB"] = torch.empty(grid_elems, dtype=torch.complex64, device=dev)
+        lay["kspace"] = torch.empty(max(ksp_elems, 1), dtype=torch.complex64, device=dev)
+        lay["images"] = {}
+        goff, koff = 0, 0
+        offs = {}
+        for n, members in groups.items():
+            g = gsizes[n]
+            rows = sum(s.req.ncoils for s in members)
+            lay["images"][n] = torch.empty((rows, n, n), dtype=torch.complex64, device=dev)
+            lay.setdefault("gslice", {})[n] = (goff, goff + rows * g * g)
+            for s in members:
+                offs[s.req.rid] = (goff, koff)
+                goff += s.req.ncoils * g * g
+                koff += s.req.ncoils * s.req.npts
+        lay["offs"] = offs
+        lay["grid_batch"] = _NativeBatch([s.plan for s in act], [s.req.ncoils for s in act],
+                                         [offs[s.req.rid][0] for s in act],
+                                         [offs[s.req.rid][1] for s in act], dev)
+        cg = [s for s in act if s.it >= 0]
+        lay["interp_batch"] = (_NativeBatch([s.plan for s in cg], [s.req.ncoils for s in cg],
+                                            [offs[s.req.rid][0] for s in cg],
+                                            [offs[s.req.rid][1] for s in cg], dev) if cg else None)
+        del img_rows
+        return lay
+
+    def _layout_for_step(self):
+        key = tuple((s.req.rid, s.it >= 0) for s in sorted(self.active, key=lambda s: (s.req.shape, s.req.rid)))
+        if key != self._layout_key:
+            self._layout = self._build_layout()
+            self._layout_key = key
+        return self._layout
+
+    def step(self):
+        """Admit what fits, then advance every active request by one iteration."""
+        self._admit()
+        for s in list(self.active):
+            if s.req.cancelled:
+                self._finish(s, error="cancelled")
+        if not self.active:
+            return False
+        lay = self._layout_for_step()
+        dev = self.device
+        with torch.cuda.device(dev):
+            # coil expansion of the search directions (CG requests)
+            for n, members in lay["groups"].items():
+                img = lay["images"][n]
+                row = 0
+                for s in members:
+                    c = s.req.ncoils
+                    if s.it >= 0:
+                        torch.mul(s.smaps, s.p, out=img[row:row + c])
+                    row += c
+            # forward NUFFT of every CG request: pad + FFT per geometry, one interp launch
+            if lay["interp_batch"] is not None:
+                for n, members in lay["groups"].items():
+                    a, b = lay["gslice"][n]
+                    g = lay["gsizes"][n]
+                    ga = lay["gridA"][a:b].view(-1, g, g)
+                    members[0].plan.interpolator.apod_pad(lay["images"][n], ga, ga.shape[0])
+                    torch.fft.fftn(ga, dim=(-2, -1), out=lay["gridB"][a:b].view(-1, g, g))
+                lay["interp_batch"].interp(lay["gridB"], lay["kspace"])
+            # measured data of requests that start this step
+            for s in lay["order"]:
+                if s.it < 0:
+                    _, ko = lay["offs"][s.req.rid]
+                    lay["kspace"][ko:ko + s.y.numel()].copy_(s.y.reshape(-1))
+            # adjoint of every request: one gridding launch, IFFT + crop per geometry
+            lay["grid_batch"].grid(lay["kspace"], lay["gridA"])
+            for n, members in lay["groups"].items():
+                a, b = lay["gslice"][n]
+                g = lay["gsizes"][n]
+                gb = lay["gridB"][a:b].view(-1, g, g)
+                torch.fft.ifftn(lay["gridA"][a:b].view(-1, g, g), dim=(-2, -1), out=gb)
+                members[0].plan.interpolator.crop_apod(gb, lay["images"][n], gb.shape[0])
+            # coil combination + CG update per request
+            finished = []
+            for n, members in lay["groups"].items():
+                img = lay["images"][n]
+                row = 0
+                for s in members:
+                    c = s.req.ncoils
+                    v = (s.smaps.conj() * img[row:row + c]).sum(0)
+                    row += c
+                    if s.it < 0:
+                        s.r = v
+                        s.p = v.clone()
+                        s.rs_old = torch.real(torch.vdot(v.reshape(-1), v.reshape(-1)))
+                        s.it = 0
+                    else:
+                        ap = v
+                        alpha = s.rs_old / torch.vdot(s.p.reshape(-1), ap.reshape(-1))
+                        s.x = s.x + s.p * alpha
+                        s.r = s.r + ap * (-alpha)
+                        rs_new = torch.real(torch.vdot(s.r.reshape(-1), s.r.reshape(-1)))
+                        s.p = s.r + s.p * (rs_new / s.rs_old)
+                        s.rs_old = rs_new
+                        s.it += 1
+                        if s.it >= s.req.niter:
+                            finished.append(s)
+            if finished:
+                torch.cuda.synchronize(dev)
+                for s in finished:
+                    self._finish(s)
+        self.steps += 1
+        return True
+
+    def _finish(self, s, error=None):
+        self.active.remove(s)
+        self.reserved -= s.bytes
+        s.req.result = None if error else s.x
+        s.req.error = error
+        s.req.t_done = self.now()
+        self.done.append(s.req)
+
+    # ---- driving a stream ----------------------------------------------------
+    def run(self, requests, realtime=True, on_step=None):
+        """Serve ``requests`` (arrival times in seconds from now). Returns them
+        with ``result`` / ``error`` and timing fields filled."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        i = 0
+        total = len(pending)
+        while len(self.done) < total:
+            now = self.now()
+            while i < total and (not realtime or pending[i].arrival <= now):
+                self.submit(pending[i])
+                i += 1
+            progressed = self.step()
+            if on_step is not None:
+                on_step(self)
+            if not progressed and i < total and realtime:
+                wait = pending[i].arrival - self.now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+            elif not progressed and not self.queue and i >= total:
+                break
+        torch.cuda.synchronize(self.device)
+        return requests
+
+
+class _NativeBatch:
+    """tn_batch over arbitrary plans with explicit arena offsets."""
+
+    def __init__(self, plans, ncoils, grid_off, samp_off, device):
+        import ctypes
+        import weakref
+
+        from . import _lib
+
+        n = len(plans)
+        lib = _lib.load()
+        handles = (ctypes.c_void_p * n)(*[p.interpolator.handle.value for p in plans])
+        nb = (ctypes.c_int64 * n)(*ncoils)
+        go = (ctypes.c_int64 * n)(*grid_off)
+        so = (ctypes.c_int64 * n)(*samp_off)
+        out = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _lib.check(lib.tn_batch_create(handles, nb, go, so, None, n, ctypes.byref(out)), "tn_batch_create")
+        self.handle = out
+        self.device = device
+        self.plans = plans
+        from .batch import _destroy_batch
+
+        self._fin = weakref.finalize(self, _destroy_batch, out.value, device)
+
+    def interp(self, grid, kspace):
+        from . import _lib
+
+        _lib.check(_lib.load().tn_batch_interp(self.handle, _core.ptr(grid), _core.ptr(kspace), None,
+                                               _core.stream_ptr(self.device)), "tn_batch_interp")
+
+    def grid(self, kspace, grid):
+        from . import _lib
+
+        _lib.check(_lib.load().tn_batch_grid(self.handle, _core.ptr(kspace), _core.ptr(grid), None,
+                                             _core.stream_ptr(self.device)), "tn_batch_grid")
+
+
+def run_plain(requests, device=None, width=4, oversamp=1.25, realtime=True, replan=False):
+    """The same requests one at a time through the plain GPU NUFFT.
+
+    Each request runs its CG-SENSE with apply_nufft / apply_nufft_adj calls
+    (one request on the device at a time, plan reused across its iterations
+    unless ``replan``, where every call plans like deepmr's nufft/nufft_adj).
+    """
+    dev = _core.compute_device(device)
+    t0 = time.perf_counter()
+    for req in sorted(requests, key=lambda r: r.arrival):
+        if realtime:
+            wait = req.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        req.t_admit = time.perf_counter() - t0
+        try:
+            with torch.cuda.device(dev):
+                smaps = req.smaps.to(dev, torch.complex64)
+                y = req.kspace.to(dev, torch.complex64).reshape(req.ncoils, *req.coord.shape[:-1])
+                plan = None if replan else _fft.plan_nufft(req.coord, req.shape, width, oversamp, device=dev)
+
+                def fwd(v):
+                    if replan:
+                        return _fft.nufft(v, req.coord, req.shape, device=dev, width=width, oversamp=oversamp)
+                    return _fft.apply_nufft(v, plan)
+
+                def adj(k):
+                    if replan:
+                        return _fft.nufft_adj(k, req.coord, req.shape, device=dev, width=width,
+                                              oversamp=oversamp)
+                    return _fft.apply_nufft_adj(k, plan)
+
+                def EHE(v):
+                    return (smaps.conj() * adj(fwd(smaps * v))).sum(0)
+
+                b = (smaps.conj() * adj(y)).sum(0)
+                x = torch.zeros_like(b)
+                r = b
+                p = b.clone()
+                rs_old = torch.real(torch.vdot(r.reshape(-1), r.reshape(-1)))
+                for _ in range(req.niter):
+                    ap = EHE(p)
+                    alpha = rs_old / torch.vdot(p.reshape(-1), ap.reshape(-1))
+                    x = x + p * alpha
+                    r = r + ap * (-alpha)
+                    rs_new = torch.real(torch.vdot(r.reshape(-1), r.reshape(-1)))
+                    p = r + p * (rs_new / rs_old)
+                    rs_old = rs_new
+                torch.cuda.synchronize(dev)
+            req.result = x
+        except Exception as exc:
+            req.error = f"{type(exc).__name__}: {exc}"
+        req.t_done = time.perf_counter() - t0
+    return requests
+
+
+def latency_stats(requests):
+    """p50 / p99 latency (s) and completed recon/s over the stream."""
+    ok = [r for r in requests if r.error is None and r.t_done is not None]
+    if not ok:
+        return dict(p50=None, p99=None, recon_per_s=0.0, completed=0)
+    lat = np.array([r.t_done - r.arrival for r in ok])
+    span = max(r.t_done for r in ok) - min(r.arrival for r in requests)
+    return dict(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                recon_per_s=len(ok) / span if span > 0 else float("inf"), completed=len(ok))
+
+
+def make_stream(nreq, rate_hz, seed=0, sizes=(128, 192, 256, 320), coils=(4, 32),
+                samples=(10_000, 200_000), niter=10, device="cpu", traj_pool=None):
+    """Synthetic request stream: Poisson arrivals at ``rate_hz``; per request a
+    matrix size from ``sizes``, U{coils} coils, a radial or spiral trajectory
+    with U[samples] samples per coil, complex N(0,1) k-space and Gaussian-bump
+    coil maps. ``traj_pool`` > 0 draws trajectories from that many shared
+    keys per size (repeated trajectories reuse resident plans)."""
+    from . import testing
+
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator().manual_seed(seed)
+    t = 0.0
+    out = []
+    maps_cache = {}
+    for i in range(nreq):
+        t += rng.exponential(1.0 / rate_hz) if rate_hz > 0 else 0.0
+        n = int(rng.choice(sizes))
+        c = int(rng.integers(coils[0], coils[1] + 1))
+        m = int(rng.integers(samples[0], samples[1] + 1))
+        kind = "radial" if rng.random() < 0.5 else "spiral"
+        key = None
+        trng = rng
+        if traj_pool:
+            j = int(rng.integers(traj_pool))
+            key = (kind, n, j)
+            trng = np.random.default_rng([seed, 0 if kind == "radial" else 1, n, j])
+            m = int(trng.integers(samples[0], samples[1] + 1))
+        if kind == "radial":
+            ns = 2 * n
+            nv = max(1, m // ns)
+            coord = testing.golden_angle_radial(nv, ns, n, offset=float(trng.uniform(0, np.pi)))
+        else:
+            ns = 1024
+            nv = max(1, m // ns)
+            coord = testing.vd_spiral(nv, ns, n, rotation=float(trng.uniform(0, 2 * np.pi)))
+        if (c, n) not in maps_cache:
+            maps_cache[(c, n)] = torch.from_numpy(testing.gaussian_coil_maps(c, (n, n), seed=c))
+        k = testing.complex_normal((c, *coord.shape[:-1]), generator=gen)
+        out.append(ReconRequest(rid=i, coord=coord, kspace=k.to(device), smaps=maps_cache[(c, n)].to(device),
+                                shape=n, niter=niter, arrival=t, traj_key=key))
+    return out

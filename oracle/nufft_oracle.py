@@ -28,6 +28,22 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBS = {}
+_FAST = False          # use the -ffast-math build of the loops (CPU baseline timing)
+_FFT_BACKEND = "numpy"  # "torch": torch's CPU FFT, as deepmr's CPU path uses
+
+
+def use_fast_loops(flag):
+    """Route the raw loops through liborc_nufft_fast.so (baseline timing)."""
+    global _FAST
+    _FAST = bool(flag)
+
+
+def set_fft_backend(name):
+    """FFT used by fft_c / ifft_c: "numpy" (checker) or "torch" (deepmr's CPU FFT)."""
+    global _FFT_BACKEND
+    if name not in ("numpy", "torch"):
+        raise ValueError(name)
+    _FFT_BACKEND = name
 
 
 def load_loops(fast=False):
@@ -274,9 +290,20 @@ def resize(x, oshape):
     return out
 
 
+def _torch_fft(x, ax, inverse):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(x))
+    t = torch.fft.ifftshift(t, dim=ax)
+    t = torch.fft.ifftn(t, dim=ax) if inverse else torch.fft.fftn(t, dim=ax)
+    return torch.fft.fftshift(t, dim=ax).numpy()
+
+
 def fft_c(x, axes, norm=None):
     """Centred FFT, ifftshift -> fftn -> fftshift (dm/fft/fft.py:9-87)."""
     ax = tuple(a % x.ndim for a in sorted(axes))
+    if _FFT_BACKEND == "torch" and norm is None:
+        return _torch_fft(x, ax, False)
     y = np.fft.fftn(np.fft.ifftshift(x, axes=ax), axes=ax, norm=norm)
     return np.fft.fftshift(y, axes=ax)
 
@@ -284,6 +311,8 @@ def fft_c(x, axes, norm=None):
 def ifft_c(x, axes, norm=None):
     """Centred IFFT (dm/fft/fft.py:90-167); norm=None scales by 1/prod(G)."""
     ax = tuple(a % x.ndim for a in sorted(axes))
+    if _FFT_BACKEND == "torch" and norm is None:
+        return _torch_fft(x, ax, True)
     y = np.fft.ifftn(np.fft.ifftshift(x, axes=ax), axes=ax, norm=norm)
     return np.fft.fftshift(y, axes=ax)
 
@@ -297,12 +326,12 @@ def _tables(interp, ndim):
     return idx, val
 
 
-def interpolate_raw(grid, interp, fast=False):
+def interpolate_raw(grid, interp, fast=None):
     """Kernel call on (nframes, batch, *G) -> (nframes, batch, npts).
 
     dm/fft/_interp/interp.py:154-227 (through oracle/nufft_loops.c).
     """
-    lib = load_loops(fast)
+    lib = load_loops(_FAST if fast is None else fast)
     ndim = interp["ndim"]
     idx, val = _tables(interp, ndim)
     grid = np.ascontiguousarray(grid, np.complex64)
@@ -326,12 +355,12 @@ def interpolate_raw(grid, interp, fast=False):
     return out
 
 
-def grid_raw(samp, interp, fast=False):
+def grid_raw(samp, interp, fast=None):
     """Kernel call on (nframes, batch, npts) -> (nframes, batch, *G).
 
     dm/fft/_interp/grid.py:211-284 (through oracle/nufft_loops.c).
     """
-    lib = load_loops(fast)
+    lib = load_loops(_FAST if fast is None else fast)
     ndim = interp["ndim"]
     idx, val = _tables(interp, ndim)
     samp = np.ascontiguousarray(samp, np.complex64)
