@@ -44,6 +44,8 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
             j.shift[d] = p.shift[zyx];
             j.T[d] = plan->tile[zyx];
             j.NT[d] = plan->ntiles[zyx];
+            j.IT[d] = plan->itile[zyx];
+            j.INT_[d] = plan->nitiles[zyx];
             j.beta[d] = p.beta[zyx];
         }
         j.inv_scale = p.inv_scale;
@@ -53,6 +55,8 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.coord = plan->d_coord + (int64_t)f * p.npts * p.ndim;
         j.entries = plan->d_entries;
         j.bin_off = plan->d_bin_off + (int64_t)f * plan->ntiles_total;
+        j.pentries = plan->d_pentries;
+        j.pbin_off = plan->d_pbin_off + (int64_t)f * plan->nitiles_total;
         // deepmr layout (batch, nframes, ...): frame f of coil b
         j.grid_off = grid_off + (int64_t)f * gvol;
         j.grid_bstride = (int64_t)p.nframes * gvol;
@@ -76,18 +80,31 @@ void free_tables(LaunchTables &t) {
     if (t.d_jobs) (void)hipFree(t.d_jobs);
     if (t.d_interp) (void)hipFree(t.d_interp);
     if (t.d_grid) (void)hipFree(t.d_grid);
+    if (t.d_fold) (void)hipFree(t.d_fold);
+    if (t.d_slot_base) (void)hipFree(t.d_slot_base);
+    if (t.d_scratch) (void)hipFree(t.d_scratch);
     t = LaunchTables{};
 }
 
 static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &interp,
-                         const std::vector<Item> &grid, LaunchTables &t) {
+                         const std::vector<Item> &grid, const std::vector<Item> &fold,
+                         const std::vector<int64_t> &slot_base, LaunchTables &t) {
     t.n_interp = (int64_t)interp.size();
     t.n_grid = (int64_t)grid.size();
+    t.n_fold = (int64_t)fold.size();
     if (t.n_interp >= INT32_MAX || t.n_grid >= INT32_MAX)
         return fail(TN_E_RANGE, "work list too long");
     TN_HIP(hipMalloc(&t.d_jobs, sizeof(Job) * std::max<size_t>(jobs.size(), 1)));
     TN_HIP(hipMalloc(&t.d_interp, sizeof(Item) * std::max<size_t>(interp.size(), 1)));
     TN_HIP(hipMalloc(&t.d_grid, sizeof(Item) * std::max<size_t>(grid.size(), 1)));
+    TN_HIP(hipMalloc(&t.d_fold, sizeof(Item) * std::max<size_t>(fold.size(), 1)));
+    TN_HIP(hipMalloc(&t.d_slot_base, sizeof(int64_t) * std::max<size_t>(slot_base.size(), 1)));
+    TN_HIP(hipMalloc(&t.d_scratch, sizeof(float) * std::max<int64_t>(t.scratch_floats, 1)));
+    if (!fold.empty())
+        TN_HIP(hipMemcpy(t.d_fold, fold.data(), sizeof(Item) * fold.size(), hipMemcpyHostToDevice));
+    if (!slot_base.empty())
+        TN_HIP(hipMemcpy(t.d_slot_base, slot_base.data(), sizeof(int64_t) * slot_base.size(),
+                         hipMemcpyHostToDevice));
     if (!jobs.empty())
         TN_HIP(hipMemcpy(t.d_jobs, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice));
     if (!interp.empty())
@@ -105,32 +122,85 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                             LaunchTables &t) {
     t.kind_interp = interp_kind(jobs);
     t.kind_grid = t.kind_interp;
-    int64_t max_cells = 1;
-    for (const Job &j : jobs) max_cells = std::max<int64_t>(max_cells, (int64_t)j.T[0] * j.T[1] * j.T[2]);
+    int64_t max_cells = 1, max_region = 1;
+    for (const Job &j : jobs) {
+        max_cells = std::max<int64_t>(max_cells, (int64_t)j.T[0] * j.T[1] * j.T[2]);
+        max_region = std::max<int64_t>(max_region, (int64_t)(j.IT[0] + j.W[0] - 1) *
+                                                       (j.IT[1] + j.W[1] - 1) * (j.IT[2] + j.W[2] - 1));
+    }
     const size_t per_coil = (size_t)max_cells * 2 * sizeof(float);
-    int lds_coils = (int)std::max<size_t>(1, kGridLdsBudget / per_coil);
-    if (per_coil > 160 * 1024) return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes)", per_coil);
+    const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
+    if (per_coil > kGridLdsBudget)
+        return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes per coil)", per_coil);
+    const int lds_coils = (int)std::max<size_t>(1, kGridLdsBudget / per_coil);
+    const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
 
-    int max_gc = 1;
-    std::vector<Item> interp, grid;
+    int mode = plan_of.empty() ? kInterpGather : plan_of[0]->p.interp_mode;
+    if (mode == kInterpAuto) mode = kInterpStaged;
+    if (mode == kInterpStaged && region_coil > kInterpLdsBudget) mode = kInterpGather;
+    t.interp_mode = mode;
+
+    int max_gc = 1, max_ic = 1;
+    std::vector<Item> interp, grid, fold;
+    std::vector<int64_t> slot_base;
+    int64_t scratch = 0;
     for (size_t ji = 0; ji < jobs.size(); ++ji) {
         const Job &j = jobs[ji];
         const tn_plan *pl = plan_of[ji];
-        const int ic = chunk_for(pl->p.interp_chunk, j.nbatch, j.nbatch >= 8 ? (int)((j.nbatch + 1) / 2) : j.nbatch);
         const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, lds_coils), lds_coils);
         max_gc = std::max(max_gc, gc);
-        const int64_t nblk = (j.npts + kInterpBlockPts - 1) / kInterpBlockPts;
-        for (int64_t b = 0; b < nblk; ++b)
-            for (int c0 = 0; c0 < j.nbatch; c0 += ic)
-                interp.push_back(make_int4((int)ji, (int)b, c0, std::min(j.nbatch, c0 + ic)));
+        if (mode == kInterpStaged) {
+            const int ic = std::min(chunk_for(pl->p.interp_chunk, j.nbatch, 4), region_coils);
+            max_ic = std::max(max_ic, ic);
+            const int64_t nt = (int64_t)j.INT_[0] * j.INT_[1] * j.INT_[2];
+            const int64_t f = (j.pbin_off - pl->d_pbin_off) / std::max<int64_t>(pl->nitiles_total, 1);
+            const int32_t *off = pl->h_pbin_off.data() + f * pl->nitiles_total;
+            for (int64_t tl = 0; tl < nt; ++tl) {
+                for (int32_t e0 = off[tl]; e0 < off[tl + 1]; e0 += kInterpSegment) {
+                    const int32_t e1 = std::min<int32_t>(off[tl + 1], e0 + kInterpSegment);
+                    for (int c0 = 0; c0 < j.nbatch; c0 += ic)
+                        interp.push_back(Item{(int)ji, (int)tl, c0, std::min(j.nbatch, c0 + ic), e0, e1, 0, 0});
+                }
+            }
+        } else {
+            const int ic = chunk_for(pl->p.interp_chunk, j.nbatch, j.nbatch >= 8 ? (int)((j.nbatch + 1) / 2) : j.nbatch);
+            const int64_t nblk = (j.npts + kInterpBlockPts - 1) / kInterpBlockPts;
+            for (int64_t b = 0; b < nblk; ++b)
+                for (int c0 = 0; c0 < j.nbatch; c0 += ic)
+                    interp.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + ic), 0, 0, 0, 0});
+        }
         const int64_t ntiles = (int64_t)j.NT[0] * j.NT[1] * j.NT[2];
-        for (int64_t tl = 0; tl < ntiles; ++tl)
-            for (int c0 = 0; c0 < j.nbatch; c0 += gc)
-                grid.push_back(make_int4((int)ji, (int)tl, c0, std::min(j.nbatch, c0 + gc)));
+        const int64_t fg = (j.bin_off - pl->d_bin_off) / std::max<int64_t>(pl->ntiles_total, 1);
+        const int32_t *goff = pl->h_bin_off.data() + fg * pl->ntiles_total;
+        const int split = pl->p.grid_split == 0 ? kGridSplitDefault
+                          : pl->p.grid_split < 0 ? INT32_MAX : pl->p.grid_split;
+        const int64_t cells = (int64_t)j.T[0] * j.T[1] * j.T[2];
+        for (int64_t tl = 0; tl < ntiles; ++tl) {
+            const int32_t b0 = goff[tl], b1 = goff[tl + 1];
+            const int nparts = (int)std::max<int64_t>(1, ((int64_t)(b1 - b0) + split - 1) / split);
+            for (int c0 = 0; c0 < j.nbatch; c0 += gc) {
+                const int c1 = std::min(j.nbatch, c0 + gc);
+                if (nparts == 1) {
+                    grid.push_back(Item{(int)ji, (int)tl, c0, c1, b0, b1, 0, -1});
+                    continue;
+                }
+                const int slot = (int)slot_base.size();
+                slot_base.push_back(scratch);
+                scratch += (int64_t)nparts * 2 * (c1 - c0) * cells;
+                const int32_t per = (b1 - b0 + nparts - 1) / nparts;
+                for (int q = 0; q < nparts; ++q) {
+                    const int32_t e0 = b0 + q * per, e1 = std::min(b1, e0 + per);
+                    grid.push_back(Item{(int)ji, (int)tl, c0, c1, e0, e1, q, slot});
+                }
+                fold.push_back(Item{(int)ji, (int)tl, c0, c1, slot, nparts, 0, 0});
+            }
+        }
     }
+    t.scratch_floats = scratch;
     t.grid_coils = max_gc;
     t.grid_lds = per_coil * (size_t)max_gc;
-    return upload_tables(jobs, interp, grid, t);
+    t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
+    return upload_tables(jobs, interp, grid, fold, slot_base, t);
 }
 
 }  // namespace tn
@@ -157,10 +227,11 @@ int validate(const tn_plan_params *p) {
             return fail(TN_E_ARG, "axis %d: width %d unsupported (1..8, <= grid)", d, p->width[d]);
         if (p->shift[d] < 0 || p->shift[d] >= p->grid[d])
             return fail(TN_E_ARG, "axis %d: shift %d outside [0, grid)", d, p->shift[d]);
-        if (p->tile[d] < 0) return fail(TN_E_ARG, "axis %d: negative tile", d);
+        if (p->tile[d] < 0 || p->itile[d] < 0) return fail(TN_E_ARG, "axis %d: negative tile", d);
     }
     if ((int64_t)p->grid[0] * p->grid[1] * p->grid[2] >= INT32_MAX)
         return fail(TN_E_RANGE, "grid too large");
+    if (p->interp_mode < 0 || p->interp_mode > 2) return fail(TN_E_ARG, "interp_mode must be 0, 1 or 2");
     return TN_OK;
 }
 
@@ -172,13 +243,25 @@ void choose_tiles(tn_plan *pl) {
     } else {
         def[0] = 16; def[1] = 16; def[2] = 16;
     }
+    int idef[3];
+    if (p.ndim == 2) {
+        idef[0] = 1; idef[1] = 32; idef[2] = 32;
+    } else {
+        idef[0] = 8; idef[1] = 8; idef[2] = 16;
+    }
     pl->ntiles_total = 1;
+    pl->nitiles_total = 1;
     for (int d = 0; d < 3; ++d) {
         int t = p.tile[d] > 0 ? p.tile[d] : def[d];
         t = std::min(t, p.grid[d]);
         pl->tile[d] = t;
         pl->ntiles[d] = (p.grid[d] + t - 1) / t;
         pl->ntiles_total *= pl->ntiles[d];
+        int it = p.itile[d] > 0 ? p.itile[d] : idef[d];
+        it = std::min(it, p.grid[d]);
+        pl->itile[d] = it;
+        pl->nitiles[d] = (p.grid[d] + it - 1) / it;
+        pl->nitiles_total *= pl->nitiles[d];
     }
 }
 
@@ -191,6 +274,8 @@ void destroy(tn_plan *pl) {
     if (pl->d_coord) (void)hipFree(pl->d_coord);
     if (pl->d_entries) (void)hipFree(pl->d_entries);
     if (pl->d_bin_off) (void)hipFree(pl->d_bin_off);
+    if (pl->d_pentries) (void)hipFree(pl->d_pentries);
+    if (pl->d_pbin_off) (void)hipFree(pl->d_pbin_off);
     for (int d = 0; d < 3; ++d)
         if (pl->d_apod[d]) (void)hipFree(pl->d_apod[d]);
     (void)hipSetDevice(cur);
@@ -287,10 +372,13 @@ int64_t tn_plan_device_bytes(const tn_plan *plan) {
     int64_t b = sizeof(float) * (int64_t)p.nframes * p.npts * p.ndim;
     b += sizeof(float4) * plan->nentries;
     b += sizeof(int32_t) * (plan->ntiles_total * p.nframes + 1);
+    b += sizeof(float4) * (int64_t)p.nframes * p.npts;
+    b += sizeof(int32_t) * (plan->nitiles_total * p.nframes + 1);
     for (int d = 0; d < 3; ++d)
         if (plan->d_apod[d]) b += sizeof(float) * p.shape[d];
     for (const auto &kv : plan->cache)
-        b += sizeof(Job) * p.nframes + sizeof(Item) * (kv.second.n_interp + kv.second.n_grid);
+        b += sizeof(Job) * p.nframes + sizeof(Item) * (kv.second.n_interp + kv.second.n_grid + kv.second.n_fold) +
+             sizeof(float) * kv.second.scratch_floats;
     return b;
 }
 

@@ -17,6 +17,12 @@ constexpr int kWave = 64;
 constexpr int kMaxWidth = 16;      // largest kernel width supported per axis
 constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
 constexpr size_t kGridLdsBudget = 64 * 1024;  // LDS per gridding workgroup
+constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation workgroup
+constexpr int kInterpSegment = 2048;  // max samples per staged-interpolation work item
+constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
+
+// Interpolation variants.
+enum InterpMode { kInterpAuto = 0, kInterpGather = 1, kInterpStaged = 2 };
 
 // One trajectory (frame) bound to its data placement for a launch. All
 // per-axis arrays are (x, y, z) so that axis d matches coordinate column d.
@@ -26,6 +32,9 @@ struct alignas(16) Job {
     int32_t shift[3];   // output index = (tap + shift) mod G
     int32_t T[3];       // gridding tile
     int32_t NT[3];      // tiles per axis
+    int32_t IT[3];      // staged-interpolation tile
+    int32_t INT_[3];    // staged-interpolation tiles per axis
+    int32_t pad1[2];
     float beta[3];
     float inv_scale;
     int32_t ndim;
@@ -35,6 +44,9 @@ struct alignas(16) Job {
     const float *coord;       // (npts, ndim) grid units
     const float4 *entries;    // (x, y, z, bits(sample)) sorted by tile
     const int32_t *bin_off;   // (ntiles + 1) offsets into entries
+    const float4 *pentries;   // each sample once, sorted by the interpolation
+                              // tile holding its footprint origin
+    const int32_t *pbin_off;  // (interp tiles + 1) offsets into pentries
     int64_t grid_off;         // complex elements from the grid base
     int64_t grid_bstride;     // complex elements between coils
     int64_t samp_off;
@@ -42,20 +54,33 @@ struct alignas(16) Job {
     int64_t w_off;            // offset into the weight base (-1: no weight)
 };
 
-// Work item: {job, unit, first coil, end coil}; unit = sample block
-// (interpolation) or tile (gridding).
-using Item = int4;
+// Work item. Gathering interpolation: unit = sample block. Staged
+// interpolation: unit = tile, [e0, e1) = slice of its entries. Gridding:
+// unit = tile, [e0, e1) = slice of its entries; part / slot >= 0 when the
+// tile is split (partial `part` of fold slot `slot`), slot = -1 otherwise.
+// Fold items: unit = tile, e0 = slot, e1 = number of parts.
+struct alignas(16) Item {
+    int32_t job, unit, c0, c1;
+    int32_t e0, e1, part, slot;
+};
 
 struct LaunchTables {
     Job *d_jobs = nullptr;
     Item *d_interp = nullptr;
     Item *d_grid = nullptr;
+    Item *d_fold = nullptr;
+    int64_t *d_slot_base = nullptr;  // float offset of each fold slot in d_scratch
+    float *d_scratch = nullptr;      // partial tiles of split gridding tiles
     int64_t n_interp = 0;
     int64_t n_grid = 0;
+    int64_t n_fold = 0;
+    int64_t scratch_floats = 0;
     int grid_coils = 1;        // coils per gridding workgroup (LDS bound)
     size_t grid_lds = 0;       // dynamic LDS bytes for the gridding kernel
     int kind_interp = 0;       // kernel variant selector
     int kind_grid = 0;
+    int interp_mode = kInterpGather;
+    size_t interp_lds = 0;     // dynamic LDS bytes for the staged interpolation
 };
 
 }  // namespace tn
@@ -66,6 +91,13 @@ struct tn_plan {
     int32_t tile[3] = {1, 1, 1};   // (z, y, x)
     int32_t ntiles[3] = {1, 1, 1};
     int64_t ntiles_total = 1;
+    int32_t itile[3] = {1, 1, 1};  // staged-interpolation tile (z, y, x)
+    int32_t nitiles[3] = {1, 1, 1};
+    int64_t nitiles_total = 1;
+    float4 *d_pentries = nullptr;
+    int32_t *d_pbin_off = nullptr;
+    std::vector<int32_t> h_pbin_off;  // host copies (work-list construction)
+    std::vector<int32_t> h_bin_off;
     float *d_coord = nullptr;
     float4 *d_entries = nullptr;
     int32_t *d_bin_off = nullptr;
@@ -97,6 +129,7 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp,
                   const float *weight, hipStream_t stream);
 int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid,
                 const float *weight, hipStream_t stream);
+int launch_fold(const LaunchTables &t, float2 *grid, hipStream_t stream);
 int launch_apod_pad(const tn_plan *plan, const float2 *image, float2 *grid,
                     int64_t nimg, hipStream_t stream);
 int launch_crop_apod(const tn_plan *plan, const float2 *grid, float2 *image,

@@ -59,8 +59,8 @@ __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ 
                                                         float2 *__restrict__ samp,
                                                         const float *__restrict__ weight) {
     const Item it = items[blockIdx.x];
-    const Job &J = jobs[it.x];
-    const int64_t p = (int64_t)it.y * kInterpBlockPts + threadIdx.x;
+    const Job &J = jobs[it.job];
+    const int64_t p = (int64_t)it.unit * kInterpBlockPts + threadIdx.x;
     if (p >= J.npts) return;
 
     const float *c = J.coord + p * NDIM;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ 
     if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
 
     const int Gx = J.G[0], Gy = J.G[1];
-    for (int b = it.z; b < it.w; ++b) {
+    for (int b = it.c0; b < it.c1; ++b) {
         const float2 *g = grid + J.grid_off + (int64_t)b * J.grid_bstride;
         float re = 0.0f, im = 0.0f;
 #pragma unroll
@@ -102,16 +102,112 @@ __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ 
     }
 }
 
+// Staged interpolation: one workgroup loads the (tile + W-1 halo) region of
+// the grid for a few coils into LDS with coalesced row reads, then every lane
+// interpolates one sample whose footprint origin lies in the tile, gathering
+// its taps from LDS. Samples come from the plan's primary bins (each sample
+// exactly once); hot tiles are split into several work items by entry range.
+template <int NDIM, int WC>
+__global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__restrict__ jobs,
+                                                               const Item *__restrict__ items,
+                                                               const float2 *__restrict__ grid,
+                                                               float2 *__restrict__ samp,
+                                                               const float *__restrict__ weight) {
+    extern __shared__ float2 region[];
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int tile = it.unit;
+    const int Tx = J.IT[0], Ty = J.IT[1], Tz = J.IT[2];
+    const int tx = tile % J.INT_[0];
+    const int ty = (tile / J.INT_[0]) % J.INT_[1];
+    const int tz = tile / (J.INT_[0] * J.INT_[1]);
+    const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
+    const int Rx = Tx + J.W[0] - 1, Ry = Ty + J.W[1] - 1, Rz = Tz + J.W[2] - 1;
+    const int Gx = J.G[0], Gy = J.G[1], Gz = J.G[2];
+    const int cells = Rx * Ry * Rz;
+    const int ncoil = it.c1 - it.c0;
+
+    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
+        const int c = i / cells;
+        const int cell = i - c * cells;
+        const int lx = cell % Rx;
+        const int ly = (cell / Rx) % Ry;
+        const int lz = cell / (Rx * Ry);
+        int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+        while (gx >= Gx) gx -= Gx;
+        while (gy >= Gy) gy -= Gy;
+        while (gz >= Gz) gz -= Gz;
+        region[i] = grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride +
+                         ((int64_t)gz * Gy + gy) * Gx + gx];
+    }
+    __syncthreads();
+
+    for (int e = it.e0 + (int)threadIdx.x; e < it.e1; e += kBlock) {
+        const float4 en = J.pentries[e];
+        const int p = __float_as_int(en.w);
+        float w[3][WC];
+        int base[3];
+        const float c3[3] = {en.x, en.y, en.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d < NDIM) {
+                axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base[d], w[d]);
+            } else {
+                base[d] = 0;
+#pragma unroll
+                for (int k = 0; k < WC; ++k) w[d][k] = k == 0 ? 1.0f : 0.0f;
+            }
+        }
+        // local origin of the footprint inside the region (clamped: a sample
+        // always bins into the tile holding its origin, the clamp only keeps
+        // LDS addressing in range)
+        const int lx0 = min(max(base[0] - ox, 0), Tx - 1);
+        const int ly0 = min(max(base[1] - oy, 0), Ty - 1);
+        const int lz0 = min(max(base[2] - oz, 0), Tz - 1);
+        const int nx = J.W[0], ny = NDIM > 1 ? J.W[1] : 1, nz = NDIM > 2 ? J.W[2] : 1;
+        float sc = J.inv_scale;
+        if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
+        for (int b = 0; b < ncoil; ++b) {
+            const float2 *r = region + (size_t)b * cells;
+            float re = 0.0f, im = 0.0f;
+#pragma unroll
+            for (int kz = 0; kz < WC; ++kz) {
+                if (kz < nz) {
+#pragma unroll
+                    for (int ky = 0; ky < WC; ++ky) {
+                        if (ky < ny) {
+                            const float vzy = w[2][kz] * w[1][ky];
+                            const float2 *row = r + ((lz0 + kz) * Ry + (ly0 + ky)) * Rx + lx0;
+#pragma unroll
+                            for (int kx = 0; kx < WC; ++kx) {
+                                if (kx < nx) {
+                                    const float v = vzy * w[0][kx];
+                                    const float2 s = row[kx];
+                                    re += v * s.x;
+                                    im += v * s.y;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            samp[J.samp_off + (int64_t)(it.c0 + b) * J.samp_bstride + p] = make_float2(re * sc, im * sc);
+        }
+    }
+}
+
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jobs,
                                                       const Item *__restrict__ items,
                                                       const float2 *__restrict__ samp,
                                                       float2 *__restrict__ grid,
-                                                      const float *__restrict__ weight) {
+                                                      const float *__restrict__ weight,
+                                                      float *__restrict__ scratch,
+                                                      const int64_t *__restrict__ slot_base) {
     extern __shared__ float lds[];
     const Item it = items[blockIdx.x];
-    const Job &J = jobs[it.x];
-    const int tile = it.y;
+    const Job &J = jobs[it.job];
+    const int tile = it.unit;
     const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
     const int tx = tile % J.NT[0];
     const int ty = (tile / J.NT[0]) % J.NT[1];
@@ -119,15 +215,15 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
     const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
     const int ex = min(Tx, J.G[0] - ox), ey = min(Ty, J.G[1] - oy), ez = min(Tz, J.G[2] - oz);
     const int cells = Tx * Ty * Tz;
-    const int ncoil = it.w - it.z;
+    const int ncoil = it.c1 - it.c0;
 
     for (int i = threadIdx.x; i < 2 * ncoil * cells; i += kBlock) lds[i] = 0.0f;
     __syncthreads();
 
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    const int e0 = J.bin_off[tile];
-    const int e1 = J.bin_off[tile + 1];
+    const int e0 = it.e0;
+    const int e1 = it.e1;
     if (wave < ncoil) {
         for (int e = e0 + lane; e < e1; e += kWave) {
             const float4 en = J.entries[e];
@@ -145,11 +241,11 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
             }
             float sc = J.inv_scale;
             if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-            for (int b = it.z + wave; b < it.w; b += kBlock / kWave) {
+            for (int b = it.c0 + wave; b < it.c1; b += kBlock / kWave) {
                 float2 s = samp[J.samp_off + (int64_t)b * J.samp_bstride + p];
                 s.x *= sc;
                 s.y *= sc;
-                float *acc = lds + (size_t)2 * (b - it.z) * cells;
+                float *acc = lds + (size_t)2 * (b - it.c0) * cells;
 #pragma unroll
                 for (int kz = 0; kz < WC; ++kz) {
                     if (lz[kz] >= 0) {
@@ -178,6 +274,13 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
     }
     __syncthreads();
 
+    if (it.slot >= 0) {
+        // split tile: store this partial; the fold kernel sums the parts in
+        // part order and writes the tile
+        float *dst = scratch + slot_base[it.slot] + (int64_t)it.part * 2 * ncoil * cells;
+        for (int i = threadIdx.x; i < 2 * ncoil * cells; i += kBlock) dst[i] = lds[i];
+        return;
+    }
     const int Gx = J.G[0], Gy = J.G[1];
     for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
         const int b = i / cells;
@@ -187,8 +290,47 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
         const int lz = cell / (Tx * Ty);
         if (lx < ex && ly < ey && lz < ez) {
             const float *a = lds + 2 * ((size_t)b * cells + cell);
-            grid[J.grid_off + (int64_t)(it.z + b) * J.grid_bstride +
+            grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
                  ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(a[0], a[1]);
+        }
+    }
+}
+
+// Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
+__global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jobs,
+                                                      const Item *__restrict__ items,
+                                                      const float *__restrict__ scratch,
+                                                      const int64_t *__restrict__ slot_base,
+                                                      float2 *__restrict__ grid) {
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int tile = it.unit;
+    const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
+    const int tx = tile % J.NT[0];
+    const int ty = (tile / J.NT[0]) % J.NT[1];
+    const int tz = tile / (J.NT[0] * J.NT[1]);
+    const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
+    const int ex = min(Tx, J.G[0] - ox), ey = min(Ty, J.G[1] - oy), ez = min(Tz, J.G[2] - oz);
+    const int cells = Tx * Ty * Tz;
+    const int ncoil = it.c1 - it.c0;
+    const int nparts = it.e1;
+    const float *src = scratch + slot_base[it.e0];
+    const int64_t pstride = 2 * (int64_t)ncoil * cells;
+    const int Gx = J.G[0], Gy = J.G[1];
+    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
+        const int b = i / cells;
+        const int cell = i - b * cells;
+        const int lx = cell % Tx;
+        const int ly = (cell / Tx) % Ty;
+        const int lz = cell / (Tx * Ty);
+        if (lx < ex && ly < ey && lz < ez) {
+            float re = src[2 * i], im = src[2 * i + 1];
+            for (int q = 1; q < nparts; ++q) {
+                re += src[q * pstride + 2 * i];
+                im += src[q * pstride + 2 * i + 1];
+            }
+            grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
+                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(re, im);
         }
     }
 }
@@ -271,7 +413,8 @@ unsigned stride_blocks(int64_t n) {
 
 // ---- dispatch ----------------------------------------------------------------
 using InterpFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *);
-using GridFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *);
+using GridFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, float *,
+                        const int64_t *);
 
 template <int NDIM>
 InterpFn pick_interp(int wc) {
@@ -284,6 +427,21 @@ InterpFn pick_interp(int wc) {
         case 6: return interp_kernel<NDIM, 6>;
         case 7: return interp_kernel<NDIM, 7>;
         case 8: return interp_kernel<NDIM, 8>;
+        default: return nullptr;
+    }
+}
+
+template <int NDIM>
+InterpFn pick_staged(int wc) {
+    switch (wc) {
+        case 1: return interp_staged_kernel<NDIM, 1>;
+        case 2: return interp_staged_kernel<NDIM, 2>;
+        case 3: return interp_staged_kernel<NDIM, 3>;
+        case 4: return interp_staged_kernel<NDIM, 4>;
+        case 5: return interp_staged_kernel<NDIM, 5>;
+        case 6: return interp_staged_kernel<NDIM, 6>;
+        case 7: return interp_staged_kernel<NDIM, 7>;
+        case 8: return interp_staged_kernel<NDIM, 8>;
         default: return nullptr;
     }
 }
@@ -319,10 +477,12 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
                   hipStream_t stream) {
     if (t.n_interp == 0) return TN_OK;
     const int ndim = t.kind_interp / 16, wc = t.kind_interp % 16;
-    InterpFn fn = ndim == 2 ? pick_interp<2>(wc) : ndim == 3 ? pick_interp<3>(wc) : nullptr;
+    const bool staged = t.interp_mode == kInterpStaged;
+    InterpFn fn = staged ? (ndim == 2 ? pick_staged<2>(wc) : ndim == 3 ? pick_staged<3>(wc) : nullptr)
+                         : (ndim == 2 ? pick_interp<2>(wc) : ndim == 3 ? pick_interp<3>(wc) : nullptr);
     if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
-    hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), 0, stream, t.d_jobs,
-                       t.d_interp, grid, samp, weight);
+    hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), staged ? t.interp_lds : 0, stream,
+                       t.d_jobs, t.d_interp, grid, samp, weight);
     TN_HIP(hipGetLastError());
     return TN_OK;
 }
@@ -334,8 +494,13 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
     GridFn fn = ndim == 2 ? pick_grid<2>(wc) : ndim == 3 ? pick_grid<3>(wc) : nullptr;
     if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
     hipLaunchKernelGGL(fn, dim3((unsigned)t.n_grid), dim3(kBlock), t.grid_lds, stream, t.d_jobs,
-                       t.d_grid, samp, grid, weight);
+                       t.d_grid, samp, grid, weight, t.d_scratch, t.d_slot_base);
     TN_HIP(hipGetLastError());
+    if (t.n_fold > 0) {
+        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
+                           t.d_fold, t.d_scratch, t.d_slot_base, grid);
+        TN_HIP(hipGetLastError());
+    }
     return TN_OK;
 }
 

@@ -68,6 +68,23 @@ __global__ void bin_count_kernel(BinGeom geo, const float *__restrict__ coord,
     }
 }
 
+// Primary bins: every sample once, keyed by the tile holding its footprint
+// origin (the staged interpolation loads that tile plus a W-1 halo).
+__global__ void bin_primary_kernel(BinGeom geo, const float *__restrict__ coord,
+                                   uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int64_t total = geo.npts * geo.nframes;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frame = i / geo.npts;
+        int t[3] = {0, 0, 0};
+        for (int d = 0; d < geo.ndim; ++d)
+            t[d] = axis_base(geo.ax[d], coord[i * geo.ndim + d]) / geo.ax[d].T;
+        const int64_t tile = ((int64_t)t[2] * geo.NT[1] + t[1]) * geo.NT[0] + t[0];
+        keys[i] = (uint32_t)(frame * geo.ntiles + tile);
+        vals[i] = (int32_t)(i - frame * geo.npts);
+    }
+}
+
 __global__ void bin_emit_kernel(BinGeom geo, const float *__restrict__ coord,
                                 const int32_t *__restrict__ offsets,
                                 uint32_t *__restrict__ keys, int32_t *__restrict__ vals) {
@@ -168,7 +185,7 @@ __global__ void count_mask_kernel(const uint8_t *__restrict__ mask, int64_t n,
     if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
 
-BinGeom make_geom(const tn_plan *plan) {
+BinGeom make_geom(const tn_plan *plan, bool primary = false) {
     BinGeom g{};
     g.ndim = plan->p.ndim;
     g.npts = plan->p.npts;
@@ -178,10 +195,10 @@ BinGeom make_geom(const tn_plan *plan) {
         g.ax[d].G = plan->p.grid[zyx];
         g.ax[d].W = plan->p.width[zyx];
         g.ax[d].shift = plan->p.shift[zyx];
-        g.ax[d].T = plan->tile[zyx];
-        g.NT[d] = plan->ntiles[zyx];
+        g.ax[d].T = primary ? plan->itile[zyx] : plan->tile[zyx];
+        g.NT[d] = primary ? plan->nitiles[zyx] : plan->ntiles[zyx];
     }
-    g.ntiles = plan->ntiles_total;
+    g.ntiles = primary ? plan->nitiles_total : plan->ntiles_total;
     return g;
 }
 
@@ -200,77 +217,114 @@ struct DevBuf {
 
 }  // namespace
 
+namespace {
+
+// Stable radix sort of (key, sample) pairs, then per-bin offsets and the
+// sorted entry records (coordinates + sample index).
+int sort_entries(const BinGeom &geo, const float *coord, void *keys, void *vals, int64_t n,
+                 int32_t *bin_off, float4 **entries, hipStream_t stream) {
+    const int64_t nbins = geo.nframes * geo.ntiles;
+    DevBuf keys_sorted, vals_sorted, sort_tmp;
+    TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
+    TN_HIP(hipMalloc(&vals_sorted.p, sizeof(int32_t) * n));
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)nbins) ++end_bit;
+    size_t sort_bytes = 0;
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)keys,
+                                              (uint32_t *)keys_sorted.p, (int32_t *)vals,
+                                              (int32_t *)vals_sorted.p, (int)n, 0, end_bit,
+                                              stream));
+    TN_HIP(hipMalloc(&sort_tmp.p, sort_bytes));
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, sort_bytes, (uint32_t *)keys,
+                                              (uint32_t *)keys_sorted.p, (int32_t *)vals,
+                                              (int32_t *)vals_sorted.p, (int)n, 0, end_bit,
+                                              stream));
+    hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(n + 1)), dim3(kBlock), 0, stream,
+                       (const uint32_t *)keys_sorted.p, n, nbins, bin_off);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipMalloc(entries, sizeof(float4) * std::max<int64_t>(n, 1)));
+    hipLaunchKernelGGL(bin_gather_kernel, dim3(grid_blocks(n)), dim3(kBlock), 0, stream, geo,
+                       coord, (const uint32_t *)keys_sorted.p, (const int32_t *)vals_sorted.p, n,
+                       *entries);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipStreamSynchronize(stream));  // temporaries are freed on scope exit
+    return TN_OK;
+}
+
+}  // namespace
+
 int build_bins(tn_plan *plan, hipStream_t stream) {
     const BinGeom geo = make_geom(plan);
+    const BinGeom pgeo = make_geom(plan, true);
     const int64_t total = geo.npts * geo.nframes;
     const int64_t nbins = geo.nframes * geo.ntiles;
-    if (nbins >= (int64_t)UINT32_MAX) return fail(TN_E_RANGE, "too many tiles (%lld)", (long long)nbins);
+    const int64_t npbins = pgeo.nframes * pgeo.ntiles;
+    if (nbins >= (int64_t)UINT32_MAX || npbins >= (int64_t)UINT32_MAX)
+        return fail(TN_E_RANGE, "too many tiles (%lld)", (long long)std::max(nbins, npbins));
 
     TN_HIP(hipMalloc(&plan->d_bin_off, sizeof(int32_t) * (nbins + 1)));
+    TN_HIP(hipMalloc(&plan->d_pbin_off, sizeof(int32_t) * (npbins + 1)));
+    plan->h_pbin_off.assign(npbins + 1, 0);
+    plan->h_bin_off.assign(nbins + 1, 0);
     if (total == 0) {
         TN_HIP(hipMemsetAsync(plan->d_bin_off, 0, sizeof(int32_t) * (nbins + 1), stream));
+        TN_HIP(hipMemsetAsync(plan->d_pbin_off, 0, sizeof(int32_t) * (npbins + 1), stream));
+        TN_HIP(hipStreamSynchronize(stream));
         plan->nentries = 0;
         return TN_OK;
     }
 
-    DevBuf counts, offsets, scan_tmp;
-    TN_HIP(hipMalloc(&counts.p, sizeof(int32_t) * total));
-    TN_HIP(hipMalloc(&offsets.p, sizeof(int32_t) * (total + 1)));
-    hipLaunchKernelGGL(bin_count_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
-                       plan->d_coord, (int32_t *)counts.p);
-    TN_HIP(hipGetLastError());
-
-    // exclusive scan of the per-sample copy counts (total + 1 entries: the
-    // last one is the number of entries)
-    size_t tmp_bytes = 0;
-    TN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int32_t *)counts.p,
-                                            (int32_t *)offsets.p, (int)total, stream));
-    TN_HIP(hipMalloc(&scan_tmp.p, tmp_bytes));
-    TN_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tmp_bytes, (int32_t *)counts.p,
-                                            (int32_t *)offsets.p, (int)total, stream));
-    int32_t last_off = 0, last_cnt = 0;
-    TN_HIP(hipMemcpyAsync(&last_off, (int32_t *)offsets.p + (total - 1), sizeof(int32_t),
-                          hipMemcpyDeviceToHost, stream));
-    TN_HIP(hipMemcpyAsync(&last_cnt, (int32_t *)counts.p + (total - 1), sizeof(int32_t),
-                          hipMemcpyDeviceToHost, stream));
-    TN_HIP(hipStreamSynchronize(stream));
-    const int64_t n = (int64_t)last_off + last_cnt;
-    if (n >= INT32_MAX) return fail(TN_E_RANGE, "too many tile entries (%lld)", (long long)n);
-    plan->nentries = n;
-
-    DevBuf keys, vals, keys_sorted, vals_sorted, sort_tmp;
-    TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * n));
-    TN_HIP(hipMalloc(&vals.p, sizeof(int32_t) * n));
-    TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
-    TN_HIP(hipMalloc(&vals_sorted.p, sizeof(int32_t) * n));
-    hipLaunchKernelGGL(bin_emit_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
-                       plan->d_coord, (const int32_t *)offsets.p, (uint32_t *)keys.p,
-                       (int32_t *)vals.p);
-    TN_HIP(hipGetLastError());
-
-    int end_bit = 1;
-    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)nbins) ++end_bit;
-    size_t sort_bytes = 0;
-    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)keys.p,
-                                              (uint32_t *)keys_sorted.p, (int32_t *)vals.p,
-                                              (int32_t *)vals_sorted.p, (int)n, 0, end_bit,
-                                              stream));
-    TN_HIP(hipMalloc(&sort_tmp.p, sort_bytes));
-    TN_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, sort_bytes, (uint32_t *)keys.p,
-                                              (uint32_t *)keys_sorted.p, (int32_t *)vals.p,
-                                              (int32_t *)vals_sorted.p, (int)n, 0, end_bit,
-                                              stream));
-
-    hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(n + 1)), dim3(kBlock), 0, stream,
-                       (const uint32_t *)keys_sorted.p, n, nbins, plan->d_bin_off);
-    TN_HIP(hipGetLastError());
-    TN_HIP(hipMalloc(&plan->d_entries, sizeof(float4) * std::max<int64_t>(n, 1)));
-    hipLaunchKernelGGL(bin_gather_kernel, dim3(grid_blocks(n)), dim3(kBlock), 0, stream, geo,
-                       plan->d_coord, (const uint32_t *)keys_sorted.p,
-                       (const int32_t *)vals_sorted.p, n, plan->d_entries);
-    TN_HIP(hipGetLastError());
-    // temporaries are freed on scope exit; make sure the device is done
-    TN_HIP(hipStreamSynchronize(stream));
+    // ---- gridding bins: every tile a footprint touches --------------------
+    {
+        DevBuf counts, offsets, scan_tmp;
+        TN_HIP(hipMalloc(&counts.p, sizeof(int32_t) * total));
+        TN_HIP(hipMalloc(&offsets.p, sizeof(int32_t) * (total + 1)));
+        hipLaunchKernelGGL(bin_count_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
+                           plan->d_coord, (int32_t *)counts.p);
+        TN_HIP(hipGetLastError());
+        size_t tmp_bytes = 0;
+        TN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int32_t *)counts.p,
+                                                (int32_t *)offsets.p, (int)total, stream));
+        TN_HIP(hipMalloc(&scan_tmp.p, tmp_bytes));
+        TN_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tmp_bytes, (int32_t *)counts.p,
+                                                (int32_t *)offsets.p, (int)total, stream));
+        int32_t last_off = 0, last_cnt = 0;
+        TN_HIP(hipMemcpyAsync(&last_off, (int32_t *)offsets.p + (total - 1), sizeof(int32_t),
+                              hipMemcpyDeviceToHost, stream));
+        TN_HIP(hipMemcpyAsync(&last_cnt, (int32_t *)counts.p + (total - 1), sizeof(int32_t),
+                              hipMemcpyDeviceToHost, stream));
+        TN_HIP(hipStreamSynchronize(stream));
+        const int64_t n = (int64_t)last_off + last_cnt;
+        if (n >= INT32_MAX) return fail(TN_E_RANGE, "too many tile entries (%lld)", (long long)n);
+        plan->nentries = n;
+        DevBuf keys, vals;
+        TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * n));
+        TN_HIP(hipMalloc(&vals.p, sizeof(int32_t) * n));
+        hipLaunchKernelGGL(bin_emit_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
+                           plan->d_coord, (const int32_t *)offsets.p, (uint32_t *)keys.p,
+                           (int32_t *)vals.p);
+        TN_HIP(hipGetLastError());
+        int rc = sort_entries(geo, plan->d_coord, keys.p, vals.p, n, plan->d_bin_off,
+                              &plan->d_entries, stream);
+        if (rc) return rc;
+        plan->h_bin_off.assign(nbins + 1, 0);
+        TN_HIP(hipMemcpy(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1),
+                         hipMemcpyDeviceToHost));
+    }
+    // ---- interpolation bins: every sample once -----------------------------
+    {
+        DevBuf keys, vals;
+        TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * total));
+        TN_HIP(hipMalloc(&vals.p, sizeof(int32_t) * total));
+        hipLaunchKernelGGL(bin_primary_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream,
+                           pgeo, plan->d_coord, (uint32_t *)keys.p, (int32_t *)vals.p);
+        TN_HIP(hipGetLastError());
+        int rc = sort_entries(pgeo, plan->d_coord, keys.p, vals.p, total, plan->d_pbin_off,
+                              &plan->d_pentries, stream);
+        if (rc) return rc;
+        TN_HIP(hipMemcpy(plan->h_pbin_off.data(), plan->d_pbin_off, sizeof(int32_t) * (npbins + 1),
+                         hipMemcpyDeviceToHost));
+    }
     return TN_OK;
 }
 

@@ -154,7 +154,8 @@ class NativePlan:
     """
 
     def __init__(self, coord, shape, grid, width, beta, inv_scale, shift, device,
-                 apod=None, tile=None, interp_chunk=0, grid_chunk=0):
+                 apod=None, tile=None, interp_chunk=0, grid_chunk=0, itile=None, interp_mode=0,
+                 grid_split=0):
         lib = _lib.load()
         coord = np.ascontiguousarray(coord, dtype=np.float32)
         if coord.ndim != 3:
@@ -178,8 +179,11 @@ class NativePlan:
         p.beta[:] = [0.0] * pad + [float(b) for b in beta]
         p.shift[:] = [0] * pad + [int(s) for s in shift]
         p.tile[:] = [0, 0, 0] if tile is None else [1] * pad + [int(t) for t in tile]
+        p.itile[:] = [0, 0, 0] if itile is None else [1] * pad + [int(t) for t in itile]
+        p.interp_mode = int(interp_mode)
         p.interp_chunk = int(interp_chunk)
         p.grid_chunk = int(grid_chunk)
+        p.grid_split = int(grid_split)
         p.inv_scale = float(inv_scale)
         keep = []
         for d in range(ndim):

@@ -29,8 +29,11 @@ class PlanParams(ctypes.Structure):
         ("beta", c_float * 3),
         ("shift", c_int32 * 3),
         ("tile", c_int32 * 3),
+        ("itile", c_int32 * 3),
+        ("interp_mode", c_int32),
         ("interp_chunk", c_int32),
         ("grid_chunk", c_int32),
+        ("grid_split", c_int32),
         ("inv_scale", c_float),
         ("apod", c_void_p * 3),
     ]

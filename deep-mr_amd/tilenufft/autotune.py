@@ -22,9 +22,12 @@ from .fft.nufft import retune
 def default_candidates(ndim):
     if ndim == 2:
         tiles = [(16, 16), (32, 32), (32, 64), (64, 32), (64, 64), (16, 64)]
+        itiles = [(16, 16), (32, 32), (16, 64), (32, 64)]
     else:
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
-    return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16])
+        itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
+    return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16],
+                itiles=itiles, staged_chunks=[1, 2, 4, 8])
 
 
 def _cache_path():
@@ -53,15 +56,15 @@ def _time(fn, reps, device):
     return float(np.median([s.elapsed_time(e) for s, e in zip(start, end)]))
 
 
-def _apply(plans, tile, ic, gc):
+def _apply(plans, tile, ic, gc, mode=0, itile=None):
     for p in plans:
-        retune(p, tile=tile, interp_chunk=ic, grid_chunk=gc)
+        retune(p, tile=tile, interp_chunk=ic, grid_chunk=gc, interp_mode=mode, itile=itile)
 
 
 def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=False):
     """Pick tile / chunk sizes for ``plans`` (rebuilt in place) and return them.
 
-    Returns dict(tile, grid_chunk, interp_chunk, grid_ms, interp_ms).
+    Returns dict(tile, grid_chunk, interp_mode, itile, interp_chunk, grid_ms, interp_ms).
     """
     ncoils = [int(ncoils)] * len(plans) if np.isscalar(ncoils) else list(ncoils)
     key = _key(plans, ncoils)
@@ -75,7 +78,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
             cache = {}
     if use_cache and key in cache:
         best = cache[key]
-        _apply(plans, tuple(best["tile"]), best["interp_chunk"], best["grid_chunk"])
+        _apply(plans, tuple(best["tile"]), best["interp_chunk"], best["grid_chunk"],
+               best.get("interp_mode", 0), tuple(best["itile"]) if best.get("itile") else None)
         return dict(best, cached=True)
 
     dev = plans[0].interpolator.device
@@ -104,19 +108,28 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
             results.append(("grid_chunk", gc, t))
             if t < best_gt:
                 best_gc, best_gt = gc, t
-        # 3) coils per interpolation item
-        best_ic, best_it = 0, float("inf")
-        for ic in [0] + list(cand["interp_chunks"]):
-            _apply(plans, best_tile, ic, best_gc)
-            b = BatchedNUFFT(plans, ncoils)
-            t = _time(lambda: b.interp(b.grid), reps, dev)
-            results.append(("interp_chunk", ic, t))
+        # 3) interpolation: gather variant (coils per item) and LDS-staged
+        #    variant (tile x coils per item)
+        interp_cands = [(1, None, ic) for ic in [0] + list(cand["interp_chunks"])]
+        interp_cands += [(2, it, ic) for it in cand.get("itiles", []) for ic in cand.get("staged_chunks", [])]
+        best_i, best_it = None, float("inf")
+        for mode, itile, ic in interp_cands:
+            try:
+                _apply(plans, best_tile, ic, best_gc, mode, itile)
+                b = BatchedNUFFT(plans, ncoils)
+                t = _time(lambda: b.interp(b.grid), reps, dev)
+            except RuntimeError as exc:  # e.g. a tile too large for LDS
+                results.append(("interp", (mode, itile, ic), str(exc)))
+                continue
+            results.append(("interp", (mode, itile, ic), t))
             if verbose:
-                print(f"autotune interp_chunk {ic}: interp {t:.3f} ms", flush=True)
+                print(f"autotune interp mode={mode} itile={itile} chunk={ic}: {t:.3f} ms", flush=True)
             if t < best_it:
-                best_ic, best_it = ic, t
-        _apply(plans, best_tile, best_ic, best_gc)
-    best = dict(tile=list(best_tile), grid_chunk=best_gc, interp_chunk=best_ic, grid_ms=best_gt,
+                best_i, best_it = (mode, itile, ic), t
+        mode, itile, best_ic = best_i
+        _apply(plans, best_tile, best_ic, best_gc, mode, itile)
+    best = dict(tile=list(best_tile), grid_chunk=best_gc, interp_mode=mode,
+                itile=list(itile) if itile else None, interp_chunk=best_ic, grid_ms=best_gt,
                 interp_ms=best_it)
     if use_cache:
         cache[key] = best

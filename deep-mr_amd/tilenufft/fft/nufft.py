@@ -62,6 +62,9 @@ class NUFFTPlan:
     tile: tuple = None
     interp_chunk: int = 0
     grid_chunk: int = 0
+    itile: tuple = None
+    interp_mode: int = 0
+    grid_split: int = 0
 
     def to(self, device):
         """Move the device plan to another HIP device ("cpu" keeps it where it is)."""
@@ -88,14 +91,22 @@ def _build_native(plan, device, shift):
     return _core.NativePlan(
         plan.coord, plan.shape, plan.os_shape, plan.width, plan.beta, 1.0 / plan.scale, sh,
         device, apod=apod, tile=plan.tile, interp_chunk=plan.interp_chunk,
-        grid_chunk=plan.grid_chunk,
+        grid_chunk=plan.grid_chunk, itile=plan.itile, interp_mode=plan.interp_mode,
+        grid_split=plan.grid_split,
     )
 
 
-def retune(plan, tile=None, interp_chunk=None, grid_chunk=None):
+def retune(plan, tile=None, interp_chunk=None, grid_chunk=None, itile=None, interp_mode=None,
+           grid_split=None):
     """Rebuild the device plan of ``plan`` with other launch parameters."""
+    if grid_split is not None:
+        plan.grid_split = int(grid_split)
     if tile is not None:
         plan.tile = tuple(int(t) for t in tile)
+    if itile is not None:
+        plan.itile = tuple(int(t) for t in itile)
+    if interp_mode is not None:
+        plan.interp_mode = int(interp_mode)
     if interp_chunk is not None:
         plan.interp_chunk = int(interp_chunk)
     if grid_chunk is not None:
@@ -115,14 +126,17 @@ def _expand_coord(coord):
 
 
 def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, interp_chunk=0,
-               grid_chunk=0):
+               grid_chunk=0, itile=None, interp_mode=0, grid_split=0):
     """Precompute a NUFFT (deepmr ``plan_nufft``, src/deepmr/fft/nufft.py:27-132).
 
     Parameters follow deepmr: ``coord`` (ncontrasts, nviews, nsamples, ndim)
     in [-N/2, N/2) with x first; ``shape``, ``width``, ``oversamp`` scalar or
     (z, y, x). ``device`` selects the HIP device ("cpu" = current HIP device).
-    ``tile`` (z, y, x), ``interp_chunk`` and ``grid_chunk`` (coils per work
-    item) optionally override the launch parameters (0 / None = automatic).
+    ``tile`` / ``itile`` (gridding / staged-interpolation tiles, (z, y, x)),
+    ``interp_mode`` (1 gather, 2 LDS-staged), ``interp_chunk`` and
+    ``grid_chunk`` (coils per work item) and ``grid_split`` (max samples per
+    gridding workgroup before a hot tile is split; < 0 never) optionally
+    override the launch parameters (0 / None = automatic).
     """
     coord = torch.as_tensor(coord).detach().cpu().to(torch.float32).numpy().copy()
     ndim = coord.shape[-1]
@@ -164,6 +178,9 @@ def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, in
         tile=None if tile is None else tuple(int(t) for t in tile),
         interp_chunk=int(interp_chunk),
         grid_chunk=int(grid_chunk),
+        itile=None if itile is None else tuple(int(t) for t in itile),
+        interp_mode=int(interp_mode),
+        grid_split=int(grid_split),
     )
     plan.interpolator = _build_native(plan, dev, shift=True)
     return plan

@@ -58,8 +58,13 @@ typedef struct tn_plan_params {
                            NUFFT uses G - G/2 (folds the centring shifts), the
                            plain interpolator 0                                 */
     int32_t tile[3];    /* gridding tile; 0 = automatic                         */
+    int32_t itile[3];   /* staged-interpolation tile; 0 = automatic             */
+    int32_t interp_mode;  /* 0 automatic, 1 gather through L1/L2, 2 staged in LDS */
     int32_t interp_chunk; /* coils per interpolation work item; 0 = automatic  */
     int32_t grid_chunk;   /* coils per gridding workgroup; 0 = automatic (LDS) */
+    int32_t grid_split;   /* max samples per gridding workgroup before a hot tile
+                             is split across workgroups (partials folded in a
+                             fixed order); 0 = automatic, < 0 = never split   */
     float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
     const float *apod[3]; /* host apodization vectors of length shape[d], or
                              NULL for ones (src/deepmr/fft/nufft.py:423-447)   */

@@ -208,3 +208,34 @@ def test_numpy_in_numpy_out_and_real_input():
     y = tilenufft.fft.nufft(x, coord, device=DEV, oversamp=2.0)
     assert isinstance(y, np.ndarray) and y.dtype == np.complex64
     assert rel(y, orc.nufft(x, coord, n, 4, 2.0)) < RTOL
+
+
+@pytest.mark.parametrize("ndim, mode, itile", [(2, 1, None), (2, 2, None), (2, 2, (16, 16)),
+                                               (2, 2, (7, 13)), (3, 2, None), (3, 2, (4, 8, 8)),
+                                               (3, 1, None), (3, 2, (5, 3, 7))])
+def test_interp_variants_match_oracle(ndim, mode, itile):
+    # gathering and LDS-staged interpolation, including tiles that do not
+    # divide the grid (partial edge tiles, regions wrapping around the grid)
+    n = 36 if ndim == 2 else 14
+    print(f"interp variant ndim={ndim} mode={mode} itile={itile}", flush=True)
+    coord = testing.golden_angle_radial(24, 72, n) if ndim == 2 else testing.kooshball(50, 28, n)
+    x = cnormal((3,) + (n,) * ndim, 31)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 1.5, device=DEV, interp_mode=mode, itile=itile)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    assert rel(y, orc.apply_nufft(x, orc.plan_nufft(coord, n, 4, 1.5))) < RTOL
+
+
+@pytest.mark.parametrize("split", [-1, 3, 64, 500])
+def test_grid_split_matches_oracle_and_is_deterministic(split):
+    # hot tiles split across workgroups, partials folded in a fixed order
+    n = 40
+    print(f"gridding split={split}", flush=True)
+    coord = testing.vd_spiral(12, 300, n)  # dense centre -> hot tiles
+    k = cnormal((3, 12, 300), 41)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV, grid_split=split)
+    kt = torch.from_numpy(k).to(DEV)
+    a = tilenufft.fft.apply_nufft_adj(kt, plan)
+    b = tilenufft.fft.apply_nufft_adj(kt, plan)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+    assert rel(a.cpu().numpy(), orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0))) < RTOL
