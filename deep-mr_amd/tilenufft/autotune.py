@@ -27,7 +27,7 @@ def default_candidates(ndim):
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
         itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
     return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16],
-                itiles=itiles, staged_chunks=[1, 2, 4, 8])
+                itiles=itiles, staged_chunks=[1, 2, 4, 8], grid_splits=[512, 1024, 4096, -1])
 
 
 def _cache_path():
@@ -56,9 +56,10 @@ def _time(fn, reps, device):
     return float(np.median([s.elapsed_time(e) for s, e in zip(start, end)]))
 
 
-def _apply(plans, tile, ic, gc, mode=0, itile=None):
+def _apply(plans, tile, ic, gc, mode=0, itile=None, split=0):
     for p in plans:
-        retune(p, tile=tile, interp_chunk=ic, grid_chunk=gc, interp_mode=mode, itile=itile)
+        retune(p, tile=tile, interp_chunk=ic, grid_chunk=gc, interp_mode=mode, itile=itile,
+               grid_split=split)
 
 
 def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=False):
@@ -79,7 +80,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
     if use_cache and key in cache:
         best = cache[key]
         _apply(plans, tuple(best["tile"]), best["interp_chunk"], best["grid_chunk"],
-               best.get("interp_mode", 0), tuple(best["itile"]) if best.get("itile") else None)
+               best.get("interp_mode", 0), tuple(best["itile"]) if best.get("itile") else None,
+               best.get("grid_split", 0))
         return dict(best, cached=True)
 
     dev = plans[0].interpolator.device
@@ -108,6 +110,17 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
             results.append(("grid_chunk", gc, t))
             if t < best_gt:
                 best_gc, best_gt = gc, t
+        # 2b) hot-tile split threshold
+        best_split = 0
+        for sp in cand.get("grid_splits", []):
+            _apply(plans, best_tile, 0, best_gc, split=sp)
+            b = BatchedNUFFT(plans, ncoils)
+            t = _time(lambda: b.gridding(b.grid), reps, dev)
+            results.append(("grid_split", sp, t))
+            if verbose:
+                print(f"autotune grid_split {sp}: grid {t:.3f} ms", flush=True)
+            if t < best_gt:
+                best_split, best_gt = sp, t
         # 3) interpolation: gather variant (coils per item) and LDS-staged
         #    variant (tile x coils per item)
         interp_cands = [(1, None, ic) for ic in [0] + list(cand["interp_chunks"])]
@@ -115,7 +128,7 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         best_i, best_it = None, float("inf")
         for mode, itile, ic in interp_cands:
             try:
-                _apply(plans, best_tile, ic, best_gc, mode, itile)
+                _apply(plans, best_tile, ic, best_gc, mode, itile, best_split)
                 b = BatchedNUFFT(plans, ncoils)
                 t = _time(lambda: b.interp(b.grid), reps, dev)
             except RuntimeError as exc:  # e.g. a tile too large for LDS
@@ -127,8 +140,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
             if t < best_it:
                 best_i, best_it = (mode, itile, ic), t
         mode, itile, best_ic = best_i
-        _apply(plans, best_tile, best_ic, best_gc, mode, itile)
-    best = dict(tile=list(best_tile), grid_chunk=best_gc, interp_mode=mode,
+        _apply(plans, best_tile, best_ic, best_gc, mode, itile, best_split)
+    best = dict(tile=list(best_tile), grid_chunk=best_gc, grid_split=best_split, interp_mode=mode,
                 itile=list(itile) if itile else None, interp_chunk=best_ic, grid_ms=best_gt,
                 interp_ms=best_it)
     if use_cache:
