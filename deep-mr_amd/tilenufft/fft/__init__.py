@@ -11,9 +11,12 @@ __all__ = [
     "NUFFTPlan",
     "set_autograd_mode",
     "get_autograd_mode",
+    "plan_toeplitz_nufft",
+    "apply_nufft_selfadj",
 ]
 
 from .fft import fft, ifft  # noqa: F401
+from .toeplitz import apply_nufft_selfadj, plan_toeplitz_nufft  # noqa: F401
 from .nufft import (  # noqa: F401
     NUFFTPlan,
     apply_nufft,

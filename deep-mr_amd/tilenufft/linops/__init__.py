@@ -2,4 +2,4 @@
 
 from .base import Add, Compose, Identity, Linop, Multiply  # noqa: F401
 from .coil import SenseAdjointOp, SenseOp  # noqa: F401
-from .nufft import NUFFTAdjointOp, NUFFTOp  # noqa: F401
+from .nufft import NUFFTAdjointOp, NUFFTGramOp, NUFFTOp  # noqa: F401

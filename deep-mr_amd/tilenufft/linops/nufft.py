@@ -5,7 +5,7 @@ directions (fused into the native kernels when it has the trajectory's
 shape); ``.H`` shares the device plan.
 """
 
-__all__ = ["NUFFTOp", "NUFFTAdjointOp"]
+__all__ = ["NUFFTOp", "NUFFTAdjointOp", "NUFFTGramOp"]
 
 import torch
 
@@ -76,3 +76,20 @@ def _dev(op, device):
     if plan is not None:
         return plan.interpolator.device
     return None if device is None or torch.device(device).type == "cpu" else device
+
+
+class NUFFTGramOp(base.Linop):
+    """Self-adjoint NUFFT (A^H A) through a Toeplitz kernel
+    (src/deepmr/linops/nufft.py:189-241)."""
+
+    def __init__(self, coord, shape, basis=None, weight=None, device="cpu", threadsperblock=128,
+                 width=6, **kwargs):
+        super().__init__(ndim=coord.shape[-1])
+        self.toeplitz_kern = _fft.plan_toeplitz_nufft(coord, shape, basis, weight, width, device)
+        self.threadsperblock = threadsperblock
+
+    def forward(self, x):
+        return _fft.apply_nufft_selfadj(x, self.toeplitz_kern, threadsperblock=self.threadsperblock)
+
+    def _adjoint_linop(self):
+        return self

@@ -195,33 +195,48 @@ class ReconServer:
 
     # ---- one step ------------------------------------------------------------
     def _build_layout(self):
-        """Arenas and batched launch tables for the current active set."""
+        """Arenas, stacked CG state and batched launch tables for the active set.
+
+        Requests with the same matrix size form a group: their coil images,
+        padded grids and CG vectors are stacked so that each step costs a
+        fixed number of launches per group, and all groups share one
+        interpolation and one gridding launch.
+        """
+        self._store_state()
         act = sorted(self.active, key=lambda s: (s.req.shape, s.req.rid))
+        dev = self.device
         groups = collections.OrderedDict()
         for s in act:
             groups.setdefault(s.req.shape, []).append(s)
-        dev = self.device
-        img_rows = sum(s.req.ncoils for s in act)
-        # per-geometry contiguous slices of the arenas
-        gsizes = {n: math.ceil(np.float32(self.oversamp) * n) for n in groups}
-        grid_elems = sum(s.req.ncoils * gsizes[s.req.shape] ** 2 for s in act)
+        grid_elems = sum(s.req.ncoils * math.ceil(np.float32(self.oversamp) * s.req.shape) ** 2 for s in act)
         ksp_elems = sum(s.req.ncoils * s.req.npts for s in act)
-        lay = dict(order=act, groups=groups, gsizes=gsizes)
+        lay = dict(order=act, groups=[])
         lay["gridA"] = torch.empty(grid_elems, dtype=torch.complex64, device=dev)
         lay["gridB"] = torch.empty(grid_elems, dtype=torch.complex64, device=dev)
         lay["kspace"] = torch.empty(max(ksp_elems, 1), dtype=torch.complex64, device=dev)
-        lay["images"] = {}
-        goff, koff = 0, 0
+        goff = koff = 0
         offs = {}
         for n, members in groups.items():
-            g = gsizes[n]
+            g = math.ceil(np.float32(self.oversamp) * n)
             rows = sum(s.req.ncoils for s in members)
-            lay["images"][n] = torch.empty((rows, n, n), dtype=torch.complex64, device=dev)
-            lay.setdefault("gslice", {})[n] = (goff, goff + rows * g * g)
+            row2req = torch.cat([torch.full((s.req.ncoils,), i, dtype=torch.long)
+                                 for i, s in enumerate(members)]).to(dev)
+            init = torch.tensor([s.it < 0 for s in members], device=dev)
+            grp = dict(n=n, g=g, members=members, rows=rows, row2req=row2req, init=init,
+                       any_init=bool(init.any()), any_cg=bool((~init).any()),
+                       gslice=(goff, goff + rows * g * g),
+                       smaps=torch.cat([s.smaps for s in members]),
+                       images=torch.empty((rows, n, n), dtype=torch.complex64, device=dev))
+            grp["X"] = torch.stack([s.x for s in members])
+            grp["R"] = torch.stack([s.r if s.r is not None else s.x for s in members])
+            grp["P"] = torch.stack([s.p if s.p is not None else s.x for s in members])
+            grp["rs"] = torch.stack([s.rs_old if s.rs_old is not None else
+                                     torch.ones((), device=dev) for s in members]).reshape(-1)
             for s in members:
                 offs[s.req.rid] = (goff, koff)
                 goff += s.req.ncoils * g * g
                 koff += s.req.ncoils * s.req.npts
+            lay["groups"].append(grp)
         lay["offs"] = offs
         lay["grid_batch"] = _NativeBatch([s.plan for s in act], [s.req.ncoils for s in act],
                                          [offs[s.req.rid][0] for s in act],
@@ -230,8 +245,21 @@ class ReconServer:
         lay["interp_batch"] = (_NativeBatch([s.plan for s in cg], [s.req.ncoils for s in cg],
                                             [offs[s.req.rid][0] for s in cg],
                                             [offs[s.req.rid][1] for s in cg], dev) if cg else None)
-        del img_rows
         return lay
+
+    def _store_state(self):
+        """Write the stacked CG state of the current layout back to the requests."""
+        lay = self._layout
+        if lay is None:
+            return
+        for grp in lay["groups"]:
+            for i, s in enumerate(grp["members"]):
+                if s.it < 0 or s not in self.active:
+                    continue
+                s.x, s.r, s.p = grp["X"][i].clone(), grp["R"][i].clone(), grp["P"][i].clone()
+                s.rs_old = grp["rs"][i].clone()
+        self._layout = None
+        self._layout_key = None
 
     def _layout_for_step(self):
         key = tuple((s.req.rid, s.it >= 0) for s in sorted(self.active, key=lambda s: (s.req.shape, s.req.rid)))
@@ -249,72 +277,85 @@ class ReconServer:
         if not self.active:
             return False
         lay = self._layout_for_step()
-        dev = self.device
-        with torch.cuda.device(dev):
-            # coil expansion of the search directions (CG requests)
-            for n, members in lay["groups"].items():
-                img = lay["images"][n]
-                row = 0
-                for s in members:
-                    c = s.req.ncoils
-                    if s.it >= 0:
-                        torch.mul(s.smaps, s.p, out=img[row:row + c])
-                    row += c
-            # forward NUFFT of every CG request: pad + FFT per geometry, one interp launch
+        with torch.cuda.device(self.device):
+            # forward NUFFT of the search directions (CG requests): coil
+            # expansion, pad + FFT per group, one interpolation launch
             if lay["interp_batch"] is not None:
-                for n, members in lay["groups"].items():
-                    a, b = lay["gslice"][n]
-                    g = lay["gsizes"][n]
+                for grp in lay["groups"]:
+                    a, b = grp["gslice"]
+                    g = grp["g"]
+                    if grp["any_cg"]:
+                        torch.mul(grp["smaps"], grp["P"].index_select(0, grp["row2req"]), out=grp["images"])
                     ga = lay["gridA"][a:b].view(-1, g, g)
-                    members[0].plan.interpolator.apod_pad(lay["images"][n], ga, ga.shape[0])
+                    grp["members"][0].plan.interpolator.apod_pad(grp["images"], ga, grp["rows"])
                     torch.fft.fftn(ga, dim=(-2, -1), out=lay["gridB"][a:b].view(-1, g, g))
                 lay["interp_batch"].interp(lay["gridB"], lay["kspace"])
-            # measured data of requests that start this step
-            for s in lay["order"]:
-                if s.it < 0:
-                    _, ko = lay["offs"][s.req.rid]
-                    lay["kspace"][ko:ko + s.y.numel()].copy_(s.y.reshape(-1))
-            # adjoint of every request: one gridding launch, IFFT + crop per geometry
+            # measured k-space of requests starting this step
+            for grp in lay["groups"]:
+                if grp["any_init"]:
+                    for s in grp["members"]:
+                        if s.it < 0:
+                            _, ko = lay["offs"][s.req.rid]
+                            lay["kspace"][ko:ko + s.y.numel()].copy_(s.y.reshape(-1))
+            # adjoint of every request: one gridding launch, IFFT + crop per group
             lay["grid_batch"].grid(lay["kspace"], lay["gridA"])
-            for n, members in lay["groups"].items():
-                a, b = lay["gslice"][n]
-                g = lay["gsizes"][n]
+            for grp in lay["groups"]:
+                a, b = grp["gslice"]
+                g = grp["g"]
                 gb = lay["gridB"][a:b].view(-1, g, g)
                 torch.fft.ifftn(lay["gridA"][a:b].view(-1, g, g), dim=(-2, -1), out=gb)
-                members[0].plan.interpolator.crop_apod(gb, lay["images"][n], gb.shape[0])
-            # coil combination + CG update per request
+                grp["members"][0].plan.interpolator.crop_apod(gb, grp["images"], grp["rows"])
+                self._cg_update(grp)
             finished = []
-            for n, members in lay["groups"].items():
-                img = lay["images"][n]
-                row = 0
-                for s in members:
-                    c = s.req.ncoils
-                    v = (s.smaps.conj() * img[row:row + c]).sum(0)
-                    row += c
-                    if s.it < 0:
-                        s.r = v
-                        s.p = v.clone()
-                        s.rs_old = torch.real(torch.vdot(v.reshape(-1), v.reshape(-1)))
-                        s.it = 0
-                    else:
-                        ap = v
-                        alpha = s.rs_old / torch.vdot(s.p.reshape(-1), ap.reshape(-1))
-                        s.x = s.x + s.p * alpha
-                        s.r = s.r + ap * (-alpha)
-                        rs_new = torch.real(torch.vdot(s.r.reshape(-1), s.r.reshape(-1)))
-                        s.p = s.r + s.p * (rs_new / s.rs_old)
-                        s.rs_old = rs_new
-                        s.it += 1
-                        if s.it >= s.req.niter:
-                            finished.append(s)
+            for grp in lay["groups"]:
+                for s in grp["members"]:
+                    s.it += 1
+                    if s.it >= s.req.niter:
+                        finished.append(s)
             if finished:
-                torch.cuda.synchronize(dev)
+                torch.cuda.synchronize(self.device)
+                self._store_state()
                 for s in finished:
                     self._finish(s)
         self.steps += 1
         return True
 
+    @staticmethod
+    def _cg_update(grp):
+        """Coil combination and one CGStep (src/deepmr/optim/cg.py:203-212) for
+        every request of the group; requests starting this step set
+        r = p = E^H y instead."""
+        nreq = len(grp["members"])
+        n = grp["n"]
+        ap = torch.zeros((nreq, n, n), dtype=torch.complex64, device=grp["images"].device)
+        ap.index_add_(0, grp["row2req"], grp["smaps"].conj() * grp["images"])
+        X, R, P, rs = grp["X"], grp["R"], grp["P"], grp["rs"]
+        if grp["any_cg"]:
+            alpha = rs / (P.conj() * ap).sum(dim=(-2, -1))
+            a3 = alpha[:, None, None]
+            x_new = X + P * a3
+            r_new = R + ap * (-a3)
+            rs_new = torch.real((r_new.conj() * r_new).sum(dim=(-2, -1)))
+            p_new = r_new + P * (rs_new / rs)[:, None, None]
+        if grp["any_init"]:
+            rs_init = torch.real((ap.conj() * ap).sum(dim=(-2, -1)))
+            if grp["any_cg"]:
+                m = grp["init"]
+                m3 = m[:, None, None]
+                x_new = torch.where(m3, torch.zeros_like(X), x_new)
+                r_new = torch.where(m3, ap, r_new)
+                p_new = torch.where(m3, ap, p_new)
+                rs_new = torch.where(m, rs_init, rs_new)
+            else:
+                x_new, r_new, p_new, rs_new = torch.zeros_like(X), ap, ap.clone(), rs_init
+        X.copy_(x_new)
+        R.copy_(r_new)
+        P.copy_(p_new)
+        rs.copy_(rs_new)
+
     def _finish(self, s, error=None):
+        if self._layout is not None:
+            self._store_state()
         self.active.remove(s)
         self.reserved -= s.bytes
         s.req.result = None if error else s.x

@@ -530,3 +530,38 @@ def nufft(image, coord, shape=None, width=4, oversamp=1.25, basis_adjoint=None):
 def nufft_adj(kspace, coord, shape, width=4, oversamp=1.25, basis=None):
     """dm/fft/__init__.py:221-286."""
     return apply_nufft_adj(kspace, plan_nufft(coord, shape, width, oversamp), basis=basis)
+
+
+# --------------------------------------------------------------------------
+# Toeplitz normal operator (dm/fft/toeplitz.py:15-157, dm/fft/nufft.py:609-669)
+# --------------------------------------------------------------------------
+def plan_toeplitz(coord, shape, dcf=None, width=4):
+    """PSF kernel on the 2x grid (non-low-rank), dm/fft/toeplitz.py:15-157."""
+    coord = np.asarray(coord, np.float32)
+    ndim = coord.shape[-1]
+    oversamp = np.asarray([2.0] * ndim)
+    shape = np.asarray([shape] * ndim if np.isscalar(shape) else list(shape)[-ndim:], np.int16)
+    dcf = np.ones(coord.shape[:-1], np.float32) if dcf is None else np.asarray(dcf, np.float32)
+    is_cart = cartesian_axes(coord, shape)
+    oversamp[is_cart] = 1.0
+    kshape = np.ceil(oversamp * shape).astype(np.int16)
+    kcoord = (coord * oversamp.astype(np.float32)).astype(np.float32)
+    psf = nufft_adj(dcf.astype(np.complex64), kcoord, [int(s) for s in kshape], width=width)
+    ax = tuple(range(-ndim, 0))
+    kern = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(psf, axes=ax), axes=ax, norm="ortho"), axes=ax)
+    if kern.shape[0] == 1:
+        kern = kern[0]
+    kern = np.fft.ifftshift(kern, axes=ax)
+    kern = kern / np.mean(np.abs(kern[kern != 0]))
+    return np.nan_to_num(kern).astype(np.complex64), tuple(int(s) for s in kshape)
+
+
+def apply_selfadj(image, kern, kshape):
+    """Pad -> FFT (ortho) -> kernel product -> IFFT (ortho) -> crop (dm/fft/nufft.py:609-669)."""
+    x = np.asarray(image).astype(np.complex64)
+    ndim = len(kshape)
+    oshape = x.shape[-ndim:]
+    x = resize(x, list(x.shape[:-ndim]) + list(kshape))
+    ax = tuple(range(-ndim, 0))
+    y = np.fft.ifftn(kern * np.fft.fftn(x, axes=ax, norm="ortho"), axes=ax, norm="ortho")
+    return resize(y, list(y.shape[:-ndim]) + list(oshape)).astype(np.complex64)
