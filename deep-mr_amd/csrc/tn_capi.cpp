@@ -122,17 +122,19 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                             LaunchTables &t) {
     t.kind_interp = interp_kind(jobs);
     t.kind_grid = t.kind_interp;
-    int64_t max_cells = 1, max_region = 1;
+    int64_t max_plane = 1, max_region = 1;
     for (const Job &j : jobs) {
-        max_cells = std::max<int64_t>(max_cells, (int64_t)j.T[0] * j.T[1] * j.T[2]);
+        max_plane = std::max<int64_t>(max_plane, (int64_t)j.T[2] * j.T[1] * (j.T[0] + kGridPad) + kWave);
         max_region = std::max<int64_t>(max_region, (int64_t)(j.IT[0] + j.W[0] - 1) *
                                                        (j.IT[1] + j.W[1] - 1) * (j.IT[2] + j.W[2] - 1));
     }
-    const size_t per_coil = (size_t)max_cells * 2 * sizeof(float);
+    const int wc = t.kind_grid % 16;
+    if (grid_lds_bytes(1, max_plane, wc) > kGridLdsBudget)
+        return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
+                    grid_lds_bytes(1, max_plane, wc));
+    int lds_coils = 1;
+    while (lds_coils < 64 && grid_lds_bytes(lds_coils + 1, max_plane, wc) <= kGridLdsBudget) ++lds_coils;
     const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
-    if (per_coil > kGridLdsBudget)
-        return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes per coil)", per_coil);
-    const int lds_coils = (int)std::max<size_t>(1, kGridLdsBudget / per_coil);
     const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
 
     int mode = plan_of.empty() ? kInterpGather : plan_of[0]->p.interp_mode;
@@ -150,7 +152,9 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, lds_coils), lds_coils);
         max_gc = std::max(max_gc, gc);
         if (mode == kInterpStaged) {
-            const int ic = std::min(chunk_for(pl->p.interp_chunk, j.nbatch, 4), region_coils);
+            // one item per (tile, segment) covers every coil; the region is
+            // loaded `ic` coils at a time
+            const int ic = std::min(chunk_for(pl->p.interp_chunk, j.nbatch, region_coils), region_coils);
             max_ic = std::max(max_ic, ic);
             const int64_t nt = (int64_t)j.INT_[0] * j.INT_[1] * j.INT_[2];
             const int64_t f = (j.pbin_off - pl->d_pbin_off) / std::max<int64_t>(pl->nitiles_total, 1);
@@ -158,8 +162,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             for (int64_t tl = 0; tl < nt; ++tl) {
                 for (int32_t e0 = off[tl]; e0 < off[tl + 1]; e0 += kInterpSegment) {
                     const int32_t e1 = std::min<int32_t>(off[tl + 1], e0 + kInterpSegment);
-                    for (int c0 = 0; c0 < j.nbatch; c0 += ic)
-                        interp.push_back(Item{(int)ji, (int)tl, c0, std::min(j.nbatch, c0 + ic), e0, e1, 0, 0});
+                    interp.push_back(Item{(int)ji, (int)tl, 0, j.nbatch, e0, e1, 0, 0});
                 }
             }
         } else {
@@ -174,7 +177,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         const int32_t *goff = pl->h_bin_off.data() + fg * pl->ntiles_total;
         const int split = pl->p.grid_split == 0 ? kGridSplitDefault
                           : pl->p.grid_split < 0 ? INT32_MAX : pl->p.grid_split;
-        const int64_t cells = (int64_t)j.T[0] * j.T[1] * j.T[2];
+        const int64_t cells = (int64_t)j.T[0] * j.T[1] * j.T[2];  // partial size per coil
         for (int64_t tl = 0; tl < ntiles; ++tl) {
             const int32_t b0 = goff[tl], b1 = goff[tl + 1];
             const int nparts = (int)std::max<int64_t>(1, ((int64_t)(b1 - b0) + split - 1) / split);
@@ -198,8 +201,9 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     }
     t.scratch_floats = scratch;
     t.grid_coils = max_gc;
-    t.grid_lds = per_coil * (size_t)max_gc;
+    t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc);
     t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
+    t.interp_coils = max_ic;
     return upload_tables(jobs, interp, grid, fold, slot_base, t);
 }
 

@@ -18,8 +18,17 @@ constexpr int kMaxWidth = 16;      // largest kernel width supported per axis
 constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
 constexpr size_t kGridLdsBudget = 64 * 1024;  // LDS per gridding workgroup
 constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation workgroup
-constexpr int kInterpSegment = 2048;  // max samples per staged-interpolation work item
+constexpr int kStagedEpt = 2;  // samples per lane in the staged interpolation
+constexpr int kInterpSegment = 256 * kStagedEpt;  // samples per staged work item
 constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
+constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch
+constexpr int kGridPad = 4;      // accumulator row padding (floats)
+
+// Dynamic LDS bytes of the gridding kernel for `ncoil` coils of a tile.
+inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc) {
+    return sizeof(float) * (2 * (size_t)ncoil * plane + 4 * (size_t)kGridBatch +
+                            3 * (size_t)wc * kGridBatch + 2 * (size_t)ncoil * kGridBatch);
+}
 
 // Interpolation variants.
 enum InterpMode { kInterpAuto = 0, kInterpGather = 1, kInterpStaged = 2 };
@@ -81,6 +90,7 @@ struct LaunchTables {
     int kind_grid = 0;
     int interp_mode = kInterpGather;
     size_t interp_lds = 0;     // dynamic LDS bytes for the staged interpolation
+    int interp_coils = 1;      // coils per LDS region load (staged interpolation)
 };
 
 }  // namespace tn

@@ -16,12 +16,12 @@ __device__ __forceinline__ float kb_eval(float u, float beta) {
     if (r < 0.0f) return 0.0f;
     const float x = beta * sqrtf(r);
     if (x < 3.75f) {
-        const float t = x / 3.75f;
+        const float t = x * (1.0f / 3.75f);
         const float t2 = t * t;
         return 1.0f + t2 * (3.5156229f + t2 * (3.0899424f + t2 * (1.2067492f +
                t2 * (0.2659732f + t2 * (0.0360768f + t2 * 0.0045813f)))));
     }
-    const float q = 3.75f / x;  // 1 / t
+    const float q = 3.75f * __builtin_amdgcn_rcpf(x);  // 1 / t
     const float poly = 0.39894228f + q * (0.01328592f + q * (0.00225319f +
                        q * (-0.00157565f + q * (0.00916281f + q * (-0.02057706f +
                        q * (0.02635537f + q * (-0.01647633f + q * 0.00392377f)))))));
@@ -52,12 +52,13 @@ __device__ __forceinline__ void axis_taps(float g, int W, int G, int shift, floa
         return;
     }
     const float half = 0.5f * (float)W;
+    const float inv_half = 2.0f / (float)W;  // exact for the usual even widths
     const float x0 = ceilf(g - half);
     base = wrap_mod((int)x0 + shift, G);
 #pragma unroll
     for (int k = 0; k < WMAX; ++k) {
         if (k < W) {
-            const float u = ((x0 + (float)k) - g) / half;
+            const float u = ((x0 + (float)k) - g) * inv_half;
             w[k] = kb_eval(u, beta);
         } else {
             w[k] = 0.0f;

@@ -102,17 +102,19 @@ __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ 
     }
 }
 
-// Staged interpolation: one workgroup loads the (tile + W-1 halo) region of
-// the grid for a few coils into LDS with coalesced row reads, then every lane
-// interpolates one sample whose footprint origin lies in the tile, gathering
-// its taps from LDS. Samples come from the plan's primary bins (each sample
-// exactly once); hot tiles are split into several work items by entry range.
+// Staged interpolation: a workgroup takes one segment (<= kBlock * kStagedEpt
+// samples) of one interpolation tile. Each lane evaluates the taps of its
+// samples once and keeps them in registers; then, coil chunk by coil chunk,
+// the workgroup loads the (tile + W-1 halo) grid region of the chunk into
+// LDS with coalesced row reads and every lane gathers its taps from LDS.
+// Samples come from the plan's primary bins (each sample exactly once).
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items,
                                                                const float2 *__restrict__ grid,
                                                                float2 *__restrict__ samp,
-                                                               const float *__restrict__ weight) {
+                                                               const float *__restrict__ weight,
+                                                               int chunk) {
     extern __shared__ float2 region[];
     const Item it = items[blockIdx.x];
     const Job &J = jobs[it.job];
@@ -125,77 +127,110 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
     const int Rx = Tx + J.W[0] - 1, Ry = Ty + J.W[1] - 1, Rz = Tz + J.W[2] - 1;
     const int Gx = J.G[0], Gy = J.G[1], Gz = J.G[2];
     const int cells = Rx * Ry * Rz;
-    const int ncoil = it.c1 - it.c0;
+    const int nx = J.W[0], ny = NDIM > 1 ? J.W[1] : 1, nz = NDIM > 2 ? J.W[2] : 1;
 
-    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
-        const int c = i / cells;
-        const int cell = i - c * cells;
-        const int lx = cell % Rx;
-        const int ly = (cell / Rx) % Ry;
-        const int lz = cell / (Rx * Ry);
-        int gx = ox + lx, gy = oy + ly, gz = oz + lz;
-        while (gx >= Gx) gx -= Gx;
-        while (gy >= Gy) gy -= Gy;
-        while (gz >= Gz) gz -= Gz;
-        region[i] = grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride +
-                         ((int64_t)gz * Gy + gy) * Gx + gx];
-    }
-    __syncthreads();
-
-    for (int e = it.e0 + (int)threadIdx.x; e < it.e1; e += kBlock) {
-        const float4 en = J.pentries[e];
-        const int p = __float_as_int(en.w);
-        float w[3][WC];
-        int base[3];
-        const float c3[3] = {en.x, en.y, en.z};
+    // taps of this lane's samples, kept across coil chunks
+    float w[kStagedEpt][3][WC];
+    int loc[kStagedEpt];  // region offset of the footprint origin, -1 = none
+    int pnt[kStagedEpt];
+    float scl[kStagedEpt];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (d < NDIM) {
-                axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base[d], w[d]);
-            } else {
-                base[d] = 0;
+    for (int q = 0; q < kStagedEpt; ++q) {
+        const int e = it.e0 + q * kBlock + (int)threadIdx.x;
+        loc[q] = -1;
+        pnt[q] = 0;
+        scl[q] = 0.0f;
+        if (e < it.e1) {
+            const float4 en = J.pentries[e];
+            const float c3[3] = {en.x, en.y, en.z};
+            int base[3] = {0, 0, 0};
 #pragma unroll
-                for (int k = 0; k < WC; ++k) w[d][k] = k == 0 ? 1.0f : 0.0f;
+            for (int d = 0; d < 3; ++d) {
+                if (d < NDIM) {
+                    axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base[d], w[q][d]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < WC; ++k) w[q][d][k] = k == 0 ? 1.0f : 0.0f;
+                }
             }
+            // a sample always bins into the tile holding its origin; the
+            // clamp only keeps LDS addressing in range
+            const int lx0 = min(max(base[0] - ox, 0), Tx - 1);
+            const int ly0 = min(max(base[1] - oy, 0), Ty - 1);
+            const int lz0 = min(max(base[2] - oz, 0), Tz - 1);
+            loc[q] = (lz0 * Ry + ly0) * Rx + lx0;
+            pnt[q] = __float_as_int(en.w);
+            float sc = J.inv_scale;
+            if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + pnt[q]];
+            scl[q] = sc;
         }
-        // local origin of the footprint inside the region (clamped: a sample
-        // always bins into the tile holding its origin, the clamp only keeps
-        // LDS addressing in range)
-        const int lx0 = min(max(base[0] - ox, 0), Tx - 1);
-        const int ly0 = min(max(base[1] - oy, 0), Ty - 1);
-        const int lz0 = min(max(base[2] - oz, 0), Tz - 1);
-        const int nx = J.W[0], ny = NDIM > 1 ? J.W[1] : 1, nz = NDIM > 2 ? J.W[2] : 1;
-        float sc = J.inv_scale;
-        if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-        for (int b = 0; b < ncoil; ++b) {
-            const float2 *r = region + (size_t)b * cells;
-            float re = 0.0f, im = 0.0f;
+    }
+
+    for (int cc = it.c0; cc < it.c1; cc += chunk) {
+        const int ncoil = min(chunk, it.c1 - cc);
+        __syncthreads();  // previous chunk consumed
+        for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
+            const int c = i / cells;
+            const int cell = i - c * cells;
+            const int lx = cell % Rx;
+            const int ly = (cell / Rx) % Ry;
+            const int lz = cell / (Rx * Ry);
+            int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+            while (gx >= Gx) gx -= Gx;
+            while (gy >= Gy) gy -= Gy;
+            while (gz >= Gz) gz -= Gz;
+            region[i] = grid[J.grid_off + (int64_t)(cc + c) * J.grid_bstride +
+                             ((int64_t)gz * Gy + gy) * Gx + gx];
+        }
+        __syncthreads();
 #pragma unroll
-            for (int kz = 0; kz < WC; ++kz) {
-                if (kz < nz) {
+        for (int q = 0; q < kStagedEpt; ++q) {
+            if (loc[q] < 0) continue;
+            for (int b = 0; b < ncoil; ++b) {
+                const float2 *r = region + (size_t)b * cells + loc[q];
+                float re = 0.0f, im = 0.0f;
 #pragma unroll
-                    for (int ky = 0; ky < WC; ++ky) {
-                        if (ky < ny) {
-                            const float vzy = w[2][kz] * w[1][ky];
-                            const float2 *row = r + ((lz0 + kz) * Ry + (ly0 + ky)) * Rx + lx0;
+                for (int kz = 0; kz < WC; ++kz) {
+                    if (kz < nz) {
 #pragma unroll
-                            for (int kx = 0; kx < WC; ++kx) {
-                                if (kx < nx) {
-                                    const float v = vzy * w[0][kx];
-                                    const float2 s = row[kx];
-                                    re += v * s.x;
-                                    im += v * s.y;
+                        for (int ky = 0; ky < WC; ++ky) {
+                            if (ky < ny) {
+                                const float vzy = w[q][2][kz] * w[q][1][ky];
+                                const float2 *row = r + (kz * Ry + ky) * Rx;
+#pragma unroll
+                                for (int kx = 0; kx < WC; ++kx) {
+                                    if (kx < nx) {
+                                        const float v = vzy * w[q][0][kx];
+                                        const float2 s = row[kx];
+                                        re += v * s.x;
+                                        im += v * s.y;
+                                    }
                                 }
                             }
                         }
                     }
                 }
+                samp[J.samp_off + (int64_t)(cc + b) * J.samp_bstride + pnt[q]] =
+                    make_float2(re * scl[q], im * scl[q]);
             }
-            samp[J.samp_off + (int64_t)(it.c0 + b) * J.samp_bstride + p] = make_float2(re * sc, im * sc);
         }
     }
 }
 
+// Gridding. One workgroup owns one tile for a set of coils and walks the
+// tile's entries in batches of kGridBatch:
+//   A. one lane per entry evaluates the taps (per axis: wrapped origin + W
+//      weights) and all lanes gather the scaled sample values of every coil;
+//      both are staged in LDS once for the four waves;
+//   B. every wave spreads the whole batch for the coils it owns, one entry
+//      per lane: the plan orders each tile's entries round-robin over
+//      footprint origins, so the 64 lanes of an LDS atomic add mostly hit
+//      distinct cells even on dense trajectory centres. Accumulators are
+//      planar with a padded row pitch.
+// Each accumulator cell is updated by a single wave (waves own whole coils,
+// or row bands when there are fewer than four coils), in the fixed entry
+// order, so the result is deterministic. The tile is written once (or
+// stored as a partial of a split tile, folded by fold_kernel).
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jobs,
                                                       const Item *__restrict__ items,
@@ -204,7 +239,8 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
                                                       const float *__restrict__ weight,
                                                       float *__restrict__ scratch,
                                                       const int64_t *__restrict__ slot_base) {
-    extern __shared__ float lds[];
+    extern __shared__ float smem[];
+    constexpr int B = kGridBatch;
     const Item it = items[blockIdx.x];
     const Job &J = jobs[it.job];
     const int tile = it.unit;
@@ -214,58 +250,127 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
     const int tz = tile / (J.NT[0] * J.NT[1]);
     const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
     const int ex = min(Tx, J.G[0] - ox), ey = min(Ty, J.G[1] - oy), ez = min(Tz, J.G[2] - oz);
-    const int cells = Tx * Ty * Tz;
+    const int P = Tx + kGridPad;
+    const int rowsP = Tz * Ty * P;
+    const int plane = rowsP + kWave;  // + one trash cell per lane
     const int ncoil = it.c1 - it.c0;
+    const int Gx = J.G[0], Gy = J.G[1], Gz = J.G[2];
 
-    for (int i = threadIdx.x; i < 2 * ncoil * cells; i += kBlock) lds[i] = 0.0f;
-    __syncthreads();
+    float *acc = smem;                                     // [ncoil][re, im][Tz*Ty*P + trash]
+    int *sbase = (int *)(acc + 2 * ncoil * plane);         // [3][B]
+    int *sidx = sbase + 3 * B;                             // [B]
+    float *sw = (float *)(sidx + B);                       // [3][WC][B]
+    float2 *sval = (float2 *)(sw + 3 * WC * B);            // [ncoil][B]
 
-    const int wave = threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    const int e0 = it.e0;
-    const int e1 = it.e1;
-    if (wave < ncoil) {
-        for (int e = e0 + lane; e < e1; e += kWave) {
-            const float4 en = J.entries[e];
-            const int p = __float_as_int(en.w);
-            Taps<NDIM, WC> t;
-            compute_taps<NDIM, WC>(J, en.x, en.y, en.z, t);
-            // local tile coordinates (-1 outside this tile)
-            int lx[WC], ly[WC], lz[WC];
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave, lane = tid % kWave;
+    const int nx = J.W[0];
+    const int ny = NDIM > 1 ? J.W[1] : 1;
+    const int nz = NDIM > 2 ? J.W[2] : 1;
+
+    // wave ownership: whole coils (>= 4 coils) or (coil, row band)
+    const int rows = Tz * Ty;
+    int coil0, cstep, row_lo = 0, row_hi = rows;
+    if (ncoil >= kBlock / kWave) {
+        coil0 = wave;
+        cstep = kBlock / kWave;
+    } else {
+        const int nb = (kBlock / kWave) / ncoil;
+        const int band = wave / ncoil;
+        coil0 = band < nb ? wave % ncoil : ncoil;
+        cstep = ncoil;  // one coil per wave
+        const int rpb = (rows + nb - 1) / nb;
+        row_lo = min(rows, band * rpb);
+        row_hi = min(rows, row_lo + rpb);
+    }
+
+    for (int i = tid; i < 2 * ncoil * plane; i += kBlock) acc[i] = 0.0f;
+
+    for (int b0 = it.e0; b0 < it.e1; b0 += B) {
+        const int nb_ = min(B, it.e1 - b0);
+        __syncthreads();  // previous batch consumed (and accumulators zeroed)
+        // A1. taps of the batch, one entry per lane
+        if (tid < nb_) {
+            const float4 en = J.entries[b0 + tid];
+            const float c3[3] = {en.x, en.y, en.z};
+            sidx[tid] = __float_as_int(en.w);
 #pragma unroll
-            for (int k = 0; k < WC; ++k) {
-                const int ax = t.idx[0][k] - ox, ay = t.idx[1][k] - oy, az = t.idx[2][k] - oz;
-                lx[k] = (k < t.n[0] && ax >= 0 && ax < ex) ? ax : -1;
-                ly[k] = (k < t.n[1] && ay >= 0 && ay < ey) ? ay : -1;
-                lz[k] = (k < t.n[2] && az >= 0 && az < ez) ? az : -1;
+            for (int d = 0; d < 3; ++d) {
+                float w[WC];
+                int base = 0;
+                if (d < NDIM) {
+                    axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base, w);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < WC; ++k) w[k] = k == 0 ? 1.0f : 0.0f;
+                }
+                sbase[d * B + tid] = base;
+#pragma unroll
+                for (int k = 0; k < WC; ++k) sw[(d * WC + k) * B + tid] = w[k];
             }
+        }
+        __syncthreads();
+        // A2. scaled sample values of every coil of the item
+        for (int i = tid; i < nb_ * ncoil; i += kBlock) {
+            const int c = i / nb_;
+            const int j = i - c * nb_;
+            const int p = sidx[j];
             float sc = J.inv_scale;
             if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-            for (int b = it.c0 + wave; b < it.c1; b += kBlock / kWave) {
-                float2 s = samp[J.samp_off + (int64_t)b * J.samp_bstride + p];
-                s.x *= sc;
-                s.y *= sc;
-                float *acc = lds + (size_t)2 * (b - it.c0) * cells;
+            float2 v = samp[J.samp_off + (int64_t)(it.c0 + c) * J.samp_bstride + p];
+            v.x *= sc;
+            v.y *= sc;
+            sval[c * B + j] = v;
+        }
+        __syncthreads();
+        // B. spread, one entry per lane. The tap loop is branch-free: taps
+        // outside the tile (or outside this wave's row band) get weight 0
+        // and a per-lane trash address, so one wait for the sample value
+        // is followed by back-to-back LDS atomics.
+        if (coil0 < ncoil) {
+            for (int r = 0; r < nb_; r += kWave) {
+                const int j = min(r + lane, nb_ - 1);
+                const bool live = r + lane < nb_;
+                const int bx = sbase[j], by = sbase[B + j], bz = sbase[2 * B + j];
+                int lx[WC], lrow[WC * WC];
+                float wx[WC], wzy[WC * WC];
+#pragma unroll
+                for (int k = 0; k < WC; ++k) {
+                    const int ax = wrap_add(bx, k, Gx) - ox;
+                    const bool okx = live && k < nx && ax >= 0 && ax < ex;
+                    lx[k] = okx ? ax : -1;
+                    wx[k] = okx ? sw[k * B + j] : 0.0f;
+                }
 #pragma unroll
                 for (int kz = 0; kz < WC; ++kz) {
-                    if (lz[kz] >= 0) {
+                    const int az = wrap_add(bz, kz, Gz) - oz;
+                    const bool okz = kz < nz && az >= 0 && az < ez;
+                    const float wz = sw[(2 * WC + kz) * B + j];
 #pragma unroll
-                        for (int ky = 0; ky < WC; ++ky) {
-                            if (ly[ky] >= 0) {
-                                const float vzy = t.w[2][kz] * t.w[1][ky];
-                                const int rowc = (lz[kz] * Ty + ly[ky]) * Tx;
+                    for (int ky = 0; ky < WC; ++ky) {
+                        const int ay = wrap_add(by, ky, Gy) - oy;
+                        const int row = az * Ty + ay;
+                        const bool ok = okz && ky < ny && ay >= 0 && ay < ey && row >= row_lo && row < row_hi;
+                        lrow[kz * WC + ky] = ok ? row * P : -1;
+                        wzy[kz * WC + ky] = ok ? wz * sw[(WC + ky) * B + j] : 0.0f;
+                    }
+                }
+                const int trash = rowsP + lane;
+                for (int c = coil0; c < ncoil; c += cstep) {
+                    const float2 sv = sval[c * B + j];
+                    float *are = acc + 2 * c * plane;
 #pragma unroll
-                                for (int kx = 0; kx < WC; ++kx) {
-                                    if (lx[kx] >= 0) {
-                                        const float v = vzy * t.w[0][kx];
-                                        float *a = acc + 2 * (rowc + lx[kx]);
-                                        __hip_atomic_fetch_add(a, v * s.x, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                                        __hip_atomic_fetch_add(a + 1, v * s.y, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                                    }
-                                }
-                            }
+                    for (int q = 0; q < WC * WC; ++q) {
+                        if (NDIM < 3 && q >= WC) break;
+#pragma unroll
+                        for (int kx = 0; kx < WC; ++kx) {
+                            const bool ok = lrow[q] >= 0 && lx[kx] >= 0;
+                            const int a = ok ? lrow[q] + lx[kx] : trash;
+                            const float v = wzy[q] * wx[kx];
+                            __hip_atomic_fetch_add(are + a, v * sv.x, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(are + plane + a, v * sv.y, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
                 }
@@ -274,24 +379,32 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
     }
     __syncthreads();
 
+    const int cells = Tx * Ty * Tz;
     if (it.slot >= 0) {
-        // split tile: store this partial; the fold kernel sums the parts in
-        // part order and writes the tile
+        // split tile: store this partial ([coil][cell](re, im)); the fold
+        // kernel sums the parts in part order and writes the tile
         float *dst = scratch + slot_base[it.slot] + (int64_t)it.part * 2 * ncoil * cells;
-        for (int i = threadIdx.x; i < 2 * ncoil * cells; i += kBlock) dst[i] = lds[i];
+        for (int i = tid; i < ncoil * cells; i += kBlock) {
+            const int b = i / cells;
+            const int cell = i - b * cells;
+            const int lx = cell % Tx;
+            const int row = cell / Tx;
+            const float *a = acc + 2 * b * plane + row * P + lx;
+            dst[2 * i] = a[0];
+            dst[2 * i + 1] = a[plane];
+        }
         return;
     }
-    const int Gx = J.G[0], Gy = J.G[1];
-    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
+    for (int i = tid; i < ncoil * cells; i += kBlock) {
         const int b = i / cells;
         const int cell = i - b * cells;
         const int lx = cell % Tx;
         const int ly = (cell / Tx) % Ty;
         const int lz = cell / (Tx * Ty);
         if (lx < ex && ly < ey && lz < ez) {
-            const float *a = lds + 2 * ((size_t)b * cells + cell);
+            const float *a = acc + 2 * b * plane + (lz * Ty + ly) * P + lx;
             grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
-                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(a[0], a[1]);
+                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(a[0], a[plane]);
         }
     }
 }
@@ -413,6 +526,7 @@ unsigned stride_blocks(int64_t n) {
 
 // ---- dispatch ----------------------------------------------------------------
 using InterpFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *);
+using StagedFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, int);
 using GridFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, float *,
                         const int64_t *);
 
@@ -432,7 +546,7 @@ InterpFn pick_interp(int wc) {
 }
 
 template <int NDIM>
-InterpFn pick_staged(int wc) {
+StagedFn pick_staged(int wc) {
     switch (wc) {
         case 1: return interp_staged_kernel<NDIM, 1>;
         case 2: return interp_staged_kernel<NDIM, 2>;
@@ -477,12 +591,17 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
                   hipStream_t stream) {
     if (t.n_interp == 0) return TN_OK;
     const int ndim = t.kind_interp / 16, wc = t.kind_interp % 16;
-    const bool staged = t.interp_mode == kInterpStaged;
-    InterpFn fn = staged ? (ndim == 2 ? pick_staged<2>(wc) : ndim == 3 ? pick_staged<3>(wc) : nullptr)
-                         : (ndim == 2 ? pick_interp<2>(wc) : ndim == 3 ? pick_interp<3>(wc) : nullptr);
-    if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
-    hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), staged ? t.interp_lds : 0, stream,
-                       t.d_jobs, t.d_interp, grid, samp, weight);
+    if (t.interp_mode == kInterpStaged) {
+        StagedFn fn = ndim == 2 ? pick_staged<2>(wc) : ndim == 3 ? pick_staged<3>(wc) : nullptr;
+        if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
+        hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), t.interp_lds, stream, t.d_jobs,
+                           t.d_interp, grid, samp, weight, t.interp_coils);
+    } else {
+        InterpFn fn = ndim == 2 ? pick_interp<2>(wc) : ndim == 3 ? pick_interp<3>(wc) : nullptr;
+        if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
+        hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), 0, stream, t.d_jobs,
+                           t.d_interp, grid, samp, weight);
+    }
     TN_HIP(hipGetLastError());
     return TN_OK;
 }

@@ -143,6 +143,53 @@ __global__ void bin_gather_kernel(BinGeom geo, const float *__restrict__ coord,
     }
 }
 
+// Reordering of gridding bins: inside each bin, entries are ordered by
+// (rank of the sample among the bin's samples with the same footprint
+// origin, origin, sample). Consecutive entries then mostly have distinct
+// origins, so the lanes of one LDS atomic in the gridding kernel update
+// distinct cells. The order is a pure function of the trajectory.
+__device__ __forceinline__ uint32_t entry_origin(const BinGeom &geo, const float4 &en) {
+    const float c[3] = {en.x, en.y, en.z};
+    int b[3] = {0, 0, 0};
+    for (int d = 0; d < geo.ndim; ++d) b[d] = axis_base(geo.ax[d], c[d]);
+    const uint32_t gy = geo.ndim > 1 ? geo.ax[1].G : 1;
+    return ((uint32_t)b[2] * gy + (uint32_t)b[1]) * (uint32_t)geo.ax[0].G + (uint32_t)b[0];
+}
+
+__global__ void origin_key_kernel(BinGeom geo, const uint32_t *__restrict__ bin_keys,
+                                  const float4 *__restrict__ entries, int64_t n,
+                                  uint64_t *__restrict__ key, int32_t *__restrict__ val) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        key[e] = ((uint64_t)bin_keys[e] << 32) | entry_origin(geo, entries[e]);
+        val[e] = (int32_t)e;
+    }
+}
+
+__global__ void group_head_kernel(const uint64_t *__restrict__ key, int64_t n, int32_t *__restrict__ head) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        head[e] = (e == 0 || key[e] != key[e - 1]) ? (int32_t)e : 0;
+}
+
+__global__ void rank_key_kernel(const uint64_t *__restrict__ key, const int32_t *__restrict__ start,
+                                int64_t n, uint64_t *__restrict__ key2) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        key2[e] = (key[e] >> 32 << 32) | (uint32_t)(e - start[e]);
+}
+
+__global__ void permute_entries_kernel(const float4 *__restrict__ src, const int32_t *__restrict__ idx,
+                                       int64_t n, float4 *__restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = src[idx[e]];
+}
+
+struct MaxOp {
+    __device__ __forceinline__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
+};
+
 // Marks every grid cell touched by a footprint (one frame).
 __global__ void footprint_mark_kernel(BinGeom geo, const float *__restrict__ coord,
                                       uint8_t *__restrict__ mask) {
@@ -221,8 +268,11 @@ namespace {
 
 // Stable radix sort of (key, sample) pairs, then per-bin offsets and the
 // sorted entry records (coordinates + sample index).
+int spread_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, float4 *entries,
+                     hipStream_t stream);
+
 int sort_entries(const BinGeom &geo, const float *coord, void *keys, void *vals, int64_t n,
-                 int32_t *bin_off, float4 **entries, hipStream_t stream) {
+                 int32_t *bin_off, float4 **entries, hipStream_t stream, bool spread = false) {
     const int64_t nbins = geo.nframes * geo.ntiles;
     DevBuf keys_sorted, vals_sorted, sort_tmp;
     TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
@@ -247,7 +297,56 @@ int sort_entries(const BinGeom &geo, const float *coord, void *keys, void *vals,
                        coord, (const uint32_t *)keys_sorted.p, (const int32_t *)vals_sorted.p, n,
                        *entries);
     TN_HIP(hipGetLastError());
+    if (spread) {
+        int rc = spread_by_origin(geo, (const uint32_t *)keys_sorted.p, n, *entries, stream);
+        if (rc) return rc;
+    }
     TN_HIP(hipStreamSynchronize(stream));  // temporaries are freed on scope exit
+    return TN_OK;
+}
+
+int spread_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, float4 *entries,
+                     hipStream_t stream) {
+    DevBuf k1, v1, k1s, v1s, head, start, k2, v2s, tmp, copy;
+    TN_HIP(hipMalloc(&k1.p, sizeof(uint64_t) * n));
+    TN_HIP(hipMalloc(&v1.p, sizeof(int32_t) * n));
+    TN_HIP(hipMalloc(&k1s.p, sizeof(uint64_t) * n));
+    TN_HIP(hipMalloc(&v1s.p, sizeof(int32_t) * n));
+    TN_HIP(hipMalloc(&head.p, sizeof(int32_t) * n));
+    TN_HIP(hipMalloc(&start.p, sizeof(int32_t) * n));
+    TN_HIP(hipMalloc(&k2.p, sizeof(uint64_t) * n));
+    TN_HIP(hipMalloc(&v2s.p, sizeof(int32_t) * n));
+    TN_HIP(hipMalloc(&copy.p, sizeof(float4) * n));
+    const unsigned nb = grid_blocks(n);
+    hipLaunchKernelGGL(origin_key_kernel, dim3(nb), dim3(kBlock), 0, stream, geo, bin_keys, entries, n,
+                       (uint64_t *)k1.p, (int32_t *)v1.p);
+    TN_HIP(hipGetLastError());
+    size_t bytes = 0, b2 = 0, b3 = 0;
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint64_t *)k1.p, (uint64_t *)k1s.p,
+                                              (int32_t *)v1.p, (int32_t *)v1s.p, (int)n, 0, 64, stream));
+    TN_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b2, (int32_t *)head.p, (int32_t *)start.p, MaxOp(),
+                                             (int)n, stream));
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, (uint64_t *)k2.p, (uint64_t *)k1s.p,
+                                              (int32_t *)v1s.p, (int32_t *)v2s.p, (int)n, 0, 64, stream));
+    TN_HIP(hipMalloc(&tmp.p, std::max(bytes, std::max(b2, b3))));
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, (uint64_t *)k1.p, (uint64_t *)k1s.p,
+                                              (int32_t *)v1.p, (int32_t *)v1s.p, (int)n, 0, 64, stream));
+    hipLaunchKernelGGL(group_head_kernel, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t *)k1s.p, n,
+                       (int32_t *)head.p);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipcub::DeviceScan::InclusiveScan(tmp.p, b2, (int32_t *)head.p, (int32_t *)start.p, MaxOp(),
+                                             (int)n, stream));
+    hipLaunchKernelGGL(rank_key_kernel, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t *)k1s.p,
+                       (const int32_t *)start.p, n, (uint64_t *)k2.p);
+    TN_HIP(hipGetLastError());
+    // stable: equal (bin, rank) keep the (bin, origin, sample) order
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, b3, (uint64_t *)k2.p, (uint64_t *)k1s.p,
+                                              (int32_t *)v1s.p, (int32_t *)v2s.p, (int)n, 0, 64, stream));
+    TN_HIP(hipMemcpyAsync(copy.p, entries, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(permute_entries_kernel, dim3(nb), dim3(kBlock), 0, stream, (const float4 *)copy.p,
+                       (const int32_t *)v2s.p, n, entries);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipStreamSynchronize(stream));
     return TN_OK;
 }
 
@@ -305,7 +404,7 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
                            (int32_t *)vals.p);
         TN_HIP(hipGetLastError());
         int rc = sort_entries(geo, plan->d_coord, keys.p, vals.p, n, plan->d_bin_off,
-                              &plan->d_entries, stream);
+                              &plan->d_entries, stream, /*spread=*/true);
         if (rc) return rc;
         plan->h_bin_off.assign(nbins + 1, 0);
         TN_HIP(hipMemcpy(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1),

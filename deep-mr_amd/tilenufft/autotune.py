@@ -27,7 +27,7 @@ def default_candidates(ndim):
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
         itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
     return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16],
-                itiles=itiles, staged_chunks=[1, 2, 4, 8], grid_splits=[512, 1024, 4096, -1])
+                itiles=itiles, staged_chunks=[2, 4, 8, 16], grid_splits=[512, 1024, 4096, -1])
 
 
 def _cache_path():

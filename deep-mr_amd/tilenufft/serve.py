@@ -124,6 +124,7 @@ class ReconServer:
         self._layout_key = None
         self._layout = None
         self.steps = 0
+        self.stats = dict(admitted=0, rejected=0, deferred_for_memory=0, deferred_for_slots=0)
         self.t0 = time.perf_counter()
 
     # ---- admission -----------------------------------------------------------
@@ -159,6 +160,8 @@ class ReconServer:
 
     def _admit(self):
         admitted = False
+        if self.queue and len(self.active) >= self.max_active:
+            self.stats["deferred_for_slots"] += 1
         while self.queue and len(self.active) < self.max_active:
             req = self.queue[0]
             if req.cancelled:
@@ -171,10 +174,12 @@ class ReconServer:
             if need > self.budget:
                 self.queue.popleft()
                 req.error = f"request needs {need} bytes, more than the HBM budget {self.budget}"
+                self.stats["rejected"] += 1
                 req.t_done = self.now()
                 self.done.append(req)
                 continue
             if self.reserved + need > self.budget:
+                self.stats["deferred_for_memory"] += 1
                 break  # wait for memory to free up (FIFO)
             self.queue.popleft()
             try:
@@ -182,12 +187,14 @@ class ReconServer:
                 plan = self._plan_for(req)
                 st = _Active(req, plan, self.device)
             except Exception as exc:  # a malformed request fails alone
+                self.stats["rejected"] += 1
                 req.error = f"{type(exc).__name__}: {exc}"
                 req.t_done = self.now()
                 self.done.append(req)
                 continue
             st.bytes = need
             self.reserved += need
+            self.stats["admitted"] += 1
             req.t_admit = self.now()
             self.active.append(st)
             admitted = True

@@ -48,21 +48,30 @@ def test_alone_equals_mix_equals_plain():
 
 def test_budget_and_slots_limit_concurrency():
     print("server: admission by HBM budget / slots", flush=True)
-    reqs = small_stream(8, seed=1)
-    need = max(serve.estimate_request_bytes(r.shape, r.ncoils, r.npts) for r in reqs)
-    srv = serve.ReconServer(DEV, hbm_budget=int(2.2 * need), max_active=8)
+    base = small_stream(1, seed=1)[0]
+    reqs = []
+    for i in range(6):  # equal sizes: exactly two fit the budget below
+        r = copy.deepcopy(base)
+        r.rid = i
+        reqs.append(r)
+    need = serve.estimate_request_bytes(base.shape, base.ncoils, base.npts)
+    srv = serve.ReconServer(DEV, hbm_budget=int(2.5 * need), max_active=8)
     peak = []
     srv.run(reqs, realtime=False, on_step=lambda s: peak.append((len(s.active), s.reserved)))
     assert all(r.error is None for r in reqs)
-    assert max(a for a, _ in peak) <= 2
+    assert max(a for a, _ in peak) == 2
     assert max(b for _, b in peak) <= srv.budget
+    assert srv.stats["deferred_for_memory"] > 0
     # requests joined while others were active (continuous batching, not a queue)
-    joined_mid = sum(1 for r in reqs if any(o.t_admit < r.t_admit < o.t_done for o in reqs if o is not r))
-    assert joined_mid > 0
+    mixed = copy.deepcopy(small_stream(8, seed=1))
+    for r in mixed:
+        r.niter = 2 + r.rid % 5  # unequal lengths: slots free up at different steps
     srv2 = serve.ReconServer(DEV, hbm_budget=8 << 30, max_active=3)
     peak2 = []
-    srv2.run(copy.deepcopy(reqs), realtime=False, on_step=lambda s: peak2.append(len(s.active)))
+    srv2.run(mixed, realtime=False, on_step=lambda s: peak2.append(len(s.active)))
     assert max(peak2) <= 3
+    joined_mid = sum(1 for r in mixed if any(o.t_admit < r.t_admit < o.t_done for o in mixed if o is not r))
+    assert joined_mid > 0
 
 
 def test_failures_stay_isolated():
