@@ -110,6 +110,46 @@ def cpu_baseline(wl, batch, frame=0, reps=5, warm=1, budget_s=30.0):
         orc.set_fft_backend("numpy")
 
 
+def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
+    """Mixed-size CG-SENSE stream (Poisson arrivals; N in {128,192,256,320},
+    4..32 coils, radial or spiral, 10k..200k samples per coil, 10 CG
+    iterations, osf 1.25, W 4) through the continuous-batching server and the
+    same requests one at a time through the plain GPU NUFFT (plan reused over
+    a request's iterations). Both runs see the same arrival times."""
+    from tilenufft import serve
+
+    def stream(n, s):
+        return serve.make_stream(n, rate_hz, seed=s, device="cpu")
+
+    # warm-up (kernel loading, rocFFT plans of the common sizes), untimed
+    serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32).run(stream(warm, seed + 1000),
+                                                                 realtime=False)
+    serve.run_plain(stream(warm, seed + 1000), dev, realtime=False)
+    torch.cuda.synchronize(dev)
+
+    reqs_s = stream(nreq, seed)
+    srv = serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32)
+    srv.run(reqs_s, realtime=True)
+    reqs_p = stream(nreq, seed)
+    serve.run_plain(reqs_p, dev, realtime=True)
+    st_s, st_p = serve.latency_stats(reqs_s), serve.latency_stats(reqs_p)
+    errs = []
+    for a, b in zip(reqs_s, reqs_p):
+        if a.result is not None and b.result is not None:
+            ra, rb = a.result.to(dev), b.result.to(dev)
+            errs.append(float(torch.linalg.vector_norm(ra - rb) / torch.linalg.vector_norm(rb)))
+    return dict(
+        stream=f"{nreq} requests, Poisson {rate_hz:g} Hz (seed {seed}), N in {{128,192,256,320}}, "
+               "4-32 coils, radial/spiral, 10k-200k samples/coil, 10 CG-SENSE iterations, osf 1.25, W 4",
+        served=dict(p50_s=st_s["p50"], p99_s=st_s["p99"], recon_per_s=st_s["recon_per_s"],
+                    completed=st_s["completed"], server_steps=srv.steps, stats=srv.stats),
+        plain=dict(p50_s=st_p["p50"], p99_s=st_p["p99"], recon_per_s=st_p["recon_per_s"],
+                   completed=st_p["completed"]),
+        speedup_recon_per_s=(st_s["recon_per_s"] / st_p["recon_per_s"]) if st_p["recon_per_s"] else None,
+        max_rel_l2_served_vs_plain=max(errs) if errs else None,
+    )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +159,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the NUFFT path "
@@ -193,6 +236,11 @@ def main():
         log("bench: CPU baseline (oracle restatement of deepmr's CPU loops) on frame 0")
         cpu, parity = cpu_baseline(wl, batch)
 
+    serving = None
+    if not args.no_serve:
+        log(f"bench: serving stream ({args.serve_requests} requests) served vs plain")
+        serving = serving_bench(dev, args.serve_requests, args.serve_rate)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -239,6 +287,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "parity_rel_l2_frame0_vs_oracle": parity,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

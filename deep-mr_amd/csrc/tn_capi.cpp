@@ -119,7 +119,7 @@ static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &
 // gridding items are (job, tile, coil range). `plan_of[j]` is the plan job j
 // came from (for its chunk preferences).
 static int build_tables_for(const std::vector<Job> &jobs, const std::vector<const tn_plan *> &plan_of,
-                            LaunchTables &t) {
+                            LaunchTables &t, bool force_gather = false) {
     t.kind_interp = interp_kind(jobs);
     t.kind_grid = t.kind_interp;
     int64_t max_plane = 1, max_region = 1;
@@ -140,6 +140,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     int mode = plan_of.empty() ? kInterpGather : plan_of[0]->p.interp_mode;
     if (mode == kInterpAuto) mode = kInterpStaged;
     if (mode == kInterpStaged && region_coil > kInterpLdsBudget) mode = kInterpGather;
+    if (force_gather) mode = kInterpGather;
     t.interp_mode = mode;
 
     int max_gc = 1, max_ic = 1;
@@ -275,6 +276,7 @@ void destroy(tn_plan *pl) {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(pl->device);
     for (auto &kv : pl->cache) free_tables(kv.second);
+    for (auto &kv : pl->cache_gather) free_tables(kv.second);
     if (pl->d_coord) (void)hipFree(pl->d_coord);
     if (pl->d_entries) (void)hipFree(pl->d_entries);
     if (pl->d_bin_off) (void)hipFree(pl->d_bin_off);
@@ -286,9 +288,10 @@ void destroy(tn_plan *pl) {
     delete pl;
 }
 
-tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc) {
-    auto it = plan->cache.find(nbatch);
-    if (it != plan->cache.end()) {
+tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc, bool gather = false) {
+    auto &cache = gather ? plan->cache_gather : plan->cache;
+    auto it = cache.find(nbatch);
+    if (it != cache.end()) {
         *rc = TN_OK;
         return &it->second;
     }
@@ -296,12 +299,31 @@ tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc) {
     make_jobs(plan, nbatch, 0, 0, 0, jobs);
     std::vector<const tn_plan *> owners(jobs.size(), plan);
     LaunchTables t;
-    *rc = build_tables_for(jobs, owners, t);
+    *rc = build_tables_for(jobs, owners, t, gather);
     if (*rc != TN_OK) {
         free_tables(t);
         return nullptr;
     }
-    return &(plan->cache[nbatch] = t);
+    return &(cache[nbatch] = t);
+}
+
+// Shared checks of the low-rank entry points: one-frame plan whose samples
+// are `npts / frame_npts` frames of `frame_npts` samples.
+int check_lowrank(const tn_plan *plan, int64_t nbatch, int32_t ncoeff, int32_t frame_npts,
+                  const void *coef, LowRank &lr) {
+    if (plan->p.nframes != 1)
+        return fail(TN_E_ARG, "low-rank transforms take a plan of all frames as one trajectory");
+    if (ncoeff < 1) return fail(TN_E_ARG, "ncoeff must be >= 1");
+    if (frame_npts < 1 || plan->p.npts % frame_npts != 0)
+        return fail(TN_E_ARG, "frame_npts (%d) must divide the plan's %lld samples", frame_npts,
+                    (long long)plan->p.npts);
+    if (!coef) return fail(TN_E_ARG, "basis is NULL");
+    if (nbatch * (int64_t)ncoeff >= INT32_MAX) return fail(TN_E_RANGE, "nbatch * ncoeff too large");
+    lr.coef = (const float2 *)coef;
+    lr.ncoeff = ncoeff;
+    lr.nframes = (int32_t)(plan->p.npts / frame_npts);
+    lr.frame_npts = frame_npts;
+    return TN_OK;
 }
 
 // Checks that the current device is the plan's device.
@@ -442,6 +464,40 @@ int tn_grid(tn_plan *plan, const void *samp, void *grid, int64_t nbatch, const f
     LaunchTables *t = plan_tables(plan, nbatch, &rc);
     if (!t) return rc;
     return launch_grid(*t, (const float2 *)samp, (float2 *)grid, weight, (hipStream_t)stream);
+}
+
+int tn_interp_lowrank(tn_plan *plan, const void *grid, void *samp, int64_t nbatch, int32_t ncoeff,
+                      int32_t frame_npts, const void *basis_adjoint, const float *weight,
+                      tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nbatch < 1 || nbatch >= INT32_MAX) return fail(TN_E_ARG, "nbatch must be >= 1");
+    if ((!grid || !samp) && plan->p.npts > 0) return fail(TN_E_ARG, "NULL data pointer");
+    LowRank lr;
+    int rc = check_lowrank(plan, nbatch, ncoeff, frame_npts, basis_adjoint, lr);
+    if (rc) return rc;
+    rc = check_device(plan->device);
+    if (rc) return rc;
+    LaunchTables *t = plan_tables(plan, nbatch, &rc, true);
+    if (!t) return rc;
+    return launch_interp_lowrank(*t, (const float2 *)grid, (float2 *)samp, weight, (hipStream_t)stream, lr);
+}
+
+int tn_grid_lowrank(tn_plan *plan, const void *samp, void *grid, int64_t nbatch, int32_t ncoeff,
+                    int32_t frame_npts, const void *basis, const float *weight, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nbatch < 1 || nbatch >= INT32_MAX) return fail(TN_E_ARG, "nbatch must be >= 1");
+    if (!grid || (!samp && plan->p.npts > 0)) return fail(TN_E_ARG, "NULL data pointer");
+    LowRank lr;
+    int rc = check_lowrank(plan, nbatch, ncoeff, frame_npts, basis, lr);
+    if (rc) return rc;
+    rc = check_device(plan->device);
+    if (rc) return rc;
+    // accumulator coils are (coil, coeff) pairs
+    LaunchTables *t = plan_tables(plan, nbatch * ncoeff, &rc);
+    if (!t) return rc;
+    return launch_grid(*t, (const float2 *)samp, (float2 *)grid, weight, (hipStream_t)stream, lr);
 }
 
 int tn_apod_pad(tn_plan *plan, const void *image, void *grid, int64_t nimg, tn_stream stream) {

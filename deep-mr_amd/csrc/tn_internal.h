@@ -73,6 +73,17 @@ struct alignas(16) Item {
     int32_t e0, e1, part, slot;
 };
 
+// Low-rank subspace arguments of a launch (ncoeff == 0: plain transform).
+// Interpolation: coef = basis_adjoint (nframes, ncoeff). Gridding: coef =
+// basis (ncoeff, nframes). The plan holds all frames as one trajectory of
+// nframes * frame_npts samples.
+struct LowRank {
+    const float2 *coef = nullptr;
+    int32_t ncoeff = 0;
+    int32_t nframes = 1;
+    int32_t frame_npts = 1;
+};
+
 struct LaunchTables {
     Job *d_jobs = nullptr;
     Item *d_interp = nullptr;
@@ -115,6 +126,7 @@ struct tn_plan {
     int64_t nentries = 0;
     int64_t footprint = -1;
     std::map<int64_t, tn::LaunchTables> cache;
+    std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank)
 };
 
 struct tn_batch {
@@ -138,7 +150,9 @@ void free_tables(LaunchTables &t);
 int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp,
                   const float *weight, hipStream_t stream);
 int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid,
-                const float *weight, hipStream_t stream);
+                const float *weight, hipStream_t stream, LowRank lr = LowRank{});
+int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *samp,
+                          const float *weight, hipStream_t stream, LowRank lr);
 int launch_fold(const LaunchTables &t, float2 *grid, hipStream_t stream);
 int launch_apod_pad(const tn_plan *plan, const float2 *image, float2 *grid,
                     int64_t nimg, hipStream_t stream);

@@ -102,6 +102,71 @@ __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ 
     }
 }
 
+// Low-rank (subspace) interpolation: the grid holds `ncoeff` coefficient
+// images per coil, (coil, coeff, *G), and the samples of frame f are
+// sum_c coef[f, c] * interp(grid[coil, c]) with coef = basis_adjoint
+// (nframes, ncoeff) (src/deepmr/fft/_interp/interp.py:268-357). The plan
+// holds every frame's samples as one trajectory (frame = p / frame_npts),
+// so the per-frame sample counts do not fragment the launch. The taps are
+// evaluated once per sample and reused for every coil and coefficient.
+template <int NDIM, int WC>
+__global__ __launch_bounds__(kBlock) void interp_lowrank_kernel(const Job *__restrict__ jobs,
+                                                                const Item *__restrict__ items,
+                                                                const float2 *__restrict__ grid,
+                                                                float2 *__restrict__ samp,
+                                                                const float *__restrict__ weight,
+                                                                LowRank lr) {
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int64_t p = (int64_t)it.unit * kInterpBlockPts + threadIdx.x;
+    if (p >= J.npts) return;
+
+    const float *c = J.coord + p * NDIM;
+    const float cx = c[0];
+    const float cy = NDIM > 1 ? c[1] : 0.0f;
+    const float cz = NDIM > 2 ? c[2] : 0.0f;
+    Taps<NDIM, WC> t;
+    compute_taps<NDIM, WC>(J, cx, cy, cz, t);
+
+    float sc = J.inv_scale;
+    if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
+    const float2 *cf = lr.coef + (p / lr.frame_npts) * lr.ncoeff;
+
+    const int Gx = J.G[0], Gy = J.G[1];
+    for (int b = it.c0; b < it.c1; ++b) {
+        float re = 0.0f, im = 0.0f;
+        for (int k = 0; k < lr.ncoeff; ++k) {
+            const float2 *g = grid + J.grid_off + ((int64_t)b * lr.ncoeff + k) * J.grid_bstride;
+            float tr = 0.0f, ti = 0.0f;
+#pragma unroll
+            for (int kz = 0; kz < WC; ++kz) {
+                if (kz < t.n[2]) {
+#pragma unroll
+                    for (int ky = 0; ky < WC; ++ky) {
+                        if (ky < t.n[1]) {
+                            const float vzy = t.w[2][kz] * t.w[1][ky];
+                            const float2 *row = g + ((int64_t)t.idx[2][kz] * Gy + t.idx[1][ky]) * Gx;
+#pragma unroll
+                            for (int kx = 0; kx < WC; ++kx) {
+                                if (kx < t.n[0]) {
+                                    const float v = vzy * t.w[0][kx];
+                                    const float2 s = row[t.idx[0][kx]];
+                                    tr += v * s.x;
+                                    ti += v * s.y;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            const float2 w = cf[k];
+            re += w.x * tr - w.y * ti;
+            im += w.x * ti + w.y * tr;
+        }
+        samp[J.samp_off + (int64_t)b * J.samp_bstride + p] = make_float2(re * sc, im * sc);
+    }
+}
+
 // Staged interpolation: a workgroup takes one segment (<= kBlock * kStagedEpt
 // samples) of one interpolation tile. Each lane evaluates the taps of its
 // samples once and keeps them in registers; then, coil chunk by coil chunk,
@@ -238,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
                                                       float2 *__restrict__ grid,
                                                       const float *__restrict__ weight,
                                                       float *__restrict__ scratch,
-                                                      const int64_t *__restrict__ slot_base) {
+                                                      const int64_t *__restrict__ slot_base,
+                                                      LowRank lr) {
     extern __shared__ float smem[];
     constexpr int B = kGridBatch;
     const Item it = items[blockIdx.x];
@@ -310,14 +376,25 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
             }
         }
         __syncthreads();
-        // A2. scaled sample values of every coil of the item
+        // A2. scaled sample values of every coil of the item. Low-rank:
+        // accumulator "coils" are (coil, coeff) pairs and the sample of
+        // frame f enters coefficient k times basis[k, f].
         for (int i = tid; i < nb_ * ncoil; i += kBlock) {
             const int c = i / nb_;
             const int j = i - c * nb_;
             const int p = sidx[j];
             float sc = J.inv_scale;
             if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-            float2 v = samp[J.samp_off + (int64_t)(it.c0 + c) * J.samp_bstride + p];
+            float2 v;
+            if (lr.ncoeff > 0) {
+                const int vc = it.c0 + c;
+                const int b = vc / lr.ncoeff, k = vc - b * lr.ncoeff;
+                const float2 s = samp[J.samp_off + (int64_t)b * J.samp_bstride + p];
+                const float2 w = lr.coef[(int64_t)k * lr.nframes + p / lr.frame_npts];
+                v = make_float2(w.x * s.x - w.y * s.y, w.x * s.y + w.y * s.x);
+            } else {
+                v = samp[J.samp_off + (int64_t)(it.c0 + c) * J.samp_bstride + p];
+            }
             v.x *= sc;
             v.y *= sc;
             sval[c * B + j] = v;
@@ -528,7 +605,23 @@ unsigned stride_blocks(int64_t n) {
 using InterpFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *);
 using StagedFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, int);
 using GridFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, float *,
-                        const int64_t *);
+                        const int64_t *, LowRank);
+using InterpLrFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, LowRank);
+
+template <int NDIM>
+InterpLrFn pick_interp_lowrank(int wc) {
+    switch (wc) {
+        case 1: return interp_lowrank_kernel<NDIM, 1>;
+        case 2: return interp_lowrank_kernel<NDIM, 2>;
+        case 3: return interp_lowrank_kernel<NDIM, 3>;
+        case 4: return interp_lowrank_kernel<NDIM, 4>;
+        case 5: return interp_lowrank_kernel<NDIM, 5>;
+        case 6: return interp_lowrank_kernel<NDIM, 6>;
+        case 7: return interp_lowrank_kernel<NDIM, 7>;
+        case 8: return interp_lowrank_kernel<NDIM, 8>;
+        default: return nullptr;
+    }
+}
 
 template <int NDIM>
 InterpFn pick_interp(int wc) {
@@ -607,19 +700,32 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
 }
 
 int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const float *weight,
-                hipStream_t stream) {
+                hipStream_t stream, LowRank lr) {
     if (t.n_grid == 0) return TN_OK;
     const int ndim = t.kind_grid / 16, wc = t.kind_grid % 16;
     GridFn fn = ndim == 2 ? pick_grid<2>(wc) : ndim == 3 ? pick_grid<3>(wc) : nullptr;
     if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
     hipLaunchKernelGGL(fn, dim3((unsigned)t.n_grid), dim3(kBlock), t.grid_lds, stream, t.d_jobs,
-                       t.d_grid, samp, grid, weight, t.d_scratch, t.d_slot_base);
+                       t.d_grid, samp, grid, weight, t.d_scratch, t.d_slot_base, lr);
     TN_HIP(hipGetLastError());
     if (t.n_fold > 0) {
         hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
                            t.d_fold, t.d_scratch, t.d_slot_base, grid);
         TN_HIP(hipGetLastError());
     }
+    return TN_OK;
+}
+
+int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *samp, const float *weight,
+                          hipStream_t stream, LowRank lr) {
+    if (t.n_interp == 0) return TN_OK;
+    if (t.interp_mode != kInterpGather) return fail(TN_E_ARG, "low-rank interpolation needs gather tables");
+    const int ndim = t.kind_interp / 16, wc = t.kind_interp % 16;
+    InterpLrFn fn = ndim == 2 ? pick_interp_lowrank<2>(wc) : ndim == 3 ? pick_interp_lowrank<3>(wc) : nullptr;
+    if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
+    hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), 0, stream, t.d_jobs, t.d_interp,
+                       grid, samp, weight, lr);
+    TN_HIP(hipGetLastError());
     return TN_OK;
 }
 

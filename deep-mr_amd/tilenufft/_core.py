@@ -48,7 +48,13 @@ def stream_ptr(device):
 
 
 def ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+    """Raw device pointer of ``t`` for the C ABI. Lazy conjugate / negative
+    views are refused: their memory does not hold the values they denote."""
+    if t is None:
+        return ctypes.c_void_p(0)
+    if t.is_conj() or t.is_neg():
+        raise ValueError("resolve conjugate/negative views (resolve_conj/resolve_neg) before a native call")
+    return ctypes.c_void_p(t.data_ptr())
 
 
 # --------------------------------------------------------------------------
@@ -234,6 +240,20 @@ class NativePlan:
     def gridding(self, samp, out, nbatch, weight=None):
         _lib.check(_lib.load().tn_grid(self.handle, ptr(samp), ptr(out), int(nbatch), ptr(weight),
                                        stream_ptr(self.device)), "tn_grid")
+        return out
+
+    def interp_lowrank(self, grid, out, nbatch, ncoeff, frame_npts, basis_adjoint, weight=None):
+        """grid (nbatch, ncoeff, *G) -> out (nbatch, npts); one-frame plan."""
+        _lib.check(_lib.load().tn_interp_lowrank(self.handle, ptr(grid), ptr(out), int(nbatch), int(ncoeff),
+                                                 int(frame_npts), ptr(basis_adjoint), ptr(weight),
+                                                 stream_ptr(self.device)), "tn_interp_lowrank")
+        return out
+
+    def gridding_lowrank(self, samp, out, nbatch, ncoeff, frame_npts, basis, weight=None):
+        """samp (nbatch, npts) -> out (nbatch, ncoeff, *G); one-frame plan."""
+        _lib.check(_lib.load().tn_grid_lowrank(self.handle, ptr(samp), ptr(out), int(nbatch), int(ncoeff),
+                                               int(frame_npts), ptr(basis), ptr(weight),
+                                               stream_ptr(self.device)), "tn_grid_lowrank")
         return out
 
     def apod_pad(self, image, out, nimg):

@@ -40,6 +40,7 @@ class Interpolator:
     coord: np.ndarray = field(default=None, repr=False)
     width: tuple = ()
     beta: tuple = ()
+    lowrank_native: object = field(default=None, repr=False, compare=False)
 
     def to(self, device):
         if torch.device(device).type == "cpu":
@@ -47,8 +48,19 @@ class Interpolator:
         target = _core.compute_device(device)
         if target != self.native.device:
             self.native = _native(self.coord, self.dshape, self.width, self.beta, self.scale, target)
+            self.lowrank_native = None
             self.device = str(target)
         return self
+
+    def merged(self):
+        """Device plan of every frame's samples as one trajectory (low-rank
+        kernels; built on first use)."""
+        nat = self.lowrank_native
+        if nat is None or nat.device != self.native.device:
+            flat = np.ascontiguousarray(self.coord.reshape(1, -1, self.ndim))
+            nat = _native(flat, self.dshape, self.width, self.beta, self.scale, self.native.device)
+            self.lowrank_native = nat
+        return nat
 
 
 def _native(coord, dshape, width, beta, scale, device):
@@ -89,7 +101,14 @@ def _as_compute(x):
     x = torch.as_tensor(x)
     if x.dtype in (torch.float16, torch.float32, torch.float64, torch.bfloat16):
         return x.to(torch.float32)
-    return x.to(torch.complex64)
+    return x.to(torch.complex64).resolve_conj()  # lazy conj views carry unconjugated memory
+
+
+def _basis(b, rows, cols, name, dev):
+    b = torch.as_tensor(b)
+    if b.dim() != 2 or tuple(b.shape) != (rows, cols):
+        raise ValueError(f"{name} must be ({rows}, {cols}), got {tuple(b.shape)}")
+    return _as_compute(b).to(dev).to(torch.complex64).resolve_conj().contiguous()
 
 
 def apply_interpolation(data_in, interpolator, basis_adjoint=None, device=None, threadsperblock=128):
@@ -97,8 +116,6 @@ def apply_interpolation(data_in, interpolator, basis_adjoint=None, device=None, 
 
     data_in (..., ncontrasts, *dshape) -> (..., ncontrasts, nviews, nsamples).
     """
-    if basis_adjoint is not None:
-        raise NotImplementedError("low-rank subspace interpolation is not available yet")
     data_in = _as_compute(data_in)
     if device is not None:
         interpolator.to(device)
@@ -109,11 +126,19 @@ def apply_interpolation(data_in, interpolator, basis_adjoint=None, device=None, 
     x = data_in.to(dev).to(torch.complex64)
     batch_shape = tuple(x.shape[:-ndim]) if nframes == 1 else tuple(x.shape[: -ndim - 1])
     nbatch = int(np.prod(batch_shape))
-    x = x.reshape(nbatch, nframes, *interpolator.dshape).contiguous()
     npts = int(np.prod(interpolator.ishape))
+    ba = None
+    nimgax = nframes
+    if basis_adjoint is not None:  # grid holds ncoeff coefficients (interp.py:268-357)
+        ncoeff = int(torch.as_tensor(basis_adjoint).shape[-1])
+        ba = _basis(basis_adjoint, nframes, ncoeff, "basis_adjoint", dev)
+        nimgax = ncoeff
+    x = x.reshape(nbatch, nimgax, *interpolator.dshape).contiguous()
     with torch.cuda.device(dev):
         out = torch.empty((nbatch, nframes, npts), dtype=torch.complex64, device=dev)
-        if nbatch > 0:
+        if nbatch > 0 and ba is not None:
+            interpolator.merged().interp_lowrank(x, out, nbatch, nimgax, npts, ba)
+        elif nbatch > 0:
             native.interp(x, out, nbatch)
     if nframes == 1:
         out = out.reshape(*batch_shape, *interpolator.ishape)
@@ -127,8 +152,6 @@ def apply_gridding(data_in, interpolator, basis=None, device=None, threadsperblo
 
     data_in (..., ncontrasts, nviews, nsamples) -> (..., ncontrasts, *dshape).
     """
-    if basis is not None:
-        raise NotImplementedError("low-rank subspace gridding is not available yet")
     data_in = _as_compute(data_in)
     if device is not None:
         interpolator.to(device)
@@ -141,14 +164,22 @@ def apply_gridding(data_in, interpolator, basis=None, device=None, threadsperblo
     nbatch = int(np.prod(batch_shape))
     npts = int(np.prod(interpolator.ishape))
     y = y.reshape(nbatch, nframes, npts).contiguous()
+    bs = None
+    nimgax = nframes
+    if basis is not None:  # ncoeff coefficient grids (grid.py:326-417)
+        ncoeff = int(torch.as_tensor(basis).shape[0])
+        bs = _basis(basis, ncoeff, nframes, "basis", dev)
+        nimgax = ncoeff
     with torch.cuda.device(dev):
-        out = torch.empty((nbatch, nframes, *interpolator.dshape), dtype=torch.complex64, device=dev)
-        if nbatch > 0:
+        out = torch.empty((nbatch, nimgax, *interpolator.dshape), dtype=torch.complex64, device=dev)
+        if nbatch > 0 and bs is not None:
+            interpolator.merged().gridding_lowrank(y, out, nbatch, nimgax, npts, bs)
+        elif nbatch > 0:
             native.gridding(y, out, nbatch)
-    if nframes == 1:
-        out = out.reshape(*batch_shape, *interpolator.dshape)
+    if nframes == 1:  # a single-frame plan keeps coefficient 0 (grid.py:117)
+        out = out[:, 0].reshape(*batch_shape, *interpolator.dshape)
     else:
-        out = out.reshape(*batch_shape, nframes, *interpolator.dshape)
+        out = out.reshape(*batch_shape, nimgax, *interpolator.dshape)
     return out.to(odevice)
 
 

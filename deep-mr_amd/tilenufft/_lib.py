@@ -50,6 +50,10 @@ _SIGNATURES = {
     "tn_plan_prepare": (c_int32, [c_void_p, c_int64]),
     "tn_interp": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "tn_grid": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "tn_interp_lowrank": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p,
+                                    c_void_p, c_void_p]),
+    "tn_grid_lowrank": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p,
+                                  c_void_p, c_void_p]),
     "tn_apod_pad": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "tn_crop_apod": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "tn_batch_create": (c_int32, [ctypes.POINTER(c_void_p), ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),

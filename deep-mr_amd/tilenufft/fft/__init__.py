@@ -15,6 +15,7 @@ __all__ = [
     "apply_nufft_selfadj",
 ]
 
+from .. import _interp  # noqa: F401  (deepmr.fft._interp)
 from .fft import fft, ifft  # noqa: F401
 from .toeplitz import apply_nufft_selfadj, plan_toeplitz_nufft  # noqa: F401
 from .nufft import (  # noqa: F401

@@ -65,12 +65,14 @@ class NUFFTPlan:
     itile: tuple = None
     interp_mode: int = 0
     grid_split: int = 0
+    lowrank_native: object = field(default=None, repr=False, compare=False)
 
     def to(self, device):
         """Move the device plan to another HIP device ("cpu" keeps it where it is)."""
         target = _core.compute_device(device) if torch.device(device).type != "cpu" else None
         if target is not None and target != self.interpolator.device:
             self.interpolator = _build_native(self, target, shift=True)
+            self.lowrank_native = None
             self.device = str(target)
         return self
 
@@ -79,7 +81,7 @@ class NUFFTPlan:
         return int(np.prod(self.ishape))
 
 
-def _build_native(plan, device, shift):
+def _build_native(plan, device, shift, coord=None):
     ndim = plan.ndim
     sh = [g - g // 2 for g in plan.os_shape] if shift else [0] * ndim
     apod = [
@@ -89,7 +91,8 @@ def _build_native(plan, device, shift):
         for d in range(ndim)
     ]
     return _core.NativePlan(
-        plan.coord, plan.shape, plan.os_shape, plan.width, plan.beta, 1.0 / plan.scale, sh,
+        plan.coord if coord is None else coord, plan.shape, plan.os_shape, plan.width, plan.beta,
+        1.0 / plan.scale, sh,
         device, apod=apod, tile=plan.tile, interp_chunk=plan.interp_chunk,
         grid_chunk=plan.grid_chunk, itile=plan.itile, interp_mode=plan.interp_mode,
         grid_split=plan.grid_split,
@@ -112,7 +115,32 @@ def retune(plan, tile=None, interp_chunk=None, grid_chunk=None, itile=None, inte
     if grid_chunk is not None:
         plan.grid_chunk = int(grid_chunk)
     plan.interpolator = _build_native(plan, plan.interpolator.device, shift=True)
+    plan.lowrank_native = None
     return plan
+
+
+def _lowrank_native(plan):
+    """Device plan holding every frame's samples as one trajectory (built on
+    first low-rank use): the subspace kernels read frame f's samples at
+    [f * npts, (f + 1) * npts) and its basis row from the sample index."""
+    dev = plan.interpolator.device
+    nat = plan.lowrank_native
+    if nat is None or nat.device != dev:
+        merged = np.ascontiguousarray(plan.coord.reshape(1, -1, plan.ndim))
+        nat = _build_native(plan, dev, shift=True, coord=merged)
+        plan.lowrank_native = nat
+    return nat
+
+
+def _basis_tensor(basis, rows, cols, name, dev):
+    """deepmr casts the basis to float32 or complex64 (nufft.py:466-474,
+    :547-555); the kernels take it as complex64 (rows, cols) on the device."""
+    b = torch.as_tensor(basis)
+    if b.dim() != 2 or tuple(b.shape) != (rows, cols):
+        raise ValueError(f"{name} must be ({rows}, {cols}), got {tuple(b.shape)}")
+    if b.dtype in (torch.float16, torch.float32, torch.float64, torch.bfloat16):
+        b = b.to(torch.float32)
+    return b.to(dev).to(torch.complex64).resolve_conj().contiguous()
 
 
 def _expand_coord(coord):
@@ -304,7 +332,7 @@ def _as_compute(x):
     x = torch.as_tensor(x)
     if x.dtype in (torch.float16, torch.float32, torch.float64, torch.bfloat16):
         return x.to(torch.float32)
-    return x.to(torch.complex64)
+    return x.to(torch.complex64).resolve_conj()  # lazy conj views carry unconjugated memory
 
 
 def _fused_weight(weight, plan, dev, nframes_axis_shape):
@@ -331,8 +359,6 @@ def _check_plan(nufft_plan, device):
 def _apply_nufft(image, nufft_plan, basis_adjoint, weight, device, threadsperblock):
     isnumpy = isinstance(image, np.ndarray)
     image = _as_compute(image)
-    if basis_adjoint is not None:
-        raise NotImplementedError("low-rank subspace NUFFT is not available in tilenufft yet")
     dev = _check_plan(nufft_plan, device)
     odevice = image.device
     plan = nufft_plan
@@ -343,14 +369,25 @@ def _apply_nufft(image, nufft_plan, basis_adjoint, weight, device, threadsperblo
     lead = tuple(x.shape[:-ndim])
     if tuple(x.shape[-ndim:]) != tuple(plan.shape):
         raise ValueError(f"image spatial shape {tuple(x.shape[-ndim:])} != plan shape {plan.shape}")
+    # low rank: the image holds ncoeff subspace coefficients where the plain
+    # transform holds the nframes contrasts (interp.py:84-92)
+    nimgax = nframes
+    ba = None
+    if basis_adjoint is not None:
+        ncoeff = int(torch.as_tensor(basis_adjoint).shape[-1])
+        ba = _basis_tensor(basis_adjoint, nframes, ncoeff, "basis_adjoint", dev)
+        nimgax = ncoeff
+        if nframes == 1 and ncoeff != 1:
+            raise ValueError("a single-frame plan takes a (1, 1) basis_adjoint")
     if nframes == 1:
         batch_shape = lead
     else:
-        if len(lead) < 1 or lead[-1] != nframes:
-            raise ValueError(f"expected {nframes} contrasts before the spatial axes, got {lead}")
+        if len(lead) < 1 or lead[-1] != nimgax:
+            what = "subspace coefficients" if ba is not None else "contrasts"
+            raise ValueError(f"expected {nimgax} {what} before the spatial axes, got {lead}")
         batch_shape = lead[:-1]
     nbatch = int(np.prod(batch_shape))
-    nimg = nbatch * nframes
+    nimg = nbatch * nimgax
     x = x.contiguous()
     native = plan.interpolator
     with torch.cuda.device(dev):
@@ -361,7 +398,9 @@ def _apply_nufft(image, nufft_plan, basis_adjoint, weight, device, threadsperblo
         ksp_shape = (*batch_shape, nframes, *plan.ishape) if nframes > 1 else (*batch_shape, *plan.ishape)
         w_fused, w_bcast = _fused_weight(weight, plan, dev, (nframes, *plan.ishape) if nframes > 1 else plan.ishape)
         out = torch.empty((nbatch, nframes, plan.npts), dtype=torch.complex64, device=dev)
-        if nbatch > 0:
+        if nbatch > 0 and ba is not None:
+            _lowrank_native(plan).interp_lowrank(k, out, nbatch, nimgax, plan.npts, ba, w_fused)
+        elif nbatch > 0:
             native.interp(k, out, nbatch, w_fused)
         out = out.reshape(ksp_shape)
         if w_bcast is not None:
@@ -375,8 +414,6 @@ def _apply_nufft(image, nufft_plan, basis_adjoint, weight, device, threadsperblo
 def _apply_nufft_adj(kspace, nufft_plan, basis, weight, device, threadsperblock):
     isnumpy = isinstance(kspace, np.ndarray)
     kspace = _as_compute(kspace)
-    if basis is not None:
-        raise NotImplementedError("low-rank subspace NUFFT is not available in tilenufft yet")
     dev = _check_plan(nufft_plan, device)
     odevice = kspace.device
     plan = nufft_plan
@@ -393,17 +430,30 @@ def _apply_nufft_adj(kspace, nufft_plan, basis, weight, device, threadsperblock)
         raise ValueError(f"k-space shape {tuple(y.shape)} does not match the plan "
                          f"({nframes} frames x {plan.npts} samples)")
     y = y.reshape(nbatch, nframes, plan.npts).contiguous()
+    # low rank: the image gets ncoeff subspace coefficients per batch entry
+    # (grid.py:89-104); a single-frame plan keeps coefficient 0 (grid.py:117)
+    nimgax = nframes
+    bs = None
+    if basis is not None:
+        ncoeff = int(torch.as_tensor(basis).shape[0])
+        bs = _basis_tensor(basis, ncoeff, nframes, "basis", dev)
+        nimgax = ncoeff
     native = plan.interpolator
     with torch.cuda.device(dev):
-        grid = torch.empty((nbatch, nframes, *plan.os_shape), dtype=torch.complex64, device=dev)
-        if nbatch > 0:
+        grid = torch.empty((nbatch, nimgax, *plan.os_shape), dtype=torch.complex64, device=dev)
+        if nbatch > 0 and bs is not None:
+            _lowrank_native(plan).gridding_lowrank(y, grid, nbatch, nimgax, plan.npts, bs, w_fused)
+        elif nbatch > 0:
             native.gridding(y, grid, nbatch, w_fused)
+        if bs is not None and nframes == 1:
+            grid = grid[:, :1].contiguous()
+            nimgax = 1
         img_g = torch.fft.ifftn(grid, dim=tuple(range(-ndim, 0)))
         del grid
-        nimg = nbatch * nframes
+        nimg = nbatch * nimgax
         img = torch.empty((nimg, *plan.shape), dtype=torch.complex64, device=dev)
         native.crop_apod(img_g, img, nimg)
-        shape = (*batch_shape, nframes, *plan.shape) if nframes > 1 else (*batch_shape, *plan.shape)
+        shape = (*batch_shape, nimgax, *plan.shape) if nframes > 1 else (*batch_shape, *plan.shape)
         img = img.reshape(shape)
     img = img.to(odevice)
     if isnumpy:

@@ -115,6 +115,28 @@ int tn_interp(tn_plan *plan, const void *grid, void *samp, int64_t nbatch,
 int tn_grid(tn_plan *plan, const void *samp, void *grid, int64_t nbatch,
             const float *weight, tn_stream stream);
 
+/* Low-rank (subspace) interpolation. The plan must hold every frame's
+ * samples as one trajectory (nframes = 1, npts = nframes_lr * frame_npts,
+ * frame f = samples [f*frame_npts, (f+1)*frame_npts)).
+ * grid (nbatch, ncoeff, *G) complex64 -> samp (nbatch, npts) complex64 with
+ * samp[b, p] = sum_k basis_adjoint[f(p), k] * interp(grid[b, k])[p] / scale;
+ * basis_adjoint is a device array (nframes_lr, ncoeff) complex64; weight as
+ * in tn_interp. Replaces _interpolate_lowrank{2,3} / _interpolate_lowrank_cuda{2,3}
+ * (src/deepmr/fft/_interp/interp.py:268-357, :559-649). */
+int tn_interp_lowrank(tn_plan *plan, const void *grid, void *samp, int64_t nbatch,
+                      int32_t ncoeff, int32_t frame_npts, const void *basis_adjoint,
+                      const float *weight, tn_stream stream);
+
+/* Low-rank (subspace) gridding, same plan convention as tn_interp_lowrank.
+ * samp (nbatch, npts) complex64 -> grid (nbatch, ncoeff, *G) complex64 with
+ * grid[b, k] = sum_f basis[k, f] * grid(samp[b, frame f]) / scale; basis is
+ * a device array (ncoeff, nframes_lr) complex64. Every grid cell is written.
+ * Replaces _gridding_lowrank{2,3} / _gridding_lowrank_cuda{2,3}_cplx
+ * (src/deepmr/fft/_interp/grid.py:326-417, :837-931). */
+int tn_grid_lowrank(tn_plan *plan, const void *samp, void *grid, int64_t nbatch,
+                    int32_t ncoeff, int32_t frame_npts, const void *basis,
+                    const float *weight, tn_stream stream);
+
 /* Fused apodize + centred zero-pad + ifftshift: image (nimg, *N) ->
  * grid (nimg, *G) ready for an unnormalised forward FFT. Replaces
  * _apodize + resize + the ifftshift of fft.fft

@@ -239,3 +239,19 @@ def test_grid_split_matches_oracle_and_is_deterministic(split):
     torch.cuda.synchronize()
     assert torch.equal(a.view(torch.float32), b.view(torch.float32))
     assert rel(a.cpu().numpy(), orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0))) < RTOL
+
+
+def test_conjugate_view_inputs():
+    """Lazy conjugate views (x.conj()) are materialised before the kernels."""
+    n = 32
+    print("conj-view inputs", flush=True)
+    coord = testing.golden_angle_radial(16, 64, n)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV)
+    x = torch.from_numpy(cnormal((2, n, n), 21)).to(DEV)
+    y1 = tilenufft.fft.apply_nufft(x.conj(), plan)
+    y2 = tilenufft.fft.apply_nufft(x.conj().resolve_conj(), plan)
+    assert torch.equal(y1, y2)
+    k = torch.from_numpy(cnormal(tuple(y1.shape), 22)).to(DEV)
+    a1 = tilenufft.fft.apply_nufft_adj(k.conj(), plan)
+    a2 = tilenufft.fft.apply_nufft_adj(k.conj().resolve_conj(), plan)
+    assert torch.equal(a1, a2)

@@ -174,3 +174,28 @@ def test_toeplitz_is_proportional_to_normal_operator():
     c = np.vdot(aha, t) / np.vdot(aha, aha)
     assert abs(c.imag) < 1e-2 * abs(c.real)
     assert rel(t, c * aha) < 5e-2
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_lowrank_equals_basis_combination_of_plain(ndim):
+    """A general complex subspace basis (ncoeff != nframes) against the plain,
+    already pinned per-frame operators: the forward of frame f is
+    sum_c basis_adjoint[f, c] * A_f(x_c) and the adjoint's coefficient c is
+    sum_f basis[c, f] * A_f^H(y_f). A transposed or conjugated basis fails."""
+    rng = np.random.default_rng(11 + ndim)
+    nf, nc, n, m = 5, 2, (10 if ndim == 2 else 6), 21
+    coord = rng.uniform(-n / 2, n / 2, (nf, 1, m, ndim)).astype(np.float32)
+    plan = orc.plan_nufft(coord, n, 4, 2.0)
+    ba = cn(rng, (nf, nc))
+    x = cn(rng, (2, nc) + (n,) * ndim)
+    y_lr = orc.apply_nufft(x, plan, basis_adjoint=ba)
+    # plain: frame f sees image sum_c ba[f, c] x_c
+    xf = np.einsum("fc,bc...->bf...", ba, x).astype(np.complex64)
+    np.testing.assert_allclose(y_lr, orc.apply_nufft(xf, plan), rtol=2e-5, atol=2e-5)
+
+    basis = cn(rng, (nc, nf))
+    k = cn(rng, (2, nf, 1, m))
+    x_lr = orc.apply_nufft_adj(k, plan, basis=basis)
+    per_frame = orc.apply_nufft_adj(k, plan)  # (2, nf, *N)
+    ref = np.einsum("cf,bf...->bc...", basis, per_frame)
+    np.testing.assert_allclose(x_lr, ref, rtol=2e-5, atol=2e-5)
