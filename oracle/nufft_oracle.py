@@ -535,33 +535,75 @@ def nufft_adj(kspace, coord, shape, width=4, oversamp=1.25, basis=None):
 # --------------------------------------------------------------------------
 # Toeplitz normal operator (dm/fft/toeplitz.py:15-157, dm/fft/nufft.py:609-669)
 # --------------------------------------------------------------------------
-def plan_toeplitz(coord, shape, dcf=None, width=4):
-    """PSF kernel on the 2x grid (non-low-rank), dm/fft/toeplitz.py:15-157."""
+def plan_toeplitz(coord, shape, dcf=None, width=4, basis=None):
+    """PSF kernel on the 2x grid, dm/fft/toeplitz.py:15-157.
+
+    With a subspace ``basis`` (ncoeff, nframes) the PSF of coefficient pair
+    (c1 -> c2) is nufft_adj(dcf * conj(basis[c1, f]), basis)[c2] (delta at
+    :97-101), and the kernel is returned as (voxels, c2, c1) (:143-147);
+    a real basis keeps the real part only (:137-138).
+    """
     coord = np.asarray(coord, np.float32)
     ndim = coord.shape[-1]
     oversamp = np.asarray([2.0] * ndim)
     shape = np.asarray([shape] * ndim if np.isscalar(shape) else list(shape)[-ndim:], np.int16)
     dcf = np.ones(coord.shape[:-1], np.float32) if dcf is None else np.asarray(dcf, np.float32)
+    isreal = False
+    if basis is not None:
+        basis = np.asarray(basis)
+        isreal = not np.iscomplexobj(basis)
+        basis = basis.astype(np.float32 if isreal else np.complex64)
+        adj = basis.conj().T  # (nframes, ncoeff)
+        ncoeff = basis.shape[0]
+        delta = np.ones((ncoeff,) + coord.shape[:-1], np.float32)
+        delta = (delta.T * adj).T
+    else:
+        delta = np.ones(coord.shape[:-1], np.float32)
     is_cart = cartesian_axes(coord, shape)
     oversamp[is_cart] = 1.0
     kshape = np.ceil(oversamp * shape).astype(np.int16)
     kcoord = (coord * oversamp.astype(np.float32)).astype(np.float32)
-    psf = nufft_adj(dcf.astype(np.complex64), kcoord, [int(s) for s in kshape], width=width)
+    psf = nufft_adj((dcf * delta).astype(np.complex64), kcoord, [int(s) for s in kshape], width=width,
+                    basis=None if basis is None else basis.astype(np.complex64))
     ax = tuple(range(-ndim, 0))
     kern = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(psf, axes=ax), axes=ax, norm="ortho"), axes=ax)
     if kern.shape[0] == 1:
         kern = kern[0]
+    if isreal:
+        kern = kern.real
     kern = np.fft.ifftshift(kern, axes=ax)
+    if basis is not None:
+        kern = kern.reshape(*kern.shape[:2], -1).transpose(2, 1, 0)
     kern = kern / np.mean(np.abs(kern[kern != 0]))
-    return np.nan_to_num(kern).astype(np.complex64), tuple(int(s) for s in kshape)
+    kern = np.nan_to_num(kern)
+    return np.ascontiguousarray(kern).astype(np.float32 if isreal else np.complex64), \
+        tuple(int(s) for s in kshape)
 
 
-def apply_selfadj(image, kern, kshape):
-    """Pad -> FFT (ortho) -> kernel product -> IFFT (ortho) -> crop (dm/fft/nufft.py:609-669)."""
+def apply_selfadj(image, kern, kshape, lowrank=False, matvec=False):
+    """Pad -> FFT (ortho) -> kernel product -> IFFT (ortho) -> crop (dm/fft/nufft.py:609-669).
+
+    Low rank (dm/fft/_interp/toeplitz.py:51-84, :109-130): k-space as
+    (voxels, batch, ncoeff); deepmr's per-voxel product is
+    out[j] += K[i][j] * in[j] over i (each coefficient scaled by the column
+    sum of the voxel's kernel). ``matvec`` instead forms out[i] = sum_j
+    K[i][j] * in[j], the Gram matrix product.
+    """
     x = np.asarray(image).astype(np.complex64)
     ndim = len(kshape)
     oshape = x.shape[-ndim:]
     x = resize(x, list(x.shape[:-ndim]) + list(kshape))
     ax = tuple(range(-ndim, 0))
-    y = np.fft.ifftn(kern * np.fft.fftn(x, axes=ax, norm="ortho"), axes=ax, norm="ortho")
+    k = np.fft.fftn(x, axes=ax, norm="ortho")
+    if lowrank:
+        shp = k.shape
+        kin = k.reshape(-1, shp[-ndim - 1], int(np.prod(shp[-ndim:]))).transpose(2, 0, 1)
+        if matvec:
+            kout = np.einsum("vij,vbj->vbi", kern, kin)
+        else:
+            kout = kin * kern.sum(axis=1)[:, None, :]
+        k = kout.transpose(1, 2, 0).reshape(shp)
+    else:
+        k = kern * k
+    y = np.fft.ifftn(k, axes=ax, norm="ortho")
     return resize(y, list(y.shape[:-ndim]) + list(oshape)).astype(np.complex64)

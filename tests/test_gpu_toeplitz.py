@@ -50,3 +50,24 @@ def test_toeplitz_matches_oracle(ndim):
     assert rel(y, orc.apply_selfadj(x, ref_k, kshape)) < 1e-5
     op = tilenufft.linops.NUFFTGramOp(torch.from_numpy(coord), n, width=4, device=DEV)
     assert rel(op(torch.from_numpy(x).to(DEV)).cpu().numpy(), y) < 1e-6
+
+
+@pytest.mark.parametrize("kind, product", list(itertools.product(["complex", "real"], ["deepmr", "matvec"])))
+def test_lowrank_toeplitz_matches_oracle(kind, product):
+    n, nf, nc = 24, 6, 3
+    print(f"low-rank toeplitz basis={kind} product={product}", flush=True)
+    coord = np.stack([testing.vd_spiral(2, 128, n, rotation=f * testing.GOLDEN_ANGLE) for f in range(nf)])
+    rng = np.random.default_rng(31)
+    if kind == "complex":
+        basis = (rng.standard_normal((nc, nf)) + 1j * rng.standard_normal((nc, nf))).astype(np.complex64)
+    else:
+        basis = rng.standard_normal((nc, nf)).astype(np.float32)
+    kern = tilenufft.fft.plan_toeplitz_nufft(torch.from_numpy(coord), n, basis=torch.from_numpy(basis),
+                                             device=DEV, lowrank_product=product)
+    k_ref, ks = orc.plan_toeplitz(coord, n, basis=basis)
+    assert kern.islowrank and tuple(kern.value.shape) == k_ref.shape
+    assert rel(kern.value.cpu().numpy(), k_ref) < 1e-5
+    x = testing.complex_normal((2, nc, n, n), seed=32)
+    out = tilenufft.fft.apply_nufft_selfadj(x.to(DEV), kern).cpu().numpy()
+    ref = orc.apply_selfadj(x.numpy(), k_ref, ks, lowrank=True, matvec=product == "matvec")
+    assert rel(out, ref) < 1e-5

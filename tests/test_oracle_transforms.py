@@ -199,3 +199,40 @@ def test_lowrank_equals_basis_combination_of_plain(ndim):
     per_frame = orc.apply_nufft_adj(k, plan)  # (2, nf, *N)
     ref = np.einsum("cf,bf...->bc...", basis, per_frame)
     np.testing.assert_allclose(x_lr, ref, rtol=2e-5, atol=2e-5)
+
+
+def test_lowrank_toeplitz_identity_basis_equals_per_frame():
+    """Identity subspace: the coefficient pairs decouple, so both low-rank
+    products (deepmr's column-sum scaling and the Gram mat-vec) equal the
+    plain multi-frame Toeplitz operator."""
+    rng = np.random.default_rng(21)
+    nf, n = 3, 12
+    coord = rng.uniform(-n / 2, n / 2, (nf, 2, 20, 2)).astype(np.float32)
+    kern, ks = orc.plan_toeplitz(coord, n)
+    kern_lr, ks_lr = orc.plan_toeplitz(coord, n, basis=np.eye(nf, dtype=np.complex64))
+    assert ks == ks_lr and kern_lr.shape == (int(np.prod(ks)), nf, nf)
+    x = cn(rng, (2, nf, n, n))
+    ref = orc.apply_selfadj(x, kern, ks)
+    for matvec in (False, True):
+        out = orc.apply_selfadj(x, kern_lr, ks, lowrank=True, matvec=matvec)
+        assert rel(out, ref) < 1e-5
+
+
+def test_lowrank_toeplitz_kernel_is_basis_combination_of_frame_kernels():
+    """K[v, c2, c1] = s * sum_f basis[c2, f] conj(basis[c1, f]) K_f[v] for one
+    scalar s (the two normalisations): pins the (c1, c2) order and the conj."""
+    rng = np.random.default_rng(22)
+    nf, nc, n = 4, 2, 10
+    coord = rng.uniform(-n / 2, n / 2, (nf, 2, 16, 2)).astype(np.float32)
+    basis = cn(rng, (nc, nf))
+    kf, ks = orc.plan_toeplitz(coord, n)  # (nf, *2N)
+    klr, _ = orc.plan_toeplitz(coord, n, basis=basis)
+    comb = np.einsum("if,jf,fv->vij", basis, basis.conj(), kf.reshape(nf, -1))
+    s = np.vdot(comb, klr) / np.vdot(comb, comb)
+    assert rel(klr, s * comb) < 1e-5
+    # and the mat-vec mode is then the Gram of the low-rank NUFFT pair up to s
+    x = cn(rng, (1, nc, n, n))
+    y = orc.apply_selfadj(x, klr, ks, lowrank=True, matvec=True)
+    per_frame = orc.apply_selfadj(np.einsum("cf,bc...->bf...", basis.conj(), x).astype(np.complex64), kf, ks)
+    ref = s * np.einsum("cf,bf...->bc...", basis, per_frame)
+    assert rel(y, ref) < 1e-5
