@@ -129,11 +129,15 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                                                        (j.IT[1] + j.W[1] - 1) * (j.IT[2] + j.W[2] - 1));
     }
     const int wc = t.kind_grid % 16;
-    if (grid_lds_bytes(1, max_plane, wc) > kGridLdsBudget)
+    const int gmode = plan_of.empty() ? kGridSpread : plan_of[0]->grid_mode;
+    for (const tn_plan *pl : plan_of)
+        if (pl->grid_mode != gmode) return fail(TN_E_ARG, "batched plans must share the gridding variant");
+    t.grid_mode = gmode;
+    if (grid_lds_bytes(1, max_plane, wc, gmode) > kGridLdsBudget)
         return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
-                    grid_lds_bytes(1, max_plane, wc));
+                    grid_lds_bytes(1, max_plane, wc, gmode));
     int lds_coils = 1;
-    while (lds_coils < 64 && grid_lds_bytes(lds_coils + 1, max_plane, wc) <= kGridLdsBudget) ++lds_coils;
+    while (lds_coils < 64 && grid_lds_bytes(lds_coils + 1, max_plane, wc, gmode) <= kGridLdsBudget) ++lds_coils;
     const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
     const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
 
@@ -202,7 +206,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     }
     t.scratch_floats = scratch;
     t.grid_coils = max_gc;
-    t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc);
+    t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc, gmode);
     t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
     t.interp_coils = max_ic;
     return upload_tables(jobs, interp, grid, fold, slot_base, t);
@@ -237,6 +241,7 @@ int validate(const tn_plan_params *p) {
     if ((int64_t)p->grid[0] * p->grid[1] * p->grid[2] >= INT32_MAX)
         return fail(TN_E_RANGE, "grid too large");
     if (p->interp_mode < 0 || p->interp_mode > 2) return fail(TN_E_ARG, "interp_mode must be 0, 1 or 2");
+    if (p->grid_mode < 0 || p->grid_mode > 2) return fail(TN_E_ARG, "grid_mode must be 0, 1 or 2");
     return TN_OK;
 }
 
@@ -360,6 +365,16 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
         return fail(TN_E_HIP, "hipGetDevice failed");
     }
     choose_tiles(pl);
+    {
+        int taps = 1;
+        for (int d = 0; d < 3; ++d) taps *= params->width[d];
+        const bool runs_ok = taps <= kRunsMaxTaps;
+        if (params->grid_mode == kGridRuns && !runs_ok) {
+            delete pl;
+            return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per sample (got %d)", kRunsMaxTaps, taps);
+        }
+        pl->grid_mode = params->grid_mode == kGridSpread ? kGridSpread : runs_ok ? kGridRuns : kGridSpread;
+    }
     const int64_t ncoord = (int64_t)params->nframes * params->npts * params->ndim;
     auto cleanup = [&](int code) {
         destroy(pl);

@@ -24,8 +24,15 @@ constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgr
 constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch
 constexpr int kGridPad = 4;      // accumulator row padding (floats)
 
+// Gridding variants: one LDS atomic per tap and sample over entries spread
+// across footprint origins, or per-lane register runs over origin-sorted
+// entries (for at most kRunsMaxTaps taps per sample).
+enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2 };
+constexpr int kRunsMaxTaps = 64;
+
 // Dynamic LDS bytes of the gridding kernel for `ncoil` coils of a tile.
-inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc) {
+inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc, int mode = kGridSpread) {
+    if (mode == kGridRuns) return sizeof(float) * 2 * (size_t)ncoil * plane;
     return sizeof(float) * (2 * (size_t)ncoil * plane + 4 * (size_t)kGridBatch +
                             3 * (size_t)wc * kGridBatch + 2 * (size_t)ncoil * kGridBatch);
 }
@@ -102,6 +109,7 @@ struct LaunchTables {
     int interp_mode = kInterpGather;
     size_t interp_lds = 0;     // dynamic LDS bytes for the staged interpolation
     int interp_coils = 1;      // coils per LDS region load (staged interpolation)
+    int grid_mode = kGridSpread;  // gridding variant (matches the plans' entry order)
 };
 
 }  // namespace tn
@@ -125,6 +133,7 @@ struct tn_plan {
     float *d_apod[3] = {nullptr, nullptr, nullptr};
     int64_t nentries = 0;
     int64_t footprint = -1;
+    int grid_mode = tn::kGridSpread;  // effective gridding variant (entry order)
     std::map<int64_t, tn::LaunchTables> cache;
     std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank)
 };

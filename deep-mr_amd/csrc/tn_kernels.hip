@@ -282,6 +282,63 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
     }
 }
 
+// Writes a finished LDS tile accumulator ([coil][re, im][rows][P]) of a
+// gridding item: straight into the grid (every in-grid cell, zeros
+// included), or, for a split tile, as partial `part` of fold slot `slot`
+// ([coil][cell](re, im)), summed later by fold_kernel in part order.
+__device__ __forceinline__ void store_tile(const Job &J, const Item &it, const float *acc, int plane,
+                                           int P, int ox, int oy, int oz, int ex, int ey, int ez,
+                                           float2 *__restrict__ grid, float *__restrict__ scratch,
+                                           const int64_t *__restrict__ slot_base) {
+    const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
+    const int Gx = J.G[0], Gy = J.G[1];
+    const int ncoil = it.c1 - it.c0;
+    const int cells = Tx * Ty * Tz;
+    const int tid = threadIdx.x;
+    if (it.slot >= 0) {
+        float *dst = scratch + slot_base[it.slot] + (int64_t)it.part * 2 * ncoil * cells;
+        for (int i = tid; i < ncoil * cells; i += kBlock) {
+            const int b = i / cells;
+            const int cell = i - b * cells;
+            const int lx = cell % Tx;
+            const int row = cell / Tx;
+            const float *a = acc + 2 * b * plane + row * P + lx;
+            dst[2 * i] = a[0];
+            dst[2 * i + 1] = a[plane];
+        }
+        return;
+    }
+    for (int i = tid; i < ncoil * cells; i += kBlock) {
+        const int b = i / cells;
+        const int cell = i - b * cells;
+        const int lx = cell % Tx;
+        const int ly = (cell / Tx) % Ty;
+        const int lz = cell / (Tx * Ty);
+        if (lx < ex && ly < ey && lz < ez) {
+            const float *a = acc + 2 * b * plane + (lz * Ty + ly) * P + lx;
+            grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
+                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(a[0], a[plane]);
+        }
+    }
+}
+
+// Scaled sample value of accumulator coil `vc` (0-based within the job) for
+// sample p. Low rank: accumulator coils are (coil, coeff) pairs and the
+// sample of frame f enters coefficient k times basis[k, f].
+__device__ __forceinline__ float2 grid_sample(const Job &J, const float2 *__restrict__ samp, int vc,
+                                              int p, float sc, const LowRank &lr) {
+    float2 v;
+    if (lr.ncoeff > 0) {
+        const int b = vc / lr.ncoeff, k = vc - b * lr.ncoeff;
+        const float2 s = samp[J.samp_off + (int64_t)b * J.samp_bstride + p];
+        const float2 w = lr.coef[(int64_t)k * lr.nframes + p / lr.frame_npts];
+        v = make_float2(w.x * s.x - w.y * s.y, w.x * s.y + w.y * s.x);
+    } else {
+        v = samp[J.samp_off + (int64_t)vc * J.samp_bstride + p];
+    }
+    return make_float2(v.x * sc, v.y * sc);
+}
+
 // Gridding. One workgroup owns one tile for a set of coils and walks the
 // tile's entries in batches of kGridBatch:
 //   A. one lane per entry evaluates the taps (per axis: wrapped origin + W
@@ -385,19 +442,7 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
             const int p = sidx[j];
             float sc = J.inv_scale;
             if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-            float2 v;
-            if (lr.ncoeff > 0) {
-                const int vc = it.c0 + c;
-                const int b = vc / lr.ncoeff, k = vc - b * lr.ncoeff;
-                const float2 s = samp[J.samp_off + (int64_t)b * J.samp_bstride + p];
-                const float2 w = lr.coef[(int64_t)k * lr.nframes + p / lr.frame_npts];
-                v = make_float2(w.x * s.x - w.y * s.y, w.x * s.y + w.y * s.x);
-            } else {
-                v = samp[J.samp_off + (int64_t)(it.c0 + c) * J.samp_bstride + p];
-            }
-            v.x *= sc;
-            v.y *= sc;
-            sval[c * B + j] = v;
+            sval[c * B + j] = grid_sample(J, samp, it.c0 + c, p, sc, lr);
         }
         __syncthreads();
         // B. spread, one entry per lane. The tap loop is branch-free: taps
@@ -456,34 +501,179 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
     }
     __syncthreads();
 
-    const int cells = Tx * Ty * Tz;
-    if (it.slot >= 0) {
-        // split tile: store this partial ([coil][cell](re, im)); the fold
-        // kernel sums the parts in part order and writes the tile
-        float *dst = scratch + slot_base[it.slot] + (int64_t)it.part * 2 * ncoil * cells;
-        for (int i = tid; i < ncoil * cells; i += kBlock) {
-            const int b = i / cells;
-            const int cell = i - b * cells;
-            const int lx = cell % Tx;
-            const int row = cell / Tx;
-            const float *a = acc + 2 * b * plane + row * P + lx;
-            dst[2 * i] = a[0];
-            dst[2 * i + 1] = a[plane];
-        }
-        return;
+    store_tile(J, it, acc, plane, P, ox, oy, oz, ex, ey, ez, grid, scratch, slot_base);
+}
+
+// Gridding by register runs. The plan sorts each tile's entries by
+// footprint origin. Every wave walks all entries of the item: lane l owns
+// the contiguous run [e0 + l*R, e0 + (l+1)*R), evaluates the taps of each
+// entry and keeps, for the current origin, the W^d tap sums of every coil
+// it owns in registers; when the origin changes (and at the end of the run)
+// the sums are added into the LDS tile with one atomic per tap. Where many
+// samples share a footprint origin (dense trajectory centres) this replaces
+// one heavily conflicting LDS atomic per sample and tap with one per run.
+// Waves own coils (or row bands when there are fewer coils than waves), so
+// each accumulator cell is updated by one wave in program order: the result
+// is deterministic.
+template <int NDIM, int WC>
+__global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict__ jobs,
+                                                           const Item *__restrict__ items,
+                                                           const float2 *__restrict__ samp,
+                                                           float2 *__restrict__ grid,
+                                                           const float *__restrict__ weight,
+                                                           float *__restrict__ scratch,
+                                                           const int64_t *__restrict__ slot_base,
+                                                           LowRank lr) {
+    constexpr int KZ = NDIM > 2 ? WC : 1;
+    constexpr int KZY = KZ * WC;
+    constexpr int TAPS = KZY * WC;
+    constexpr int CW = (NDIM == 2 && WC <= 4) ? 2 : 1;  // coils per pass (registers)
+    extern __shared__ float smem[];
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int tile = it.unit;
+    const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
+    const int tx = tile % J.NT[0];
+    const int ty = (tile / J.NT[0]) % J.NT[1];
+    const int tz = tile / (J.NT[0] * J.NT[1]);
+    const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
+    const int ex = min(Tx, J.G[0] - ox), ey = min(Ty, J.G[1] - oy), ez = min(Tz, J.G[2] - oz);
+    const int P = Tx + kGridPad;
+    const int rowsP = Tz * Ty * P;
+    const int plane = rowsP + kWave;  // + one trash cell per lane
+    const int ncoil = it.c1 - it.c0;
+    const int Gx = J.G[0], Gy = J.G[1], Gz = J.G[2];
+    const int nx = J.W[0];
+    const int ny = NDIM > 1 ? J.W[1] : 1;
+    const int nz = NDIM > 2 ? J.W[2] : 1;
+    float *acc = smem;  // [ncoil][re, im][Tz*Ty*P + trash]
+
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave, lane = tid % kWave;
+    const int rows = Tz * Ty;
+    int coil0, cstep, row_lo = 0, row_hi = rows;
+    if (ncoil >= kBlock / kWave) {
+        coil0 = wave;
+        cstep = kBlock / kWave;
+    } else {
+        const int nb = (kBlock / kWave) / ncoil;
+        const int band = wave / ncoil;
+        coil0 = band < nb ? wave % ncoil : ncoil;
+        cstep = ncoil;
+        const int rpb = (rows + nb - 1) / nb;
+        row_lo = min(rows, band * rpb);
+        row_hi = min(rows, row_lo + rpb);
     }
-    for (int i = tid; i < ncoil * cells; i += kBlock) {
-        const int b = i / cells;
-        const int cell = i - b * cells;
-        const int lx = cell % Tx;
-        const int ly = (cell / Tx) % Ty;
-        const int lz = cell / (Tx * Ty);
-        if (lx < ex && ly < ey && lz < ez) {
-            const float *a = acc + 2 * b * plane + (lz * Ty + ly) * P + lx;
-            grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
-                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(a[0], a[plane]);
+    for (int i = tid; i < 2 * ncoil * plane; i += kBlock) acc[i] = 0.0f;
+    __syncthreads();
+
+    const int n = it.e1 - it.e0;
+    const int R = (n + kWave - 1) / kWave;
+    const int s0 = it.e0 + lane * R;
+    const int s1 = min(it.e1, s0 + R);
+    const int trash = rowsP + lane;
+    for (int cb = coil0; cb < ncoil; cb += cstep * CW) {
+        float ar[CW][TAPS], ai[CW][TAPS];
+#pragma unroll
+        for (int q = 0; q < CW; ++q)
+#pragma unroll
+            for (int t = 0; t < TAPS; ++t) ar[q][t] = ai[q][t] = 0.0f;
+        int cx = -1, cy = 0, cz = 0;  // current footprint origin (-1: none)
+        for (int e = s0; e <= s1; ++e) {
+            int bx = -1, by = 0, bz = 0;
+            float w[3][WC];
+            float4 en = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (e < s1) {
+                en = J.entries[e];
+                const float c3[3] = {en.x, en.y, en.z};
+                int base[3] = {0, 0, 0};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (d < NDIM) {
+                        axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base[d], w[d]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < WC; ++k) w[d][k] = k == 0 ? 1.0f : 0.0f;
+                    }
+                }
+                bx = base[0];
+                by = base[1];
+                bz = base[2];
+            }
+            if (cx >= 0 && (bx != cx || by != cy || bz != cz)) {
+                // flush the sums of origin (cx, cy, cz)
+                int lx[WC], lrow[KZY];
+#pragma unroll
+                for (int k = 0; k < WC; ++k) {
+                    const int ax = wrap_add(cx, k, Gx) - ox;
+                    lx[k] = (k < nx && ax >= 0 && ax < ex) ? ax : -1;
+                }
+#pragma unroll
+                for (int kz = 0; kz < KZ; ++kz) {
+                    const int az = wrap_add(cz, kz, Gz) - oz;
+#pragma unroll
+                    for (int ky = 0; ky < WC; ++ky) {
+                        const int ay = wrap_add(cy, ky, Gy) - oy;
+                        const int row = az * Ty + ay;
+                        const bool ok = kz < nz && az >= 0 && az < ez && ky < ny && ay >= 0 && ay < ey &&
+                                        row >= row_lo && row < row_hi;
+                        lrow[kz * WC + ky] = ok ? row * P : -1;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < CW; ++q) {
+                    const int c = cb + q * cstep;
+                    if (c < ncoil) {
+                        float *are = acc + 2 * c * plane;
+#pragma unroll
+                        for (int zy = 0; zy < KZY; ++zy) {
+#pragma unroll
+                            for (int kx = 0; kx < WC; ++kx) {
+                                const bool ok = lrow[zy] >= 0 && lx[kx] >= 0;
+                                const int a = ok ? lrow[zy] + lx[kx] : trash;
+                                __hip_atomic_fetch_add(are + a, ar[q][zy * WC + kx], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                                __hip_atomic_fetch_add(are + plane + a, ai[q][zy * WC + kx], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                                ar[q][zy * WC + kx] = 0.0f;
+                                ai[q][zy * WC + kx] = 0.0f;
+                            }
+                        }
+                    }
+                }
+            }
+            if (e >= s1) break;
+            cx = bx;
+            cy = by;
+            cz = bz;
+            const int p = __float_as_int(en.w);
+            float sc = J.inv_scale;
+            if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
+            float wzy[KZY];
+#pragma unroll
+            for (int kz = 0; kz < KZ; ++kz)
+#pragma unroll
+                for (int ky = 0; ky < WC; ++ky) wzy[kz * WC + ky] = w[2][kz] * w[1][ky];
+#pragma unroll
+            for (int q = 0; q < CW; ++q) {
+                const int c = cb + q * cstep;
+                if (c < ncoil) {
+                    const float2 sv = grid_sample(J, samp, it.c0 + c, p, sc, lr);
+#pragma unroll
+                    for (int zy = 0; zy < KZY; ++zy) {
+                        const float fr = wzy[zy] * sv.x, fi = wzy[zy] * sv.y;
+#pragma unroll
+                        for (int kx = 0; kx < WC; ++kx) {
+                            ar[q][zy * WC + kx] += w[0][kx] * fr;
+                            ai[q][zy * WC + kx] += w[0][kx] * fi;
+                        }
+                    }
+                }
+            }
         }
     }
+    __syncthreads();
+    store_tile(J, it, acc, plane, P, ox, oy, oz, ex, ey, ez, grid, scratch, slot_base);
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
@@ -654,6 +844,27 @@ StagedFn pick_staged(int wc) {
 }
 
 template <int NDIM>
+GridFn pick_runs(int wc) {
+    switch (wc) {
+        case 1: return grid_runs_kernel<NDIM, 1>;
+        case 2: return grid_runs_kernel<NDIM, 2>;
+        case 3: return grid_runs_kernel<NDIM, 3>;
+        case 4: return grid_runs_kernel<NDIM, 4>;
+        default: break;
+    }
+    if constexpr (NDIM == 2) {
+        switch (wc) {
+            case 5: return grid_runs_kernel<2, 5>;
+            case 6: return grid_runs_kernel<2, 6>;
+            case 7: return grid_runs_kernel<2, 7>;
+            case 8: return grid_runs_kernel<2, 8>;
+            default: break;
+        }
+    }
+    return nullptr;
+}
+
+template <int NDIM>
 GridFn pick_grid(int wc) {
     switch (wc) {
         case 1: return grid_kernel<NDIM, 1>;
@@ -703,8 +914,12 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                 hipStream_t stream, LowRank lr) {
     if (t.n_grid == 0) return TN_OK;
     const int ndim = t.kind_grid / 16, wc = t.kind_grid % 16;
-    GridFn fn = ndim == 2 ? pick_grid<2>(wc) : ndim == 3 ? pick_grid<3>(wc) : nullptr;
-    if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
+    GridFn fn = nullptr;
+    if (t.grid_mode == kGridRuns)
+        fn = ndim == 2 ? pick_runs<2>(wc) : ndim == 3 ? pick_runs<3>(wc) : nullptr;
+    else
+        fn = ndim == 2 ? pick_grid<2>(wc) : ndim == 3 ? pick_grid<3>(wc) : nullptr;
+    if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d, mode=%d)", ndim, wc, t.grid_mode);
     hipLaunchKernelGGL(fn, dim3((unsigned)t.n_grid), dim3(kBlock), t.grid_lds, stream, t.d_jobs,
                        t.d_grid, samp, grid, weight, t.d_scratch, t.d_slot_base, lr);
     TN_HIP(hipGetLastError());

@@ -268,11 +268,16 @@ namespace {
 
 // Stable radix sort of (key, sample) pairs, then per-bin offsets and the
 // sorted entry records (coordinates + sample index).
-int spread_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, float4 *entries,
-                     hipStream_t stream);
+int order_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, float4 *entries,
+                    bool spread, hipStream_t stream);
+
+// Entry order inside each bin: sample order, footprint-origin order (runs
+// of equal origin, gridding by register runs) or round-robin over origins
+// (gridding by per-sample LDS atomics).
+enum EntryOrder { kOrderSample = 0, kOrderOrigin = 1, kOrderSpread = 2 };
 
 int sort_entries(const BinGeom &geo, const float *coord, void *keys, void *vals, int64_t n,
-                 int32_t *bin_off, float4 **entries, hipStream_t stream, bool spread = false) {
+                 int32_t *bin_off, float4 **entries, hipStream_t stream, int order = kOrderSample) {
     const int64_t nbins = geo.nframes * geo.ntiles;
     DevBuf keys_sorted, vals_sorted, sort_tmp;
     TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
@@ -297,16 +302,17 @@ int sort_entries(const BinGeom &geo, const float *coord, void *keys, void *vals,
                        coord, (const uint32_t *)keys_sorted.p, (const int32_t *)vals_sorted.p, n,
                        *entries);
     TN_HIP(hipGetLastError());
-    if (spread) {
-        int rc = spread_by_origin(geo, (const uint32_t *)keys_sorted.p, n, *entries, stream);
+    if (order != kOrderSample) {
+        int rc = order_by_origin(geo, (const uint32_t *)keys_sorted.p, n, *entries, order == kOrderSpread,
+                                 stream);
         if (rc) return rc;
     }
     TN_HIP(hipStreamSynchronize(stream));  // temporaries are freed on scope exit
     return TN_OK;
 }
 
-int spread_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, float4 *entries,
-                     hipStream_t stream) {
+int order_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, float4 *entries,
+                    bool spread, hipStream_t stream) {
     DevBuf k1, v1, k1s, v1s, head, start, k2, v2s, tmp, copy;
     TN_HIP(hipMalloc(&k1.p, sizeof(uint64_t) * n));
     TN_HIP(hipMalloc(&v1.p, sizeof(int32_t) * n));
@@ -331,6 +337,14 @@ int spread_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, fl
     TN_HIP(hipMalloc(&tmp.p, std::max(bytes, std::max(b2, b3))));
     TN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, (uint64_t *)k1.p, (uint64_t *)k1s.p,
                                               (int32_t *)v1.p, (int32_t *)v1s.p, (int)n, 0, 64, stream));
+    if (!spread) {  // (bin, origin, sample) order
+        TN_HIP(hipMemcpyAsync(copy.p, entries, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL(permute_entries_kernel, dim3(nb), dim3(kBlock), 0, stream, (const float4 *)copy.p,
+                           (const int32_t *)v1s.p, n, entries);
+        TN_HIP(hipGetLastError());
+        TN_HIP(hipStreamSynchronize(stream));
+        return TN_OK;
+    }
     hipLaunchKernelGGL(group_head_kernel, dim3(nb), dim3(kBlock), 0, stream, (const uint64_t *)k1s.p, n,
                        (int32_t *)head.p);
     TN_HIP(hipGetLastError());
@@ -404,7 +418,8 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
                            (int32_t *)vals.p);
         TN_HIP(hipGetLastError());
         int rc = sort_entries(geo, plan->d_coord, keys.p, vals.p, n, plan->d_bin_off,
-                              &plan->d_entries, stream, /*spread=*/true);
+                              &plan->d_entries, stream,
+                              plan->grid_mode == kGridRuns ? kOrderOrigin : kOrderSpread);
         if (rc) return rc;
         plan->h_bin_off.assign(nbins + 1, 0);
         TN_HIP(hipMemcpy(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1),

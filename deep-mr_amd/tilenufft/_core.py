@@ -161,7 +161,7 @@ class NativePlan:
 
     def __init__(self, coord, shape, grid, width, beta, inv_scale, shift, device,
                  apod=None, tile=None, interp_chunk=0, grid_chunk=0, itile=None, interp_mode=0,
-                 grid_split=0):
+                 grid_split=0, grid_mode=0):
         lib = _lib.load()
         coord = np.ascontiguousarray(coord, dtype=np.float32)
         if coord.ndim != 3:
@@ -190,6 +190,7 @@ class NativePlan:
         p.interp_chunk = int(interp_chunk)
         p.grid_chunk = int(grid_chunk)
         p.grid_split = int(grid_split)
+        p.grid_mode = int(grid_mode)
         p.inv_scale = float(inv_scale)
         keep = []
         for d in range(ndim):

@@ -34,6 +34,7 @@ class PlanParams(ctypes.Structure):
         ("interp_chunk", c_int32),
         ("grid_chunk", c_int32),
         ("grid_split", c_int32),
+        ("grid_mode", c_int32),
         ("inv_scale", c_float),
         ("apod", c_void_p * 3),
     ]

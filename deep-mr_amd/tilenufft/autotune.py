@@ -1,7 +1,8 @@
 """Autotuner for the launch parameters of the gridding / interpolation kernels.
 
-Tunable per plan: the gridding tile (bin size), the coils per gridding
-workgroup and the coils per interpolation work item. Candidates are timed on
+Tunable per plan: the gridding variant and tile (bin size), the coils per
+gridding workgroup, the hot-tile split threshold, and the interpolation
+variant, tile and coils per work item. Candidates are timed on
 the actual batch with HIP events; the choice is cached on disk keyed by the
 geometry (ndim, N, G, W, coils, samples per trajectory rounded) and the
 device name.
@@ -27,7 +28,8 @@ def default_candidates(ndim):
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
         itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
     return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16],
-                itiles=itiles, staged_chunks=[2, 4, 8, 16], grid_splits=[512, 1024, 4096, -1])
+                itiles=itiles, staged_chunks=[2, 4, 8, 16], grid_splits=[512, 1024, 4096, -1],
+                grid_modes=[2, 1])
 
 
 def _cache_path():
@@ -56,10 +58,10 @@ def _time(fn, reps, device):
     return float(np.median([s.elapsed_time(e) for s, e in zip(start, end)]))
 
 
-def _apply(plans, tile, ic, gc, mode=0, itile=None, split=0):
+def _apply(plans, tile, ic, gc, mode=0, itile=None, split=0, gmode=None):
     for p in plans:
         retune(p, tile=tile, interp_chunk=ic, grid_chunk=gc, interp_mode=mode, itile=itile,
-               grid_split=split)
+               grid_split=split, grid_mode=gmode)
 
 
 def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=False):
@@ -77,34 +79,39 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-    if use_cache and key in cache:
+    if use_cache and key in cache and "grid_mode" in cache[key]:
         best = cache[key]
         _apply(plans, tuple(best["tile"]), best["interp_chunk"], best["grid_chunk"],
                best.get("interp_mode", 0), tuple(best["itile"]) if best.get("itile") else None,
-               best.get("grid_split", 0))
+               best.get("grid_split", 0), best["grid_mode"])
         return dict(best, cached=True)
 
     dev = plans[0].interpolator.device
     cand = candidates or default_candidates(plans[0].ndim)
     results = []
     with torch.cuda.device(dev):
-        # 1) tile, automatic chunks
-        best_tile, best_t = None, float("inf")
-        for tile in cand["tiles"]:
-            _apply(plans, tile, 0, 0)
-            b = BatchedNUFFT(plans, ncoils)
-            t = _time(lambda: b.gridding(b.grid), reps, dev)
-            results.append(("tile", tile, t))
-            if verbose:
-                print(f"autotune tile {tile}: grid {t:.3f} ms", flush=True)
-            if t < best_t:
-                best_tile, best_t = tile, t
+        # 1) gridding variant x tile, automatic chunks
+        best_tile, best_t, best_gm = None, float("inf"), 0
+        for gm in cand.get("grid_modes", [0]):
+            for tile in cand["tiles"]:
+                try:
+                    _apply(plans, tile, 0, 0, gmode=gm)
+                    b = BatchedNUFFT(plans, ncoils)
+                    t = _time(lambda: b.gridding(b.grid), reps, dev)
+                except RuntimeError as exc:  # variant unsupported for this width
+                    results.append(("tile", (gm, tile), str(exc)))
+                    continue
+                results.append(("tile", (gm, tile), t))
+                if verbose:
+                    print(f"autotune grid_mode {gm} tile {tile}: grid {t:.3f} ms", flush=True)
+                if t < best_t:
+                    best_tile, best_t, best_gm = tile, t, gm
         # 2) coils per gridding workgroup
         best_gc, best_gt = 0, best_t
         for gc in cand["grid_chunks"]:
             if gc == 0:
                 continue
-            _apply(plans, best_tile, 0, gc)
+            _apply(plans, best_tile, 0, gc, gmode=best_gm)
             b = BatchedNUFFT(plans, ncoils)
             t = _time(lambda: b.gridding(b.grid), reps, dev)
             results.append(("grid_chunk", gc, t))
@@ -113,7 +120,7 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         # 2b) hot-tile split threshold
         best_split = 0
         for sp in cand.get("grid_splits", []):
-            _apply(plans, best_tile, 0, best_gc, split=sp)
+            _apply(plans, best_tile, 0, best_gc, split=sp, gmode=best_gm)
             b = BatchedNUFFT(plans, ncoils)
             t = _time(lambda: b.gridding(b.grid), reps, dev)
             results.append(("grid_split", sp, t))
@@ -128,7 +135,7 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         best_i, best_it = None, float("inf")
         for mode, itile, ic in interp_cands:
             try:
-                _apply(plans, best_tile, ic, best_gc, mode, itile, best_split)
+                _apply(plans, best_tile, ic, best_gc, mode, itile, best_split, best_gm)
                 b = BatchedNUFFT(plans, ncoils)
                 t = _time(lambda: b.interp(b.grid), reps, dev)
             except RuntimeError as exc:  # e.g. a tile too large for LDS
@@ -140,8 +147,9 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
             if t < best_it:
                 best_i, best_it = (mode, itile, ic), t
         mode, itile, best_ic = best_i
-        _apply(plans, best_tile, best_ic, best_gc, mode, itile, best_split)
-    best = dict(tile=list(best_tile), grid_chunk=best_gc, grid_split=best_split, interp_mode=mode,
+        _apply(plans, best_tile, best_ic, best_gc, mode, itile, best_split, best_gm)
+    best = dict(tile=list(best_tile), grid_mode=best_gm, grid_chunk=best_gc, grid_split=best_split,
+                interp_mode=mode,
                 itile=list(itile) if itile else None, interp_chunk=best_ic, grid_ms=best_gt,
                 interp_ms=best_it)
     if use_cache:

@@ -1,4 +1,5 @@
-"""Drop-in for the NUFFT part of ``deepmr.fft`` (src/deepmr/fft/__init__.py:145-286)."""
+"""Drop-in for the NUFFT and sparse-FFT parts of ``deepmr.fft``
+(src/deepmr/fft/__init__.py:33-286)."""
 
 __all__ = [
     "fft",
@@ -13,11 +14,25 @@ __all__ = [
     "get_autograd_mode",
     "plan_toeplitz_nufft",
     "apply_nufft_selfadj",
+    "sparse_fft",
+    "sparse_ifft",
+    "prepare_sampling",
+    "apply_sparse_fft",
+    "apply_sparse_ifft",
+    "plan_toeplitz_fft",
+    "apply_sparse_fft_selfadj",
 ]
 
 from .. import _interp  # noqa: F401  (deepmr.fft._interp)
 from .fft import fft, ifft  # noqa: F401
 from .toeplitz import apply_nufft_selfadj, plan_toeplitz_nufft  # noqa: F401
+from .sparse_fft import (  # noqa: F401
+    apply_sparse_fft,
+    apply_sparse_fft_selfadj,
+    apply_sparse_ifft,
+    plan_toeplitz_fft,
+    prepare_sampling,
+)
 from .nufft import (  # noqa: F401
     NUFFTPlan,
     apply_nufft,
@@ -26,6 +41,22 @@ from .nufft import (  # noqa: F401
     plan_nufft,
     set_autograd_mode,
 )
+
+
+def sparse_fft(image, indexes, basis_adjoint=None, device="cpu", threadsperblock=128):
+    """N-dimensional sparse FFT (src/deepmr/fft/__init__.py:33-86): centred
+    FFT of ``image`` (..., [ncontrasts], *shape) sampled at the integer
+    ``indexes`` (ncontrasts, nviews, nsamples, ndim), x first."""
+    ndim = indexes.shape[-1]
+    shape = image.shape[-ndim:]
+    mask = prepare_sampling(indexes, shape, device)
+    return apply_sparse_fft(image, mask, basis_adjoint, threadsperblock=threadsperblock)
+
+
+def sparse_ifft(kspace, indexes, shape, basis=None, device="cpu", threadsperblock=128):
+    """N-dimensional inverse sparse FFT (src/deepmr/fft/__init__.py:89-142)."""
+    mask = prepare_sampling(indexes, shape, device)
+    return apply_sparse_ifft(kspace, mask, basis, threadsperblock=threadsperblock)
 
 
 def nufft(image, coord, shape=None, basis_adjoint=None, device="cpu", threadsperblock=128,

@@ -65,6 +65,7 @@ class NUFFTPlan:
     itile: tuple = None
     interp_mode: int = 0
     grid_split: int = 0
+    grid_mode: int = 0
     lowrank_native: object = field(default=None, repr=False, compare=False)
 
     def to(self, device):
@@ -95,13 +96,15 @@ def _build_native(plan, device, shift, coord=None):
         1.0 / plan.scale, sh,
         device, apod=apod, tile=plan.tile, interp_chunk=plan.interp_chunk,
         grid_chunk=plan.grid_chunk, itile=plan.itile, interp_mode=plan.interp_mode,
-        grid_split=plan.grid_split,
+        grid_split=plan.grid_split, grid_mode=plan.grid_mode,
     )
 
 
 def retune(plan, tile=None, interp_chunk=None, grid_chunk=None, itile=None, interp_mode=None,
-           grid_split=None):
+           grid_split=None, grid_mode=None):
     """Rebuild the device plan of ``plan`` with other launch parameters."""
+    if grid_mode is not None:
+        plan.grid_mode = int(grid_mode)
     if grid_split is not None:
         plan.grid_split = int(grid_split)
     if tile is not None:
@@ -154,7 +157,7 @@ def _expand_coord(coord):
 
 
 def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, interp_chunk=0,
-               grid_chunk=0, itile=None, interp_mode=0, grid_split=0):
+               grid_chunk=0, itile=None, interp_mode=0, grid_split=0, grid_mode=0):
     """Precompute a NUFFT (deepmr ``plan_nufft``, src/deepmr/fft/nufft.py:27-132).
 
     Parameters follow deepmr: ``coord`` (ncontrasts, nviews, nsamples, ndim)
@@ -162,9 +165,11 @@ def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, in
     (z, y, x). ``device`` selects the HIP device ("cpu" = current HIP device).
     ``tile`` / ``itile`` (gridding / staged-interpolation tiles, (z, y, x)),
     ``interp_mode`` (1 gather, 2 LDS-staged), ``interp_chunk`` and
-    ``grid_chunk`` (coils per work item) and ``grid_split`` (max samples per
-    gridding workgroup before a hot tile is split; < 0 never) optionally
-    override the launch parameters (0 / None = automatic).
+    ``grid_chunk`` (coils per work item), ``grid_split`` (max samples per
+    gridding workgroup before a hot tile is split; < 0 never) and
+    ``grid_mode`` (1 per-sample LDS atomics, 2 register runs over
+    origin-sorted samples) optionally override the launch parameters
+    (0 / None = automatic).
     """
     coord = torch.as_tensor(coord).detach().cpu().to(torch.float32).numpy().copy()
     ndim = coord.shape[-1]
@@ -209,6 +214,7 @@ def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, in
         itile=None if itile is None else tuple(int(t) for t in itile),
         interp_mode=int(interp_mode),
         grid_split=int(grid_split),
+        grid_mode=int(grid_mode),
     )
     plan.interpolator = _build_native(plan, dev, shift=True)
     return plan
@@ -411,16 +417,12 @@ def _apply_nufft(image, nufft_plan, basis_adjoint, weight, device, threadsperblo
     return out
 
 
-def _apply_nufft_adj(kspace, nufft_plan, basis, weight, device, threadsperblock):
-    isnumpy = isinstance(kspace, np.ndarray)
-    kspace = _as_compute(kspace)
-    dev = _check_plan(nufft_plan, device)
-    odevice = kspace.device
-    plan = nufft_plan
+def _grid_step(y, plan, basis, weight, dev):
+    """Weighting and gridding of k-space ``y`` onto the plan's (FFT-ordered)
+    grid: returns (grid (nbatch, nimgax, *os_shape), batch_shape, nimgax).
+    ``nimgax`` is nframes, or ncoeff for a subspace ``basis``; a single-frame
+    plan keeps coefficient 0 (grid.py:117)."""
     ndim, nframes = plan.ndim, plan.nframes
-    y = kspace.to(dev)
-    if y.dtype != torch.complex64:
-        y = y.to(torch.complex64)
     w_fused, w_bcast = _fused_weight(weight, plan, dev, (nframes, *plan.ishape) if nframes > 1 else plan.ishape)
     if w_bcast is not None:
         y = w_bcast * y
@@ -431,28 +433,43 @@ def _apply_nufft_adj(kspace, nufft_plan, basis, weight, device, threadsperblock)
                          f"({nframes} frames x {plan.npts} samples)")
     y = y.reshape(nbatch, nframes, plan.npts).contiguous()
     # low rank: the image gets ncoeff subspace coefficients per batch entry
-    # (grid.py:89-104); a single-frame plan keeps coefficient 0 (grid.py:117)
+    # (grid.py:89-104)
     nimgax = nframes
     bs = None
     if basis is not None:
         ncoeff = int(torch.as_tensor(basis).shape[0])
         bs = _basis_tensor(basis, ncoeff, nframes, "basis", dev)
         nimgax = ncoeff
-    native = plan.interpolator
     with torch.cuda.device(dev):
         grid = torch.empty((nbatch, nimgax, *plan.os_shape), dtype=torch.complex64, device=dev)
         if nbatch > 0 and bs is not None:
             _lowrank_native(plan).gridding_lowrank(y, grid, nbatch, nimgax, plan.npts, bs, w_fused)
         elif nbatch > 0:
-            native.gridding(y, grid, nbatch, w_fused)
+            plan.interpolator.gridding(y, grid, nbatch, w_fused)
         if bs is not None and nframes == 1:
             grid = grid[:, :1].contiguous()
             nimgax = 1
+    return grid, batch_shape, nimgax
+
+
+def _apply_nufft_adj(kspace, nufft_plan, basis, weight, device, threadsperblock):
+    isnumpy = isinstance(kspace, np.ndarray)
+    kspace = _as_compute(kspace)
+    dev = _check_plan(nufft_plan, device)
+    odevice = kspace.device
+    plan = nufft_plan
+    ndim, nframes = plan.ndim, plan.nframes
+    y = kspace.to(dev)
+    if y.dtype != torch.complex64:
+        y = y.to(torch.complex64)
+    grid, batch_shape, nimgax = _grid_step(y, plan, basis, weight, dev)
+    nbatch = int(np.prod(batch_shape))
+    with torch.cuda.device(dev):
         img_g = torch.fft.ifftn(grid, dim=tuple(range(-ndim, 0)))
         del grid
         nimg = nbatch * nimgax
         img = torch.empty((nimg, *plan.shape), dtype=torch.complex64, device=dev)
-        native.crop_apod(img_g, img, nimg)
+        plan.interpolator.crop_apod(img_g, img, nimg)
         shape = (*batch_shape, nimgax, *plan.shape) if nframes > 1 else (*batch_shape, *plan.shape)
         img = img.reshape(shape)
     img = img.to(odevice)

@@ -110,6 +110,20 @@ def plan_toeplitz_nufft(coord, shape, basis=None, dcf=None, width=4, device="cpu
                       lowrank_product)
 
 
+def _kspace_product(k, kern, ndim):
+    """Per-voxel kernel product in k-space. Low rank: k-space (..., ncoeff,
+    *G) as (batch, ncoeff, voxels) against the (voxels, ncoeff, ncoeff)
+    kernel (_interp/toeplitz.py:51-84)."""
+    if not kern.islowrank:
+        return kern.value * k
+    shp = k.shape
+    kb = k.reshape(-1, shp[-ndim - 1], int(np.prod(shp[-ndim:])))
+    if kern.lowrank_product == "deepmr":
+        return (kb * kern.colsum().T.unsqueeze(0)).reshape(shp)
+    val = kern.value.permute(1, 2, 0)  # (i, j, voxels)
+    return (val.unsqueeze(0) * kb.unsqueeze(1)).sum(dim=2).reshape(shp)
+
+
 def _selfadj(image, kern):
     isnumpy = isinstance(image, np.ndarray)
     x = torch.as_tensor(image)
@@ -122,18 +136,7 @@ def _selfadj(image, kern):
     x = resize(x, list(x.shape[:-ndim]) + list(kern.shape))
     ax = tuple(range(-ndim, 0))
     k = torch.fft.fftn(x, dim=ax, norm="ortho")
-    if kern.islowrank:
-        # (..., ncoeff, *G) as (batch, ncoeff, voxels)  (_interp/toeplitz.py:51-84)
-        shp = k.shape
-        kb = k.reshape(-1, shp[-ndim - 1], int(np.prod(shp[-ndim:])))
-        if kern.lowrank_product == "deepmr":
-            k = (kb * kern.colsum().T.unsqueeze(0)).reshape(shp)
-        else:
-            val = kern.value.permute(1, 2, 0)  # (i, j, voxels)
-            k = (val.unsqueeze(0) * kb.unsqueeze(1)).sum(dim=2).reshape(shp)
-    else:
-        k = kern.value * k
-    y = torch.fft.ifftn(k, dim=ax, norm="ortho")
+    y = torch.fft.ifftn(_kspace_product(k, kern, ndim), dim=ax, norm="ortho")
     y = resize(y, list(y.shape[:-ndim]) + list(oshape)).to(odevice)
     return y.numpy(force=True) if isnumpy else y
 

@@ -39,19 +39,20 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_plan_params_layout_matches_c(tmp_path):
+    """Every field of the ctypes mirror sits at the C offset (and sizeof matches)."""
+    P = _lib.PlanParams
+    names = [f[0] for f in P._fields_]
+    fmt = " ".join(["%zu"] * (len(names) + 1))
+    args = ", ".join(["sizeof(tn_plan_params)"] + [f"offsetof(tn_plan_params, {n})" for n in names])
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "tilenufft.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(tn_plan_params),"
-        " offsetof(tn_plan_params, npts), offsetof(tn_plan_params, beta),"
-        " offsetof(tn_plan_params, inv_scale), offsetof(tn_plan_params, apod));return 0;}\n"
+        f'int main(void){{printf("{fmt}\\n", {args});return 0;}}\n'
     )
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    P = _lib.PlanParams
-    assert [int(v) for v in out] == [ctypes.sizeof(P), P.npts.offset, P.beta.offset,
-                                     P.inv_scale.offset, P.apod.offset]
+    assert [int(v) for v in out] == [ctypes.sizeof(P)] + [getattr(P, n).offset for n in names]
 
 
 @pytest.mark.parametrize("w, osf", [(4, 2.0), (4, 1.5), (4, 1.25), (6, 2.0), (3, 1.5)])

@@ -255,3 +255,49 @@ def test_conjugate_view_inputs():
     a1 = tilenufft.fft.apply_nufft_adj(k.conj(), plan)
     a2 = tilenufft.fft.apply_nufft_adj(k.conj().resolve_conj(), plan)
     assert torch.equal(a1, a2)
+
+
+GRID_CASES = [
+    # (ndim, N, width, osf, coils, split)
+    (2, 40, 4, 2.0, 1, 0),    # one coil: waves own row bands
+    (2, 40, 4, 2.0, 3, 3),    # tiny split: every tile in parts
+    (2, 40, 4, 2.0, 8, 64),
+    (2, 36, 6, 1.5, 2, 0),    # 36 taps
+    (2, 32, 8, 2.0, 5, 128),  # 64 taps
+    (3, 14, 4, 1.5, 4, 0),
+    (3, 14, 3, 2.0, 2, 16),
+]
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("ndim, n, width, osf, ncoil, split", GRID_CASES)
+def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf, ncoil, split):
+    """Both gridding variants (per-sample LDS atomics over origin-spread
+    entries; register runs over origin-sorted entries) on dense-centre
+    trajectories, split tiles, coil counts below and above the wave count."""
+    print(f"grid mode={mode} ndim={ndim} N={n} W={width} osf={osf} coils={ncoil} split={split}", flush=True)
+    if ndim == 2:
+        coord = testing.vd_spiral(10, 256, n)  # many samples share the centre origins
+    else:
+        coord = testing.kooshball(60, 32, n)
+    k = cnormal((ncoil,) + coord.shape[:-1], 50 + width)
+    w = np.random.default_rng(51).uniform(0.5, 1.5, coord.shape[:-1]).astype(np.float32)
+    plan = tilenufft.fft.plan_nufft(coord, n, width, osf, device=DEV, grid_mode=mode, grid_split=split)
+    kt = torch.from_numpy(k).to(DEV)
+    wt = torch.from_numpy(w)
+    a = tilenufft.fft.apply_nufft_adj(kt, plan, weight=wt)
+    b = tilenufft.fft.apply_nufft_adj(kt, plan, weight=wt)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+    ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, width, osf), weight=w)
+    assert rel(a.cpu().numpy(), ref) < RTOL
+
+
+def test_grid_runs_refuses_wide_3d_kernels():
+    coord = testing.kooshball(10, 16, 12)
+    with pytest.raises(RuntimeError, match="taps"):
+        tilenufft.fft.plan_nufft(coord, 12, 5, 2.0, device=DEV, grid_mode=2)
+    plan = tilenufft.fft.plan_nufft(coord, 12, 5, 2.0, device=DEV)  # automatic: per-sample atomics
+    k = torch.from_numpy(cnormal((1,) + coord.shape[:-1], 52)).to(DEV)
+    ref = orc.apply_nufft_adj(k.cpu().numpy(), orc.plan_nufft(coord, 12, 5, 2.0))
+    assert rel(tilenufft.fft.apply_nufft_adj(k, plan).cpu().numpy(), ref) < RTOL
