@@ -28,11 +28,19 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 // across footprint origins, or per-lane register runs over origin-sorted
 // entries (for at most kRunsMaxTaps taps per sample).
 enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2 };
-constexpr int kRunsMaxTaps = 64;
+constexpr int kRunsMaxTaps = 36;  // register budget of the run sums (2D W <= 6, 3D W <= 3)
+constexpr int kRunsEpl = 4;                     // entries per lane per staged batch
+constexpr int kRunsBatch = kRunsEpl * kWave;    // entries per staged batch (= kBlock)
+
+// 32-bit words per staged entry of the register-run gridding: origin key,
+// (tile-local tap-0 offset, wrap tap) per axis, W weights per axis.
+__host__ __device__ constexpr int runs_record_words(int ndim, int wc) { return 1 + 2 * ndim + ndim * wc; }
 
 // Dynamic LDS bytes of the gridding kernel for `ncoil` coils of a tile.
 inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc, int mode = kGridSpread) {
-    if (mode == kGridRuns) return sizeof(float) * 2 * (size_t)ncoil * plane;
+    if (mode == kGridRuns)  // interleaved accumulators, entry records, sample values
+        return sizeof(float) * (2 * (size_t)ncoil * plane + 4 + (size_t)runs_record_words(3, wc) * kRunsBatch +
+                                2 * (size_t)ncoil * kRunsBatch);
     return sizeof(float) * (2 * (size_t)ncoil * plane + 4 * (size_t)kGridBatch +
                             3 * (size_t)wc * kGridBatch + 2 * (size_t)ncoil * kGridBatch);
 }

@@ -286,10 +286,18 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
 // gridding item: straight into the grid (every in-grid cell, zeros
 // included), or, for a split tile, as partial `part` of fold slot `slot`
 // ([coil][cell](re, im)), summed later by fold_kernel in part order.
+template <bool INTERLEAVED = false>
 __device__ __forceinline__ void store_tile(const Job &J, const Item &it, const float *acc, int plane,
                                            int P, int ox, int oy, int oz, int ex, int ey, int ez,
                                            float2 *__restrict__ grid, float *__restrict__ scratch,
                                            const int64_t *__restrict__ slot_base) {
+    // planar: acc[coil][re, im][cell], `plane` floats per part; interleaved:
+    // acc[coil][cell](re, im), `plane` cells per coil
+    auto at = [&](int b, int cell_off) -> float2 {
+        if (INTERLEAVED) return ((const float2 *)acc)[b * plane + cell_off];
+        const float *a = acc + 2 * b * plane + cell_off;
+        return make_float2(a[0], a[plane]);
+    };
     const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
     const int Gx = J.G[0], Gy = J.G[1];
     const int ncoil = it.c1 - it.c0;
@@ -302,9 +310,9 @@ __device__ __forceinline__ void store_tile(const Job &J, const Item &it, const f
             const int cell = i - b * cells;
             const int lx = cell % Tx;
             const int row = cell / Tx;
-            const float *a = acc + 2 * b * plane + row * P + lx;
-            dst[2 * i] = a[0];
-            dst[2 * i + 1] = a[plane];
+            const float2 v = at(b, row * P + lx);
+            dst[2 * i] = v.x;
+            dst[2 * i + 1] = v.y;
         }
         return;
     }
@@ -315,9 +323,8 @@ __device__ __forceinline__ void store_tile(const Job &J, const Item &it, const f
         const int ly = (cell / Tx) % Ty;
         const int lz = cell / (Tx * Ty);
         if (lx < ex && ly < ey && lz < ez) {
-            const float *a = acc + 2 * b * plane + (lz * Ty + ly) * P + lx;
             grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
-                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(a[0], a[plane]);
+                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = at(b, (lz * Ty + ly) * P + lx);
         }
     }
 }
@@ -505,16 +512,33 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
 }
 
 // Gridding by register runs. The plan sorts each tile's entries by
-// footprint origin. Every wave walks all entries of the item: lane l owns
-// the contiguous run [e0 + l*R, e0 + (l+1)*R), evaluates the taps of each
-// entry and keeps, for the current origin, the W^d tap sums of every coil
-// it owns in registers; when the origin changes (and at the end of the run)
-// the sums are added into the LDS tile with one atomic per tap. Where many
-// samples share a footprint origin (dense trajectory centres) this replaces
-// one heavily conflicting LDS atomic per sample and tap with one per run.
-// Waves own coils (or row bands when there are fewer coils than waves), so
-// each accumulator cell is updated by one wave in program order: the result
-// is deterministic.
+// footprint origin, so the W^d tap sums of one origin -- the complete
+// contribution of all samples sharing that origin -- can be formed in
+// registers and added into the LDS tile once. The item's entries are taken
+// in batches of kRunsBatch:
+//   A. every thread stages one entry: it loads the record (coalesced),
+//      evaluates the taps once and stores, in SoA LDS rows, the origin key,
+//      per axis the tile-local offset of tap 0 and the tap at which the
+//      footprint wraps around the grid, and the weights; it also gathers the
+//      scaled sample value of every coil of the item (independent loads, so
+//      their latencies overlap);
+//   B. lane l of a wave walks batch entries l*R .. l*R+R-1 (R = kRunsEpl;
+//      the slots are laid out so that a wave's lanes read consecutive LDS
+//      words) and sums the taps per origin. An origin that starts and ends
+//      inside the lane's run is unique to the lane and is added at once. The
+//      lane's first and last origins may continue in the neighbouring lanes:
+//      the first-origin sums move to the previous lane when it ends on the
+//      same origin, and a segmented scan over the lanes (keyed by origin,
+//      which is sorted) gathers each last-origin sum in the last lane of its
+//      segment, which adds it. Both steps are skipped (wave ballot) when no
+//      origin spans two lanes.
+// Within one add no two lanes address the same cell (lanes with nothing to
+// add use per-lane trash cells), so the tile is updated with plain LDS
+// read-add-write instead of atomics, and dense trajectory centres --
+// thousands of samples on a few origins -- cost one add per run. Waves own
+// coils (or row bands when there are fewer coils than waves), so each
+// accumulator cell is updated by one wave in program order: the result is
+// deterministic.
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict__ jobs,
                                                            const Item *__restrict__ items,
@@ -527,7 +551,10 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
     constexpr int KZ = NDIM > 2 ? WC : 1;
     constexpr int KZY = KZ * WC;
     constexpr int TAPS = KZY * WC;
-    constexpr int CW = (NDIM == 2 && WC <= 4) ? 2 : 1;  // coils per pass (registers)
+    constexpr int RB = kRunsEpl;
+    constexpr int B = kRunsBatch;
+    constexpr int NREC = runs_record_words(NDIM, WC);  // key, (d0, kw) per axis, weights
+    static_assert(B == kWave * RB && B == kBlock, "one staged entry per thread");
     extern __shared__ float smem[];
     const Item it = items[blockIdx.x];
     const Job &J = jobs[it.job];
@@ -536,17 +563,16 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
     const int tx = tile % J.NT[0];
     const int ty = (tile / J.NT[0]) % J.NT[1];
     const int tz = tile / (J.NT[0] * J.NT[1]);
-    const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
-    const int ex = min(Tx, J.G[0] - ox), ey = min(Ty, J.G[1] - oy), ez = min(Tz, J.G[2] - oz);
+    const int o3[3] = {tx * Tx, ty * Ty, tz * Tz};
+    const int e3[3] = {min(Tx, J.G[0] - o3[0]), min(Ty, J.G[1] - o3[1]), min(Tz, J.G[2] - o3[2])};
     const int P = Tx + kGridPad;
     const int rowsP = Tz * Ty * P;
     const int plane = rowsP + kWave;  // + one trash cell per lane
     const int ncoil = it.c1 - it.c0;
-    const int Gx = J.G[0], Gy = J.G[1], Gz = J.G[2];
-    const int nx = J.W[0];
-    const int ny = NDIM > 1 ? J.W[1] : 1;
-    const int nz = NDIM > 2 ? J.W[2] : 1;
-    float *acc = smem;  // [ncoil][re, im][Tz*Ty*P + trash]
+    const int Gx = J.G[0], Gy = J.G[1];
+    float2 *acc = (float2 *)smem;                                        // [ncoil][plane](re, im)
+    int *rec = (int *)(smem + ((2 * ncoil * plane + 3) & ~3));           // [NREC][B]
+    float2 *sval = (float2 *)(rec + NREC * B);                           // [ncoil][B]
 
     const int tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid % kWave;
@@ -564,116 +590,195 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
         row_lo = min(rows, band * rpb);
         row_hi = min(rows, row_lo + rpb);
     }
-    for (int i = tid; i < 2 * ncoil * plane; i += kBlock) acc[i] = 0.0f;
-    __syncthreads();
-
-    const int n = it.e1 - it.e0;
-    const int R = (n + kWave - 1) / kWave;
-    const int s0 = it.e0 + lane * R;
-    const int s1 = min(it.e1, s0 + R);
+    for (int i = tid; i < ncoil * plane; i += kBlock) acc[i] = make_float2(0.0f, 0.0f);
     const int trash = rowsP + lane;
-    for (int cb = coil0; cb < ncoil; cb += cstep * CW) {
-        float ar[CW][TAPS], ai[CW][TAPS];
+
+    // adds the tap sums (sr, si) of an origin (tile-local tap-0 offsets d0,
+    // wrap taps kw) into coil c's tile; lanes with do_add false add zeros to
+    // their trash cell. The lanes of one call add distinct origins, so for a
+    // given tap no two lanes touch the same cell and a plain read-add-write
+    // is race-free; the compiler barrier keeps each tap's read behind the
+    // previous tap's write (a wave's LDS operations complete in order), as
+    // another lane's next tap may land on a cell this tap just wrote.
+    auto add_origin = [&](bool do_add, const int (&d0)[NDIM], const int (&kw)[NDIM], int c,
+                          const float (&sr)[TAPS], const float (&si)[TAPS]) {
+        int lx[WC], lrow[KZY];
 #pragma unroll
-        for (int q = 0; q < CW; ++q)
+        for (int k = 0; k < WC; ++k) {
+            const int a = d0[0] + k - (k >= kw[0] ? Gx : 0);
+            lx[k] = (do_add && k < J.W[0] && a >= 0 && a < e3[0]) ? a : -1;
+        }
 #pragma unroll
-            for (int t = 0; t < TAPS; ++t) ar[q][t] = ai[q][t] = 0.0f;
-        int cx = -1, cy = 0, cz = 0;  // current footprint origin (-1: none)
-        for (int e = s0; e <= s1; ++e) {
-            int bx = -1, by = 0, bz = 0;
-            float w[3][WC];
-            float4 en = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (e < s1) {
-                en = J.entries[e];
-                const float c3[3] = {en.x, en.y, en.z};
-                int base[3] = {0, 0, 0};
+        for (int kz = 0; kz < KZ; ++kz) {
+            const int az = NDIM > 2 ? d0[NDIM > 2 ? 2 : 0] + kz - (kz >= kw[NDIM > 2 ? 2 : 0] ? J.G[2] : 0) : 0;
+            const bool okz = NDIM > 2 ? (kz < J.W[2] && az >= 0 && az < e3[2]) : true;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if (d < NDIM) {
-                        axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base[d], w[d]);
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < WC; ++k) w[d][k] = k == 0 ? 1.0f : 0.0f;
-                    }
-                }
-                bx = base[0];
-                by = base[1];
-                bz = base[2];
+            for (int ky = 0; ky < WC; ++ky) {
+                const int ay = d0[1] + ky - (ky >= kw[1] ? Gy : 0);
+                const int row = az * Ty + ay;
+                const bool ok = okz && ky < J.W[1] && ay >= 0 && ay < e3[1] && row >= row_lo && row < row_hi;
+                lrow[kz * WC + ky] = ok ? row * P : -1;
             }
-            if (cx >= 0 && (bx != cx || by != cy || bz != cz)) {
-                // flush the sums of origin (cx, cy, cz)
-                int lx[WC], lrow[KZY];
+        }
+        float2 *are = acc + c * plane;
 #pragma unroll
-                for (int k = 0; k < WC; ++k) {
-                    const int ax = wrap_add(cx, k, Gx) - ox;
-                    lx[k] = (k < nx && ax >= 0 && ax < ex) ? ax : -1;
+        for (int zy = 0; zy < KZY; ++zy) {
+#pragma unroll
+            for (int kx = 0; kx < WC; ++kx) {
+                const bool ok = lrow[zy] >= 0 && lx[kx] >= 0;
+                const int a = ok ? lrow[zy] + lx[kx] : trash;
+                float2 v = are[a];
+                v.x += ok ? sr[zy * WC + kx] : 0.0f;
+                v.y += ok ? si[zy * WC + kx] : 0.0f;
+                are[a] = v;
+                __asm__ __volatile__("" ::: "memory");
+            }
+        }
+    };
+
+    for (int b0 = it.e0; b0 < it.e1; b0 += B) {
+        __syncthreads();  // previous batch consumed (and accumulators zeroed)
+        {   // A. stage batch entry tid in slot (tid % RB) * kWave + tid / RB
+            const int e = b0 + tid;
+            const int slot = (tid % RB) * kWave + tid / RB;
+            if (e < it.e1) {
+                const float4 en = J.entries[e];
+                const float c3[3] = {en.x, en.y, en.z};
+                int key = 0;
+#pragma unroll
+                for (int d = NDIM - 1; d >= 0; --d) {
+                    int base;
+                    float w[WC];
+                    axis_taps<WC>(c3[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base, w);
+                    key = key * J.G[d] + base;
+                    rec[(1 + 2 * d) * B + slot] = base - o3[d];
+                    rec[(2 + 2 * d) * B + slot] = J.G[d] - base;
+#pragma unroll
+                    for (int k = 0; k < WC; ++k) rec[(1 + 2 * NDIM + d * WC + k) * B + slot] = __float_as_int(w[k]);
                 }
+                rec[slot] = key;
+                const int p = __float_as_int(en.w);
+                float sc = J.inv_scale;
+                if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
+                for (int c = 0; c < ncoil; ++c) sval[c * B + slot] = grid_sample(J, samp, it.c0 + c, p, sc, lr);
+            }
+        }
+        __syncthreads();
+        const int nvalid = min(RB, max(0, it.e1 - (b0 + lane * RB)));  // entries in this lane's run
+        for (int c = coil0; c < ncoil; c += cstep) {
+            // B. walk the run: H = sums of the first origin, C = current origin
+            float hr[TAPS], hi[TAPS], cr[TAPS], ci[TAPS];
 #pragma unroll
-                for (int kz = 0; kz < KZ; ++kz) {
-                    const int az = wrap_add(cz, kz, Gz) - oz;
+            for (int t = 0; t < TAPS; ++t) hr[t] = hi[t] = cr[t] = ci[t] = 0.0f;
+            int hd0[NDIM], hkw[NDIM], qd0[NDIM], qkw[NDIM];
 #pragma unroll
-                    for (int ky = 0; ky < WC; ++ky) {
-                        const int ay = wrap_add(cy, ky, Gy) - oy;
-                        const int row = az * Ty + ay;
-                        const bool ok = kz < nz && az >= 0 && az < ez && ky < ny && ay >= 0 && ay < ey &&
-                                        row >= row_lo && row < row_hi;
-                        lrow[kz * WC + ky] = ok ? row * P : -1;
+            for (int d = 0; d < NDIM; ++d) hd0[d] = hkw[d] = qd0[d] = qkw[d] = 0;
+            int ka = -1, kq = -1;     // first / current origin key (-1: none yet)
+            bool first_open = true;   // the current origin is still the first one
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+                if (i < nvalid) {
+                    const int s = i * kWave + lane;
+                    const int key = rec[s];
+                    if (kq < 0) {
+                        ka = kq = key;
+#pragma unroll
+                        for (int d = 0; d < NDIM; ++d) {
+                            hd0[d] = qd0[d] = rec[(1 + 2 * d) * B + s];
+                            hkw[d] = qkw[d] = rec[(2 + 2 * d) * B + s];
+                        }
+                    } else if (key != kq) {
+                        if (first_open) {  // the first origin ends inside the run: keep it in H
+#pragma unroll
+                            for (int t = 0; t < TAPS; ++t) {
+                                hr[t] = cr[t];
+                                hi[t] = ci[t];
+                            }
+                            first_open = false;
+                        } else {  // an interior origin: unique to this lane
+                            add_origin(true, qd0, qkw, c, cr, ci);
+                        }
+#pragma unroll
+                        for (int t = 0; t < TAPS; ++t) cr[t] = ci[t] = 0.0f;
+                        kq = key;
+#pragma unroll
+                        for (int d = 0; d < NDIM; ++d) {
+                            qd0[d] = rec[(1 + 2 * d) * B + s];
+                            qkw[d] = rec[(2 + 2 * d) * B + s];
+                        }
                     }
-                }
+                    const float2 sv = sval[c * B + s];
+                    float wx[WC], wy[WC], wz[KZ];
 #pragma unroll
-                for (int q = 0; q < CW; ++q) {
-                    const int c = cb + q * cstep;
-                    if (c < ncoil) {
-                        float *are = acc + 2 * c * plane;
+                    for (int k = 0; k < WC; ++k) {
+                        wx[k] = __int_as_float(rec[(1 + 2 * NDIM + k) * B + s]);
+                        wy[k] = __int_as_float(rec[(1 + 2 * NDIM + WC + k) * B + s]);
+                    }
 #pragma unroll
-                        for (int zy = 0; zy < KZY; ++zy) {
+                    for (int k = 0; k < KZ; ++k)
+                        wz[k] = NDIM > 2 ? __int_as_float(rec[(1 + 2 * NDIM + 2 * WC + k) * B + s]) : 1.0f;
+#pragma unroll
+                    for (int kz = 0; kz < KZ; ++kz) {
+#pragma unroll
+                        for (int ky = 0; ky < WC; ++ky) {
+                            const float wzy = wz[kz] * wy[ky];
+                            const float fr = wzy * sv.x, fi = wzy * sv.y;
 #pragma unroll
                             for (int kx = 0; kx < WC; ++kx) {
-                                const bool ok = lrow[zy] >= 0 && lx[kx] >= 0;
-                                const int a = ok ? lrow[zy] + lx[kx] : trash;
-                                __hip_atomic_fetch_add(are + a, ar[q][zy * WC + kx], __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __hip_atomic_fetch_add(are + plane + a, ai[q][zy * WC + kx], __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                                ar[q][zy * WC + kx] = 0.0f;
-                                ai[q][zy * WC + kx] = 0.0f;
+                                cr[(kz * WC + ky) * WC + kx] += wx[kx] * fr;
+                                ci[(kz * WC + ky) * WC + kx] += wx[kx] * fi;
                             }
                         }
                     }
                 }
             }
-            if (e >= s1) break;
-            cx = bx;
-            cy = by;
-            cz = bz;
-            const int p = __float_as_int(en.w);
-            float sc = J.inv_scale;
-            if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-            float wzy[KZY];
+            // kb: key of the last origin (-1: empty run). H holds the first
+            // origin's sums only if the run has more than one origin.
+            const int kb = kq;
+            const bool has_h = kq >= 0 && !first_open;
+            const int kb_prev = __shfl_up(kb, 1, kWave);
+            const bool h_to_prev = has_h && lane > 0 && kb_prev == ka;
+            // 1. the first-origin sums of lane l+1 join lane l when lane l ends on that origin
+            if (__ballot(h_to_prev) != 0) {
+                const int ka_next = __shfl_down(ka, 1, kWave);
+                const bool hn_next = __shfl_down(has_h ? 1 : 0, 1, kWave) != 0;
+                const bool take_next = lane < kWave - 1 && kb >= 0 && hn_next && ka_next == kb;
 #pragma unroll
-            for (int kz = 0; kz < KZ; ++kz)
+                for (int t = 0; t < TAPS; ++t) {
+                    const float nr = __shfl_down(hr[t], 1, kWave);
+                    const float ni = __shfl_down(hi[t], 1, kWave);
+                    if (take_next) {
+                        cr[t] += nr;
+                        ci[t] += ni;
+                    }
+                }
+            }
+            if (__ballot(has_h && !h_to_prev) != 0) add_origin(has_h && !h_to_prev, hd0, hkw, c, hr, hi);
+            // 2. segmented inclusive scan of the last-origin sums over lanes
+            if (__ballot(kb >= 0 && lane > 0 && kb_prev == kb) != 0) {
 #pragma unroll
-                for (int ky = 0; ky < WC; ++ky) wzy[kz * WC + ky] = w[2][kz] * w[1][ky];
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const int kp = __shfl_up(kb, d, kWave);
+                    const bool join = lane >= d && kp == kb && kb >= 0;
 #pragma unroll
-            for (int q = 0; q < CW; ++q) {
-                const int c = cb + q * cstep;
-                if (c < ncoil) {
-                    const float2 sv = grid_sample(J, samp, it.c0 + c, p, sc, lr);
-#pragma unroll
-                    for (int zy = 0; zy < KZY; ++zy) {
-                        const float fr = wzy[zy] * sv.x, fi = wzy[zy] * sv.y;
-#pragma unroll
-                        for (int kx = 0; kx < WC; ++kx) {
-                            ar[q][zy * WC + kx] += w[0][kx] * fr;
-                            ai[q][zy * WC + kx] += w[0][kx] * fi;
+                    for (int t = 0; t < TAPS; ++t) {
+                        const float pr = __shfl_up(cr[t], d, kWave);
+                        const float pi = __shfl_up(ci[t], d, kWave);
+                        if (join) {
+                            cr[t] += pr;
+                            ci[t] += pi;
                         }
                     }
                 }
             }
+            const int kb_next = __shfl_down(kb, 1, kWave);
+            const bool seg_end = kb >= 0 && (lane == kWave - 1 || kb_next != kb);
+            add_origin(seg_end, qd0, qkw, c, cr, ci);
         }
     }
     __syncthreads();
-    store_tile(J, it, acc, plane, P, ox, oy, oz, ex, ey, ez, grid, scratch, slot_base);
+    store_tile<true>(J, it, (const float *)acc, plane, P, o3[0], o3[1], o3[2], e3[0], e3[1], e3[2], grid,
+                     scratch, slot_base);
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
@@ -844,20 +949,18 @@ StagedFn pick_staged(int wc) {
 }
 
 template <int NDIM>
-GridFn pick_runs(int wc) {
+GridFn pick_runs(int wc) {  // W^NDIM <= kRunsMaxTaps
     switch (wc) {
         case 1: return grid_runs_kernel<NDIM, 1>;
         case 2: return grid_runs_kernel<NDIM, 2>;
         case 3: return grid_runs_kernel<NDIM, 3>;
-        case 4: return grid_runs_kernel<NDIM, 4>;
         default: break;
     }
     if constexpr (NDIM == 2) {
         switch (wc) {
+            case 4: return grid_runs_kernel<2, 4>;
             case 5: return grid_runs_kernel<2, 5>;
             case 6: return grid_runs_kernel<2, 6>;
-            case 7: return grid_runs_kernel<2, 7>;
-            case 8: return grid_runs_kernel<2, 8>;
             default: break;
         }
     }

@@ -84,7 +84,7 @@ class NUFFTPlan:
 
 def _build_native(plan, device, shift, coord=None):
     ndim = plan.ndim
-    sh = [g - g // 2 for g in plan.os_shape] if shift else [0] * ndim
+    sh = [(g - g // 2) % g for g in plan.os_shape] if shift else [0] * ndim  # unit axes: 0
     apod = [
         _core.apodization(plan.shape[d], plan.oversamp[d], plan.width[d], plan.beta[d])
         if plan.width[d] != 1
