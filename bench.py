@@ -162,7 +162,13 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--tune-cache", default=None, help="directory of the autotuner's cache file")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after measuring, run 3 plain device copies of this many MiB (a known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
+    if args.tune_cache:
+        os.environ["TILENUFFT_CACHE"] = args.tune_cache
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the NUFFT path "
               "shards by frame/coil/request without communication, see DESIGN.md", flush=True)
@@ -235,6 +241,15 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle restatement of deepmr's CPU loops) on frame 0")
         cpu, parity = cpu_baseline(wl, batch)
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize(dev)
+        del src, dst
 
     serving = None
     if not args.no_serve:

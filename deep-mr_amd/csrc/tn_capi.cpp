@@ -133,11 +133,13 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     for (const tn_plan *pl : plan_of)
         if (pl->grid_mode != gmode) return fail(TN_E_ARG, "batched plans must share the gridding variant");
     t.grid_mode = gmode;
-    if (grid_lds_bytes(1, max_plane, wc, gmode) > kGridLdsBudget)
+    const int gndim = t.kind_grid / 16;
+    if (grid_lds_bytes(1, max_plane, wc, gmode, gndim) > kGridLdsBudget)
         return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
-                    grid_lds_bytes(1, max_plane, wc, gmode));
+                    grid_lds_bytes(1, max_plane, wc, gmode, gndim));
     int lds_coils = 1;
-    while (lds_coils < 64 && grid_lds_bytes(lds_coils + 1, max_plane, wc, gmode) <= kGridLdsBudget) ++lds_coils;
+    while (lds_coils < 64 && grid_lds_bytes(lds_coils + 1, max_plane, wc, gmode, gndim) <= kGridLdsBudget)
+        ++lds_coils;
     const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
     const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
 
@@ -206,7 +208,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     }
     t.scratch_floats = scratch;
     t.grid_coils = max_gc;
-    t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc, gmode);
+    t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc, gmode, gndim);
     t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
     t.interp_coils = max_ic;
     return upload_tables(jobs, interp, grid, fold, slot_base, t);

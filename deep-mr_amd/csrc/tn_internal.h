@@ -37,9 +37,9 @@ constexpr int kRunsBatch = kRunsEpl * kWave;    // entries per staged batch (= k
 __host__ __device__ constexpr int runs_record_words(int ndim, int wc) { return 1 + 2 * ndim + ndim * wc; }
 
 // Dynamic LDS bytes of the gridding kernel for `ncoil` coils of a tile.
-inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc, int mode = kGridSpread) {
+inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc, int mode = kGridSpread, int ndim = 3) {
     if (mode == kGridRuns)  // interleaved accumulators, entry records, sample values
-        return sizeof(float) * (2 * (size_t)ncoil * plane + 4 + (size_t)runs_record_words(3, wc) * kRunsBatch +
+        return sizeof(float) * (2 * (size_t)ncoil * plane + 4 + (size_t)runs_record_words(ndim, wc) * kRunsBatch +
                                 2 * (size_t)ncoil * kRunsBatch);
     return sizeof(float) * (2 * (size_t)ncoil * plane + 4 * (size_t)kGridBatch +
                             3 * (size_t)wc * kGridBatch + 2 * (size_t)ncoil * kGridBatch);

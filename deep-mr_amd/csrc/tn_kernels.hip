@@ -516,8 +516,8 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
 // contribution of all samples sharing that origin -- can be formed in
 // registers and added into the LDS tile once. The item's entries are taken
 // in batches of kRunsBatch:
-//   A. every thread stages one entry: it loads the record (coalesced),
-//      evaluates the taps once and stores, in SoA LDS rows, the origin key,
+//   A. every thread stages one entry: it loads the record, evaluates the
+//      taps once and stores, in SoA LDS rows, the origin key,
 //      per axis the tile-local offset of tap 0 and the tap at which the
 //      footprint wraps around the grid, and the weights; it also gathers the
 //      scaled sample value of every coil of the item (independent loads, so
@@ -638,9 +638,11 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
 
     for (int b0 = it.e0; b0 < it.e1; b0 += B) {
         __syncthreads();  // previous batch consumed (and accumulators zeroed)
-        {   // A. stage batch entry tid in slot (tid % RB) * kWave + tid / RB
-            const int e = b0 + tid;
-            const int slot = (tid % RB) * kWave + tid / RB;
+        {   // A. thread tid stages the entry that lane (tid % kWave) walks at step
+            //    (tid / kWave), i.e. batch entry (tid % kWave) * RB + tid / kWave,
+            //    in slot tid: LDS writes and the walk's reads are both consecutive
+            const int slot = tid;
+            const int e = b0 + (tid % kWave) * RB + tid / kWave;
             if (e < it.e1) {
                 const float4 en = J.entries[e];
                 const float c3[3] = {en.x, en.y, en.z};
