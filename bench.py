@@ -122,17 +122,22 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
         return serve.make_stream(n, rate_hz, seed=s, device="cpu")
 
     # warm-up (kernel loading, rocFFT plans of the common sizes), untimed
-    serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32).run(stream(warm, seed + 1000),
-                                                                 realtime=False)
+    warm_srv = serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32)
+    warm_srv.run(stream(warm, seed + 1000), realtime=False)
+    warm_srv.close()
     serve.run_plain(stream(warm, seed + 1000), dev, realtime=False)
     torch.cuda.synchronize(dev)
 
     reqs_s = stream(nreq, seed)
     srv = serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32)
     srv.run(reqs_s, realtime=True)
+    srv.close()
     reqs_p = stream(nreq, seed)
     serve.run_plain(reqs_p, dev, realtime=True)
-    st_s, st_p = serve.latency_stats(reqs_s), serve.latency_stats(reqs_p)
+    reqs_r = stream(nreq, seed)
+    serve.run_plain(reqs_r, dev, realtime=True, replan=True)
+    st_s, st_p, st_r = (serve.latency_stats(reqs_s), serve.latency_stats(reqs_p),
+                        serve.latency_stats(reqs_r))
     errs = []
     for a, b in zip(reqs_s, reqs_p):
         if a.result is not None and b.result is not None:
@@ -144,8 +149,15 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
         served=dict(p50_s=st_s["p50"], p99_s=st_s["p99"], recon_per_s=st_s["recon_per_s"],
                     completed=st_s["completed"], server_steps=srv.steps, stats=srv.stats),
         plain=dict(p50_s=st_p["p50"], p99_s=st_p["p99"], recon_per_s=st_p["recon_per_s"],
-                   completed=st_p["completed"]),
+                   completed=st_p["completed"],
+                   path="one request at a time, apply_nufft / apply_nufft_adj with the request's plan "
+                        "made once and reused over its CG iterations"),
+        plain_per_call=dict(p50_s=st_r["p50"], p99_s=st_r["p99"], recon_per_s=st_r["recon_per_s"],
+                            completed=st_r["completed"],
+                            path="one request at a time through the functional nufft / nufft_adj, "
+                                 "which plan on every call as deepmr's do"),
         speedup_recon_per_s=(st_s["recon_per_s"] / st_p["recon_per_s"]) if st_p["recon_per_s"] else None,
+        speedup_vs_per_call=(st_s["recon_per_s"] / st_r["recon_per_s"]) if st_r["recon_per_s"] else None,
         max_rel_l2_served_vs_plain=max(errs) if errs else None,
     )
 

@@ -15,6 +15,10 @@ budget covers their resident state (plan / sorted trajectory, coil maps,
 k-space, CG vectors and their share of the step arenas); finished requests
 leave and free their budget, so new arrivals join while others are mid-solve.
 Plans are kept resident and reused for repeated trajectories (``traj_key``).
+A request is planned as soon as it is submitted, by a worker thread on its
+own HIP stream, so planning (host arithmetic, binning and sorting on the
+device) overlaps the steps of the requests already running; its k-space and
+coil maps are copied to the device without blocking at admission.
 
 The CG recursion is deepmr's CGStep (src/deepmr/optim/cg.py:194-212) on
 E^H E x = E^H y with E = NUFFT * SENSE (src/deepmr/recon/alg/linop.py:108-148),
@@ -30,7 +34,9 @@ __all__ = ["ReconRequest", "ReconServer", "run_plain", "estimate_request_bytes",
            "make_stream"]
 
 import collections
+import concurrent.futures
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -94,8 +100,9 @@ class _Active:
     def __init__(self, req, plan, device):
         self.req = req
         self.plan = plan
-        self.smaps = req.smaps.to(device, torch.complex64).contiguous()
-        self.y = req.kspace.to(device, torch.complex64).reshape(req.ncoils, -1).contiguous()
+        # pinned host tensors copy without blocking the host
+        self.smaps = req.smaps.to(device, non_blocking=True).to(torch.complex64).contiguous()
+        self.y = req.kspace.to(device, non_blocking=True).to(torch.complex64).reshape(req.ncoils, -1).contiguous()
         n = req.shape
         self.x = torch.zeros((n, n), dtype=torch.complex64, device=device)
         self.r = None
@@ -121,6 +128,10 @@ class ReconServer:
         self.reserved = 0
         self._plans = collections.OrderedDict()  # traj_key -> NUFFTPlan (resident)
         self._plan_cache = plan_cache
+        self._plan_lock = threading.Lock()
+        self._planner = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        self._plan_stream = torch.cuda.Stream(device=self.device)  # default priority
+        self._pending = {}  # id(req) -> future of (plan, ready event)
         self._layout_key = None
         self._layout = None
         self.steps = 0
@@ -133,19 +144,35 @@ class ReconServer:
 
     def submit(self, req):
         self.queue.append(req)
+        self._pending[id(req)] = self._planner.submit(self._prepare, req)
+
+    def _prepare(self, req):
+        """Validate and plan a request on the planning stream (worker thread)."""
+        with torch.cuda.device(self.device), torch.cuda.stream(self._plan_stream):
+            self._validate(req)
+            plan = self._plan_for(req)
+            ev = torch.cuda.Event()
+            ev.record(self._plan_stream)
+        return plan, ev
+
+    def close(self):
+        """Stop the planning worker (pending plans are finished first)."""
+        self._planner.shutdown(wait=True)
 
     def _plan_for(self, req):
         key = req.traj_key
-        if key is not None and key in self._plans:
-            self._plans.move_to_end(key)
-            return self._plans[key]
+        with self._plan_lock:
+            if key is not None and key in self._plans:
+                self._plans.move_to_end(key)
+                return self._plans[key]
         plan = _fft.plan_nufft(req.coord, req.shape, self.width, self.oversamp, device=self.device)
         if plan.nframes != 1 or plan.ndim != 2:
             raise ValueError("server requests are single-frame 2D")
         if key is not None:
-            self._plans[key] = plan
-            while len(self._plans) > self._plan_cache:
-                self._plans.popitem(last=False)
+            with self._plan_lock:
+                self._plans[key] = plan
+                while len(self._plans) > self._plan_cache:
+                    self._plans.popitem(last=False)
         return plan
 
     def _validate(self, req):
@@ -166,6 +193,7 @@ class ReconServer:
             req = self.queue[0]
             if req.cancelled:
                 self.queue.popleft()
+                self._pending.pop(id(req), None)
                 req.error = "cancelled"
                 req.t_done = self.now()
                 self.done.append(req)
@@ -173,6 +201,7 @@ class ReconServer:
             need = estimate_request_bytes(req.shape, req.ncoils, req.npts, self.oversamp)
             if need > self.budget:
                 self.queue.popleft()
+                self._pending.pop(id(req), None)
                 req.error = f"request needs {need} bytes, more than the HBM budget {self.budget}"
                 self.stats["rejected"] += 1
                 req.t_done = self.now()
@@ -183,8 +212,11 @@ class ReconServer:
                 break  # wait for memory to free up (FIFO)
             self.queue.popleft()
             try:
-                self._validate(req)
-                plan = self._plan_for(req)
+                fut = self._pending.pop(id(req), None)
+                if fut is None:  # not submitted through submit()
+                    fut = self._planner.submit(self._prepare, req)
+                plan, ready = fut.result()
+                torch.cuda.current_stream(self.device).wait_event(ready)
                 st = _Active(req, plan, self.device)
             except Exception as exc:  # a malformed request fails alone
                 self.stats["rejected"] += 1
@@ -536,9 +568,12 @@ def make_stream(nreq, rate_hz, seed=0, sizes=(128, 192, 256, 320), coils=(4, 32)
             ns = 1024
             nv = max(1, m // ns)
             coord = testing.vd_spiral(nv, ns, n, rotation=float(trng.uniform(0, 2 * np.pi)))
+        pin = torch.device(device).type == "cpu" and torch.cuda.is_available()
         if (c, n) not in maps_cache:
-            maps_cache[(c, n)] = torch.from_numpy(testing.gaussian_coil_maps(c, (n, n), seed=c))
+            m = torch.from_numpy(testing.gaussian_coil_maps(c, (n, n), seed=c))
+            maps_cache[(c, n)] = m.pin_memory() if pin else m.to(device)
         k = testing.complex_normal((c, *coord.shape[:-1]), generator=gen)
-        out.append(ReconRequest(rid=i, coord=coord, kspace=k.to(device), smaps=maps_cache[(c, n)].to(device),
+        k = k.pin_memory() if pin else k.to(device)  # pinned host data copies asynchronously
+        out.append(ReconRequest(rid=i, coord=coord, kspace=k, smaps=maps_cache[(c, n)],
                                 shape=n, niter=niter, arrival=t, traj_key=key))
     return out
