@@ -368,12 +368,11 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
     }
     choose_tiles(pl);
     {
-        int taps = 1;
-        for (int d = 0; d < 3; ++d) taps *= params->width[d];
+        const int taps = params->width[1] * params->width[2];  // one z plane of the footprint
         const bool runs_ok = taps <= kRunsMaxTaps;
         if (params->grid_mode == kGridRuns && !runs_ok) {
             delete pl;
-            return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per sample (got %d)", kRunsMaxTaps, taps);
+            return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per z plane (got %d)", kRunsMaxTaps, taps);
         }
         pl->grid_mode = params->grid_mode == kGridSpread ? kGridSpread : runs_ok ? kGridRuns : kGridSpread;
     }

@@ -16,7 +16,7 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWave = 64;
 constexpr int kMaxWidth = 16;      // largest kernel width supported per axis
 constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
-constexpr size_t kGridLdsBudget = 64 * 1024;  // LDS per gridding workgroup
+constexpr size_t kGridLdsBudget = 80 * 1024;  // LDS per gridding workgroup (two per CU)
 constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation workgroup
 constexpr int kStagedEpt = 2;  // samples per lane in the staged interpolation
 constexpr int kInterpSegment = 256 * kStagedEpt;  // samples per staged work item
@@ -28,7 +28,7 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 // across footprint origins, or per-lane register runs over origin-sorted
 // entries (for at most kRunsMaxTaps taps per sample).
 enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2 };
-constexpr int kRunsMaxTaps = 36;  // register budget of the run sums (2D W <= 6, 3D W <= 3)
+constexpr int kRunsMaxTaps = 36;  // register budget of the run sums of one z plane (W <= 6)
 constexpr int kRunsEpl = 4;                     // entries per lane per staged batch
 constexpr int kRunsBatch = kRunsEpl * kWave;    // entries per staged batch (= kBlock)
 

@@ -22,6 +22,44 @@
 namespace tn {
 namespace {
 
+// Incremental decomposition of a linear index into (b, z, y, x) over the
+// dims (.., Z, Y, X) for a loop that advances by a fixed stride: the start
+// and the stride are divided once, then each step propagates carries. This
+// replaces 3-4 integer divisions by runtime values per element (each some
+// 30 VALU instructions on CDNA) in the tile, region and image loops.
+struct Idx4 {
+    int x, y, z;
+    int64_t b;
+    int sx, sy, sz;
+    int64_t sb;
+    int X, Y, Z;
+    __device__ __forceinline__ Idx4(int64_t start, int64_t stride, int X_, int Y_, int Z_)
+        : X(X_), Y(Y_), Z(Z_) {
+        split(start, x, y, z, b);
+        split(stride, sx, sy, sz, sb);
+    }
+    __device__ __forceinline__ void split(int64_t i, int &x_, int &y_, int &z_, int64_t &b_) const {
+        int64_t t = i / X;
+        x_ = (int)(i - t * X);
+        int64_t u = t / Y;
+        y_ = (int)(t - u * Y);
+        b_ = u / Z;
+        z_ = (int)(u - b_ * Z);
+    }
+    __device__ __forceinline__ void next() {  // sx < X, sy < Y, sz < Z: one carry each
+        x += sx;
+        int c = x >= X;
+        x -= c ? X : 0;
+        y += sy + c;
+        c = y >= Y;
+        y -= c ? Y : 0;
+        z += sz + c;
+        c = z >= Z;
+        z -= c ? Z : 0;
+        b += sb + c;
+    }
+};
+
 template <int NDIM, int WC>
 struct Taps {
     int idx[3][WC];
@@ -234,17 +272,13 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
     for (int cc = it.c0; cc < it.c1; cc += chunk) {
         const int ncoil = min(chunk, it.c1 - cc);
         __syncthreads();  // previous chunk consumed
-        for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
-            const int c = i / cells;
-            const int cell = i - c * cells;
-            const int lx = cell % Rx;
-            const int ly = (cell / Rx) % Ry;
-            const int lz = cell / (Rx * Ry);
-            int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+        Idx4 q(threadIdx.x, kBlock, Rx, Ry, Rz);
+        for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, q.next()) {
+            int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
             while (gx >= Gx) gx -= Gx;
             while (gy >= Gy) gy -= Gy;
             while (gz >= Gz) gz -= Gz;
-            region[i] = grid[J.grid_off + (int64_t)(cc + c) * J.grid_bstride +
+            region[i] = grid[J.grid_off + (int64_t)(cc + q.b) * J.grid_bstride +
                              ((int64_t)gz * Gy + gy) * Gx + gx];
         }
         __syncthreads();
@@ -303,28 +337,21 @@ __device__ __forceinline__ void store_tile(const Job &J, const Item &it, const f
     const int ncoil = it.c1 - it.c0;
     const int cells = Tx * Ty * Tz;
     const int tid = threadIdx.x;
+    Idx4 q(tid, kBlock, Tx, Ty, Tz);
     if (it.slot >= 0) {
         float *dst = scratch + slot_base[it.slot] + (int64_t)it.part * 2 * ncoil * cells;
-        for (int i = tid; i < ncoil * cells; i += kBlock) {
-            const int b = i / cells;
-            const int cell = i - b * cells;
-            const int lx = cell % Tx;
-            const int row = cell / Tx;
-            const float2 v = at(b, row * P + lx);
+        for (int i = tid; i < ncoil * cells; i += kBlock, q.next()) {
+            const float2 v = at((int)q.b, (q.z * Ty + q.y) * P + q.x);
             dst[2 * i] = v.x;
             dst[2 * i + 1] = v.y;
         }
         return;
     }
-    for (int i = tid; i < ncoil * cells; i += kBlock) {
-        const int b = i / cells;
-        const int cell = i - b * cells;
-        const int lx = cell % Tx;
-        const int ly = (cell / Tx) % Ty;
-        const int lz = cell / (Tx * Ty);
-        if (lx < ex && ly < ey && lz < ez) {
-            grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
-                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = at(b, (lz * Ty + ly) * P + lx);
+    for (int i = tid; i < ncoil * cells; i += kBlock, q.next()) {
+        if (q.x < ex && q.y < ey && q.z < ez) {
+            grid[J.grid_off + (int64_t)(it.c0 + q.b) * J.grid_bstride +
+                 ((int64_t)(oz + q.z) * Gy + (oy + q.y)) * Gx + (ox + q.x)] =
+                at((int)q.b, (q.z * Ty + q.y) * P + q.x);
         }
     }
 }
@@ -517,14 +544,15 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
 // registers and added into the LDS tile once. The item's entries are taken
 // in batches of kRunsBatch:
 //   A. every thread stages one entry: it loads the record, evaluates the
-//      taps once and stores, in SoA LDS rows, the origin key,
-//      per axis the tile-local offset of tap 0 and the tap at which the
-//      footprint wraps around the grid, and the weights; it also gathers the
-//      scaled sample value of every coil of the item (independent loads, so
-//      their latencies overlap);
+//      taps once and stores, in SoA LDS rows, the origin key, per axis the
+//      tile-local offset of tap 0 and the tap at which the footprint wraps
+//      around the grid, and the weights; it also gathers the scaled sample
+//      value of every coil of the item (independent loads, so their
+//      latencies overlap);
 //   B. lane l of a wave walks batch entries l*R .. l*R+R-1 (R = kRunsEpl;
 //      the slots are laid out so that a wave's lanes read consecutive LDS
-//      words) and sums the taps per origin. An origin that starts and ends
+//      words) and sums the taps per origin, one z plane of the footprint per
+//      pass (W^2 sums in registers, W passes in 3D). An origin that starts and ends
 //      inside the lane's run is unique to the lane and is added at once. The
 //      lane's first and last origins may continue in the neighbouring lanes:
 //      the first-origin sums move to the previous lane when it ends on the
@@ -548,9 +576,7 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                                                            float *__restrict__ scratch,
                                                            const int64_t *__restrict__ slot_base,
                                                            LowRank lr) {
-    constexpr int KZ = NDIM > 2 ? WC : 1;
-    constexpr int KZY = KZ * WC;
-    constexpr int TAPS = KZY * WC;
+    constexpr int TAPS = WC * WC;  // taps of one z plane: sums kept in registers per pass
     constexpr int RB = kRunsEpl;
     constexpr int B = kRunsBatch;
     constexpr int NREC = runs_record_words(NDIM, WC);  // key, (d0, kw) per axis, weights
@@ -600,36 +626,38 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
     // is race-free; the compiler barrier keeps each tap's read behind the
     // previous tap's write (a wave's LDS operations complete in order), as
     // another lane's next tap may land on a cell this tap just wrote.
-    auto add_origin = [&](bool do_add, const int (&d0)[NDIM], const int (&kw)[NDIM], int c,
+    auto add_origin = [&](bool do_add, const int (&d0)[NDIM], const int (&kw)[NDIM], int c, int kz,
                           const float (&sr)[TAPS], const float (&si)[TAPS]) {
-        int lx[WC], lrow[KZY];
+        int lx[WC], lrow[WC];
 #pragma unroll
         for (int k = 0; k < WC; ++k) {
             const int a = d0[0] + k - (k >= kw[0] ? Gx : 0);
             lx[k] = (do_add && k < J.W[0] && a >= 0 && a < e3[0]) ? a : -1;
         }
+        int az = 0;
+        bool okz = true;
+        if (NDIM > 2) {
+            constexpr int ZD = NDIM > 2 ? 2 : 0;
+            az = d0[ZD] + kz - (kz >= kw[ZD] ? J.G[2] : 0);
+            okz = kz < J.W[2] && az >= 0 && az < e3[2];
+        }
 #pragma unroll
-        for (int kz = 0; kz < KZ; ++kz) {
-            const int az = NDIM > 2 ? d0[NDIM > 2 ? 2 : 0] + kz - (kz >= kw[NDIM > 2 ? 2 : 0] ? J.G[2] : 0) : 0;
-            const bool okz = NDIM > 2 ? (kz < J.W[2] && az >= 0 && az < e3[2]) : true;
-#pragma unroll
-            for (int ky = 0; ky < WC; ++ky) {
-                const int ay = d0[1] + ky - (ky >= kw[1] ? Gy : 0);
-                const int row = az * Ty + ay;
-                const bool ok = okz && ky < J.W[1] && ay >= 0 && ay < e3[1] && row >= row_lo && row < row_hi;
-                lrow[kz * WC + ky] = ok ? row * P : -1;
-            }
+        for (int ky = 0; ky < WC; ++ky) {
+            const int ay = d0[1] + ky - (ky >= kw[1] ? Gy : 0);
+            const int row = az * Ty + ay;
+            const bool ok = okz && ky < J.W[1] && ay >= 0 && ay < e3[1] && row >= row_lo && row < row_hi;
+            lrow[ky] = ok ? row * P : -1;
         }
         float2 *are = acc + c * plane;
 #pragma unroll
-        for (int zy = 0; zy < KZY; ++zy) {
+        for (int ky = 0; ky < WC; ++ky) {
 #pragma unroll
             for (int kx = 0; kx < WC; ++kx) {
-                const bool ok = lrow[zy] >= 0 && lx[kx] >= 0;
-                const int a = ok ? lrow[zy] + lx[kx] : trash;
+                const bool ok = lrow[ky] >= 0 && lx[kx] >= 0;
+                const int a = ok ? lrow[ky] + lx[kx] : trash;
                 float2 v = are[a];
-                v.x += ok ? sr[zy * WC + kx] : 0.0f;
-                v.y += ok ? si[zy * WC + kx] : 0.0f;
+                v.x += ok ? sr[ky * WC + kx] : 0.0f;
+                v.y += ok ? si[ky * WC + kx] : 0.0f;
                 are[a] = v;
                 __asm__ __volatile__("" ::: "memory");
             }
@@ -667,8 +695,11 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
         }
         __syncthreads();
         const int nvalid = min(RB, max(0, it.e1 - (b0 + lane * RB)));  // entries in this lane's run
-        for (int c = coil0; c < ncoil; c += cstep) {
-            // B. walk the run: H = sums of the first origin, C = current origin
+        const int nzp = NDIM > 2 ? J.W[2] : 1;  // z planes of the taps, one pass each
+        for (int c = coil0; c < ncoil; c += cstep)
+        for (int kz = 0; kz < nzp; ++kz) {
+            // B. walk the run (z plane kz of the taps): H = sums of the first
+            //    origin, C = current origin
             float hr[TAPS], hi[TAPS], cr[TAPS], ci[TAPS];
 #pragma unroll
             for (int t = 0; t < TAPS; ++t) hr[t] = hi[t] = cr[t] = ci[t] = 0.0f;
@@ -698,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                             }
                             first_open = false;
                         } else {  // an interior origin: unique to this lane
-                            add_origin(true, qd0, qkw, c, cr, ci);
+                            add_origin(true, qd0, qkw, c, kz, cr, ci);
                         }
 #pragma unroll
                         for (int t = 0; t < TAPS; ++t) cr[t] = ci[t] = 0.0f;
@@ -710,26 +741,21 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                         }
                     }
                     const float2 sv = sval[c * B + s];
-                    float wx[WC], wy[WC], wz[KZ];
+                    float wx[WC], wy[WC];
 #pragma unroll
-                    for (int k = 0; k < WC; ++k) {
-                        wx[k] = __int_as_float(rec[(1 + 2 * NDIM + k) * B + s]);
-                        wy[k] = __int_as_float(rec[(1 + 2 * NDIM + WC + k) * B + s]);
+                    for (int t = 0; t < WC; ++t) {
+                        wx[t] = __int_as_float(rec[(1 + 2 * NDIM + t) * B + s]);
+                        wy[t] = __int_as_float(rec[(1 + 2 * NDIM + WC + t) * B + s]);
                     }
+                    const float wz = NDIM > 2 ? __int_as_float(rec[(1 + 2 * NDIM + 2 * WC + kz) * B + s]) : 1.0f;
 #pragma unroll
-                    for (int k = 0; k < KZ; ++k)
-                        wz[k] = NDIM > 2 ? __int_as_float(rec[(1 + 2 * NDIM + 2 * WC + k) * B + s]) : 1.0f;
+                    for (int ky = 0; ky < WC; ++ky) {
+                        const float wzy = wz * wy[ky];
+                        const float fr = wzy * sv.x, fi = wzy * sv.y;
 #pragma unroll
-                    for (int kz = 0; kz < KZ; ++kz) {
-#pragma unroll
-                        for (int ky = 0; ky < WC; ++ky) {
-                            const float wzy = wz[kz] * wy[ky];
-                            const float fr = wzy * sv.x, fi = wzy * sv.y;
-#pragma unroll
-                            for (int kx = 0; kx < WC; ++kx) {
-                                cr[(kz * WC + ky) * WC + kx] += wx[kx] * fr;
-                                ci[(kz * WC + ky) * WC + kx] += wx[kx] * fi;
-                            }
+                        for (int kx = 0; kx < WC; ++kx) {
+                            cr[ky * WC + kx] += wx[kx] * fr;
+                            ci[ky * WC + kx] += wx[kx] * fi;
                         }
                     }
                 }
@@ -755,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                     }
                 }
             }
-            if (__ballot(has_h && !h_to_prev) != 0) add_origin(has_h && !h_to_prev, hd0, hkw, c, hr, hi);
+            if (__ballot(has_h && !h_to_prev) != 0) add_origin(has_h && !h_to_prev, hd0, hkw, c, kz, hr, hi);
             // 2. segmented inclusive scan of the last-origin sums over lanes
             if (__ballot(kb >= 0 && lane > 0 && kb_prev == kb) != 0) {
 #pragma unroll
@@ -775,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
             }
             const int kb_next = __shfl_down(kb, 1, kWave);
             const bool seg_end = kb >= 0 && (lane == kWave - 1 || kb_next != kb);
-            add_origin(seg_end, qd0, qkw, c, cr, ci);
+            add_origin(seg_end, qd0, qkw, c, kz, cr, ci);
         }
     }
     __syncthreads();
@@ -804,20 +830,16 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jo
     const float *src = scratch + slot_base[it.e0];
     const int64_t pstride = 2 * (int64_t)ncoil * cells;
     const int Gx = J.G[0], Gy = J.G[1];
-    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock) {
-        const int b = i / cells;
-        const int cell = i - b * cells;
-        const int lx = cell % Tx;
-        const int ly = (cell / Tx) % Ty;
-        const int lz = cell / (Tx * Ty);
-        if (lx < ex && ly < ey && lz < ez) {
+    Idx4 c(threadIdx.x, kBlock, Tx, Ty, Tz);
+    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, c.next()) {
+        if (c.x < ex && c.y < ey && c.z < ez) {
             float re = src[2 * i], im = src[2 * i + 1];
             for (int q = 1; q < nparts; ++q) {
                 re += src[q * pstride + 2 * i];
                 im += src[q * pstride + 2 * i + 1];
             }
-            grid[J.grid_off + (int64_t)(it.c0 + b) * J.grid_bstride +
-                 ((int64_t)(oz + lz) * Gy + (oy + ly)) * Gx + (ox + lx)] = make_float2(re, im);
+            grid[J.grid_off + (int64_t)(it.c0 + c.b) * J.grid_bstride +
+                 ((int64_t)(oz + c.z) * Gy + (oy + c.y)) * Gx + (ox + c.x)] = make_float2(re, im);
         }
     }
 }
@@ -854,19 +876,16 @@ __global__ __launch_bounds__(kBlock) void apod_pad_kernel(PrePost pp, const floa
     const int64_t gvol = (int64_t)pp.G[0] * pp.G[1] * pp.G[2];
     const int64_t nvol = (int64_t)pp.N[0] * pp.N[1] * pp.N[2];
     const int64_t total = nimg * gvol;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * kBlock) {
-        const int64_t b = i / gvol;
-        const int64_t r = i - b * gvol;
-        const int x = (int)(r % pp.G[0]);
-        const int y = (int)((r / pp.G[0]) % pp.G[1]);
-        const int z = (int)(r / ((int64_t)pp.G[0] * pp.G[1]));
-        int mx = x + pp.N[0] / 2; if (mx >= pp.G[0]) mx -= pp.G[0];
-        int my = y + pp.N[1] / 2; if (my >= pp.G[1]) my -= pp.G[1];
-        int mz = z + pp.N[2] / 2; if (mz >= pp.G[2]) mz -= pp.G[2];
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    Idx4 q(i0, step, pp.G[0], pp.G[1], pp.G[2]);
+    for (int64_t i = i0; i < total; i += step, q.next()) {
+        int mx = q.x + pp.N[0] / 2; if (mx >= pp.G[0]) mx -= pp.G[0];
+        int my = q.y + pp.N[1] / 2; if (my >= pp.G[1]) my -= pp.G[1];
+        int mz = q.z + pp.N[2] / 2; if (mz >= pp.G[2]) mz -= pp.G[2];
         float2 v = make_float2(0.0f, 0.0f);
         if (mx < pp.N[0] && my < pp.N[1] && mz < pp.N[2]) {
-            v = img[b * nvol + ((int64_t)mz * pp.N[1] + my) * pp.N[0] + mx];
+            v = img[q.b * nvol + ((int64_t)mz * pp.N[1] + my) * pp.N[0] + mx];
             v = apodize(v, pp, mx, my, mz);
         }
         grid[i] = v;
@@ -878,17 +897,15 @@ __global__ __launch_bounds__(kBlock) void crop_apod_kernel(PrePost pp, const flo
     const int64_t gvol = (int64_t)pp.G[0] * pp.G[1] * pp.G[2];
     const int64_t nvol = (int64_t)pp.N[0] * pp.N[1] * pp.N[2];
     const int64_t total = nimg * nvol;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * kBlock) {
-        const int64_t b = i / nvol;
-        const int64_t r = i - b * nvol;
-        const int mx = (int)(r % pp.N[0]);
-        const int my = (int)((r / pp.N[0]) % pp.N[1]);
-        const int mz = (int)(r / ((int64_t)pp.N[0] * pp.N[1]));
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    Idx4 q(i0, step, pp.N[0], pp.N[1], pp.N[2]);
+    for (int64_t i = i0; i < total; i += step, q.next()) {
+        const int mx = q.x, my = q.y, mz = q.z;
         int x = mx - pp.N[0] / 2; if (x < 0) x += pp.G[0];
         int y = my - pp.N[1] / 2; if (y < 0) y += pp.G[1];
         int z = mz - pp.N[2] / 2; if (z < 0) z += pp.G[2];
-        float2 v = grid[b * gvol + ((int64_t)z * pp.G[1] + y) * pp.G[0] + x];
+        float2 v = grid[q.b * gvol + ((int64_t)z * pp.G[1] + y) * pp.G[0] + x];
         img[i] = apodize(v, pp, mx, my, mz);
     }
 }
@@ -951,22 +968,16 @@ StagedFn pick_staged(int wc) {
 }
 
 template <int NDIM>
-GridFn pick_runs(int wc) {  // W^NDIM <= kRunsMaxTaps
+GridFn pick_runs(int wc) {  // W^2 <= kRunsMaxTaps
     switch (wc) {
         case 1: return grid_runs_kernel<NDIM, 1>;
         case 2: return grid_runs_kernel<NDIM, 2>;
         case 3: return grid_runs_kernel<NDIM, 3>;
-        default: break;
+        case 4: return grid_runs_kernel<NDIM, 4>;
+        case 5: return grid_runs_kernel<NDIM, 5>;
+        case 6: return grid_runs_kernel<NDIM, 6>;
+        default: return nullptr;
     }
-    if constexpr (NDIM == 2) {
-        switch (wc) {
-            case 4: return grid_runs_kernel<2, 4>;
-            case 5: return grid_runs_kernel<2, 5>;
-            case 6: return grid_runs_kernel<2, 6>;
-            default: break;
-        }
-    }
-    return nullptr;
 }
 
 template <int NDIM>

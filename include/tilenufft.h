@@ -68,7 +68,8 @@ typedef struct tn_plan_params {
     int32_t grid_mode;    /* gridding variant: 0 automatic, 1 one LDS atomic per
                              tap and sample (entries spread over footprint
                              origins), 2 per-lane runs of origin-sorted
-                             entries summed in registers (W^d <= 36 taps)    */
+                             entries summed in registers, one z plane of the
+                             footprint per pass (W_y * W_x <= 36)            */
     float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
     const float *apod[3]; /* host apodization vectors of length shape[d], or
                              NULL for ones (src/deepmr/fft/nufft.py:423-447)   */

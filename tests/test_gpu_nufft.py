@@ -263,9 +263,10 @@ GRID_CASES = [
     (2, 40, 4, 2.0, 3, 3),    # tiny split: every tile in parts
     (2, 40, 4, 2.0, 8, 64),
     (2, 36, 6, 1.5, 2, 0),    # 36 taps
-    (2, 32, 8, 2.0, 5, 128),  # 64 taps (per-sample atomics only)
+    (2, 32, 8, 2.0, 5, 128),  # 64 taps per plane (per-sample atomics only)
     (2, 30, 5, 2.0, 6, 0),    # 25 taps
-    (3, 14, 4, 1.5, 4, 0),    # 64 taps (per-sample atomics only)
+    (3, 14, 4, 1.5, 4, 0),    # 4 z-plane passes of 16 taps
+    (3, 16, 6, 2.0, 3, 64),   # 6 passes of 36 taps
     (3, 14, 3, 2.0, 2, 16),
     (3, 12, 2, 2.0, 9, 0),
 ]
@@ -277,8 +278,8 @@ def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf
     """Both gridding variants (per-sample LDS atomics over origin-spread
     entries; register runs over origin-sorted entries) on dense-centre
     trajectories, split tiles, coil counts below and above the wave count."""
-    if mode == 2 and width ** ndim > 36:
-        pytest.skip("register-run gridding holds at most 36 taps per sample")
+    if mode == 2 and width * width > 36:
+        pytest.skip("register-run gridding holds at most 36 taps per z plane")
     print(f"grid mode={mode} ndim={ndim} N={n} W={width} osf={osf} coils={ncoil} split={split}", flush=True)
     if ndim == 2:
         coord = testing.vd_spiral(10, 256, n)  # many samples share the centre origins
@@ -298,10 +299,10 @@ def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf
 
 
 def test_grid_runs_refuses_wide_kernels():
-    coord = testing.kooshball(10, 16, 12)
+    coord = testing.golden_angle_radial(12, 64, 24)
     with pytest.raises(RuntimeError, match="taps"):
-        tilenufft.fft.plan_nufft(coord, 12, 4, 2.0, device=DEV, grid_mode=2)
-    plan = tilenufft.fft.plan_nufft(coord, 12, 4, 2.0, device=DEV)  # automatic: per-sample atomics
+        tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV, grid_mode=2)
+    plan = tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV)  # automatic: per-sample atomics
     k = torch.from_numpy(cnormal((1,) + coord.shape[:-1], 52)).to(DEV)
-    ref = orc.apply_nufft_adj(k.cpu().numpy(), orc.plan_nufft(coord, 12, 4, 2.0))
+    ref = orc.apply_nufft_adj(k.cpu().numpy(), orc.plan_nufft(coord, 24, 7, 2.0))
     assert rel(tilenufft.fft.apply_nufft_adj(k, plan).cpu().numpy(), ref) < RTOL
