@@ -138,7 +138,8 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
                     grid_lds_bytes(1, max_plane, wc, gmode, gndim));
     int lds_coils = 1;
-    while (lds_coils < 64 && grid_lds_bytes(lds_coils + 1, max_plane, wc, gmode, gndim) <= kGridLdsBudget)
+    while (lds_coils < (gmode == kGridRuns ? kRunsMaxCoils : 64) &&
+           grid_lds_bytes(lds_coils + 1, max_plane, wc, gmode, gndim) <= kGridLdsBudget)
         ++lds_coils;
     const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
     const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
@@ -156,7 +157,11 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     for (size_t ji = 0; ji < jobs.size(); ++ji) {
         const Job &j = jobs[ji];
         const tn_plan *pl = plan_of[ji];
-        const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, lds_coils), lds_coils);
+        // automatic coil chunk: as many as fit in LDS; for register-run gridding
+        // one coil per wave (more coils per item cost occupancy, which hides
+        // the latency of its LDS read-add-write chains)
+        const int gc_auto = gmode == kGridRuns ? std::min(lds_coils, kBlock / kWave) : lds_coils;
+        const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, gc_auto), lds_coils);
         max_gc = std::max(max_gc, gc);
         if (mode == kInterpStaged) {
             // one item per (tile, segment) covers every coil; the region is

@@ -29,8 +29,8 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 // entries (for at most kRunsMaxTaps taps per sample).
 enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2 };
 constexpr int kRunsMaxTaps = 36;  // register budget of the run sums of one z plane (W <= 6)
-constexpr int kRunsEpl = 4;                     // entries per lane per staged batch
-constexpr int kRunsBatch = kRunsEpl * kWave;    // entries per staged batch (= kBlock)
+constexpr int kRunsBatch = 256;                 // entries per staged batch (= kBlock)
+constexpr int kRunsMaxCoils = 64;               // coils per item: 4 waves x up to 16 lanes per group
 
 // 32-bit words per staged entry of the register-run gridding: origin key,
 // (tile-local tap-0 offset, wrap tap) per axis, W weights per axis.
@@ -40,7 +40,7 @@ __host__ __device__ constexpr int runs_record_words(int ndim, int wc) { return 1
 inline size_t grid_lds_bytes(int ncoil, int64_t plane, int wc, int mode = kGridSpread, int ndim = 3) {
     if (mode == kGridRuns)  // interleaved accumulators, entry records, sample values
         return sizeof(float) * (2 * (size_t)ncoil * plane + 4 + (size_t)runs_record_words(ndim, wc) * kRunsBatch +
-                                2 * (size_t)ncoil * kRunsBatch);
+                                2 * (size_t)(ncoil + 1) * kRunsBatch);
     return sizeof(float) * (2 * (size_t)ncoil * plane + 4 * (size_t)kGridBatch +
                             3 * (size_t)wc * kGridBatch + 2 * (size_t)ncoil * kGridBatch);
 }

@@ -539,33 +539,36 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
 }
 
 // Gridding by register runs. The plan sorts each tile's entries by
-// footprint origin, so the W^d tap sums of one origin -- the complete
-// contribution of all samples sharing that origin -- can be formed in
-// registers and added into the LDS tile once. The item's entries are taken
-// in batches of kRunsBatch:
-//   A. every thread stages one entry: it loads the record, evaluates the
-//      taps once and stores, in SoA LDS rows, the origin key, per axis the
-//      tile-local offset of tap 0 and the tap at which the footprint wraps
-//      around the grid, and the weights; it also gathers the scaled sample
-//      value of every coil of the item (independent loads, so their
+// footprint origin, so the W^2 tap sums of one origin and footprint z plane
+// -- the complete contribution of all samples sharing that origin -- can be
+// formed in registers and added into the LDS tile once.
+//
+// Lanes: a wave owns Cw coils of the item (Cw a power of two) and its 64
+// lanes are 64/Cw groups of Cw lanes, one lane per coil. The item's entries
+// are taken in batches of kRunsBatch:
+//   A. the threads stage the batch: each entry record is loaded, its taps
+//      are evaluated once and stored in SoA LDS rows (origin key, per axis
+//      the tile-local offset of tap 0 and the tap at which the footprint
+//      wraps around the grid, the weights), and the scaled sample values of
+//      every coil of the item are gathered (independent loads, so their
 //      latencies overlap);
-//   B. lane l of a wave walks batch entries l*R .. l*R+R-1 (R = kRunsEpl;
-//      the slots are laid out so that a wave's lanes read consecutive LDS
-//      words) and sums the taps per origin, one z plane of the footprint per
-//      pass (W^2 sums in registers, W passes in 3D). An origin that starts and ends
-//      inside the lane's run is unique to the lane and is added at once. The
-//      lane's first and last origins may continue in the neighbouring lanes:
-//      the first-origin sums move to the previous lane when it ends on the
-//      same origin, and a segmented scan over the lanes (keyed by origin,
-//      which is sorted) gathers each last-origin sum in the last lane of its
-//      segment, which adds it. Both steps are skipped (wave ballot) when no
-//      origin spans two lanes.
-// Within one add no two lanes address the same cell (lanes with nothing to
-// add use per-lane trash cells), so the tile is updated with plain LDS
-// read-add-write instead of atomics, and dense trajectory centres --
-// thousands of samples on a few origins -- cost one add per run. Waves own
-// coils (or row bands when there are fewer coils than waves), so each
-// accumulator cell is updated by one wave in program order: the result is
+//   B. group g walks batch entries g*R .. g*R+R-1 (R = kRunsBatch * Cw / 64;
+//      slots are laid out so that consecutive groups read consecutive LDS
+//      words and a group's lanes read the same word) and each lane sums the
+//      taps of its coil per origin, one z plane of the footprint per pass.
+//      An origin that starts and ends inside the group's run is unique to
+//      the group and is added at once. The run's first and last origins may
+//      continue in the neighbouring groups: the first-origin sums move to the
+//      previous group when it ends on the same origin, and a segmented scan
+//      over the groups (keyed by origin, which is sorted) gathers each
+//      last-origin sum in the last group of its segment, which adds it. Both
+//      steps are skipped (wave ballot) when no origin spans two groups.
+// Within one add no two lanes address the same cell of the same coil (lanes
+// with nothing to add use per-lane trash cells), so the tile is updated with
+// plain LDS read-add-write instead of atomics, and dense trajectory centres
+// -- thousands of samples on a few origins -- cost one add per run. Each
+// accumulator cell is updated by one wave in program order (waves own coils,
+// or row bands when there are fewer coils than waves): the result is
 // deterministic.
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict__ jobs,
@@ -577,10 +580,9 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                                                            const int64_t *__restrict__ slot_base,
                                                            LowRank lr) {
     constexpr int TAPS = WC * WC;  // taps of one z plane: sums kept in registers per pass
-    constexpr int RB = kRunsEpl;
     constexpr int B = kRunsBatch;
     constexpr int NREC = runs_record_words(NDIM, WC);  // key, (d0, kw) per axis, weights
-    static_assert(B == kWave * RB && B == kBlock, "one staged entry per thread");
+    static_assert(B == kBlock, "one staged entry per thread");
     extern __shared__ float smem[];
     const Item it = items[blockIdx.x];
     const Job &J = jobs[it.job];
@@ -595,40 +597,50 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
     const int rowsP = Tz * Ty * P;
     const int plane = rowsP + kWave;  // + one trash cell per lane
     const int ncoil = it.c1 - it.c0;
+    const int SV = ncoil + 1;         // sample-value row pitch (float2), padded against bank conflicts
     const int Gx = J.G[0], Gy = J.G[1];
     float2 *acc = (float2 *)smem;                                        // [ncoil][plane](re, im)
     int *rec = (int *)(smem + ((2 * ncoil * plane + 3) & ~3));           // [NREC][B]
-    float2 *sval = (float2 *)(rec + NREC * B);                           // [ncoil][B]
+    float2 *sval = (float2 *)(rec + NREC * B);                           // [B][SV]
 
     const int tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid % kWave;
     const int rows = Tz * Ty;
-    int coil0, cstep, row_lo = 0, row_hi = rows;
+    // lane -> (group, coil); waves own coils (cw per wave) or row bands
+    int lcw = 0;  // log2(cw)
+    while ((kBlock / kWave) << lcw < ncoil && lcw < 4) ++lcw;
+    const int cw = 1 << lcw;
+    const int ng = kWave >> lcw;             // groups per wave
+    const int grp = lane >> lcw;
+    const int R = B / ng;                    // entries per group per batch
+    int coil, row_lo = 0, row_hi = rows;
     if (ncoil >= kBlock / kWave) {
-        coil0 = wave;
-        cstep = kBlock / kWave;
+        coil = wave * cw + (lane & (cw - 1));  // >= ncoil: idle lane
     } else {
         const int nb = (kBlock / kWave) / ncoil;
         const int band = wave / ncoil;
-        coil0 = band < nb ? wave % ncoil : ncoil;
-        cstep = ncoil;
+        coil = band < nb ? wave % ncoil : ncoil;
         const int rpb = (rows + nb - 1) / nb;
         row_lo = min(rows, band * rpb);
         row_hi = min(rows, row_lo + rpb);
     }
+    const bool active = coil < ncoil;
+    const int cs = active ? coil : 0;  // a valid coil index for idle lanes' reads
     for (int i = tid; i < ncoil * plane; i += kBlock) acc[i] = make_float2(0.0f, 0.0f);
     const int trash = rowsP + lane;
 
     // adds the tap sums (sr, si) of an origin (tile-local tap-0 offsets d0,
-    // wrap taps kw) into coil c's tile; lanes with do_add false add zeros to
-    // their trash cell. The lanes of one call add distinct origins, so for a
-    // given tap no two lanes touch the same cell and a plain read-add-write
-    // is race-free; the compiler barrier keeps each tap's read behind the
-    // previous tap's write (a wave's LDS operations complete in order), as
-    // another lane's next tap may land on a cell this tap just wrote.
-    auto add_origin = [&](bool do_add, const int (&d0)[NDIM], const int (&kw)[NDIM], int c, int kz,
+    // wrap taps kw), z plane kz, into this lane's coil tile; lanes with
+    // do_add false add zeros to their trash cell. The lanes of one call add
+    // distinct origins or distinct coils, so no two lanes touch the same cell
+    // and a plain read-add-write is race-free; the compiler barrier keeps
+    // each tap's read behind the previous tap's write (a wave's LDS
+    // operations complete in order), as another lane's next tap may land on
+    // a cell this tap just wrote.
+    auto add_origin = [&](bool do_add, const int (&d0)[NDIM], const int (&kw)[NDIM], int kz,
                           const float (&sr)[TAPS], const float (&si)[TAPS]) {
         int lx[WC], lrow[WC];
+        do_add = do_add && active;
 #pragma unroll
         for (int k = 0; k < WC; ++k) {
             const int a = d0[0] + k - (k >= kw[0] ? Gx : 0);
@@ -648,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
             const bool ok = okz && ky < J.W[1] && ay >= 0 && ay < e3[1] && row >= row_lo && row < row_hi;
             lrow[ky] = ok ? row * P : -1;
         }
-        float2 *are = acc + c * plane;
+        float2 *are = acc + cs * plane;
 #pragma unroll
         for (int ky = 0; ky < WC; ++ky) {
 #pragma unroll
@@ -664,13 +676,12 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
         }
     };
 
+    const int nzp = NDIM > 2 ? J.W[2] : 1;  // z planes of the taps, one pass each
     for (int b0 = it.e0; b0 < it.e1; b0 += B) {
         __syncthreads();  // previous batch consumed (and accumulators zeroed)
-        {   // A. thread tid stages the entry that lane (tid % kWave) walks at step
-            //    (tid / kWave), i.e. batch entry (tid % kWave) * RB + tid / kWave,
-            //    in slot tid: LDS writes and the walk's reads are both consecutive
+        {   // A. slot t holds the entry that group t % ng walks at step t / ng
             const int slot = tid;
-            const int e = b0 + (tid % kWave) * RB + tid / kWave;
+            const int e = b0 + (tid % ng) * R + tid / ng;
             if (e < it.e1) {
                 const float4 en = J.entries[e];
                 const float c3[3] = {en.x, en.y, en.z};
@@ -690,13 +701,12 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                 const int p = __float_as_int(en.w);
                 float sc = J.inv_scale;
                 if (weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
-                for (int c = 0; c < ncoil; ++c) sval[c * B + slot] = grid_sample(J, samp, it.c0 + c, p, sc, lr);
+                for (int c = 0; c < ncoil; ++c) sval[slot * SV + c] = grid_sample(J, samp, it.c0 + c, p, sc, lr);
             }
         }
         __syncthreads();
-        const int nvalid = min(RB, max(0, it.e1 - (b0 + lane * RB)));  // entries in this lane's run
-        const int nzp = NDIM > 2 ? J.W[2] : 1;  // z planes of the taps, one pass each
-        for (int c = coil0; c < ncoil; c += cstep)
+        if (ncoil >= kBlock / kWave ? wave * cw >= ncoil : coil >= ncoil) continue;  // wave without work
+        const int nvalid = min(R, max(0, it.e1 - (b0 + grp * R)));  // entries in this group's run
         for (int kz = 0; kz < nzp; ++kz) {
             // B. walk the run (z plane kz of the taps): H = sums of the first
             //    origin, C = current origin
@@ -708,55 +718,52 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
             for (int d = 0; d < NDIM; ++d) hd0[d] = hkw[d] = qd0[d] = qkw[d] = 0;
             int ka = -1, kq = -1;     // first / current origin key (-1: none yet)
             bool first_open = true;   // the current origin is still the first one
+            for (int i = 0; i < nvalid; ++i) {
+                const int s = i * ng + grp;
+                const int key = rec[s];
+                if (kq < 0) {
+                    ka = kq = key;
 #pragma unroll
-            for (int i = 0; i < RB; ++i) {
-                if (i < nvalid) {
-                    const int s = i * kWave + lane;
-                    const int key = rec[s];
-                    if (kq < 0) {
-                        ka = kq = key;
-#pragma unroll
-                        for (int d = 0; d < NDIM; ++d) {
-                            hd0[d] = qd0[d] = rec[(1 + 2 * d) * B + s];
-                            hkw[d] = qkw[d] = rec[(2 + 2 * d) * B + s];
-                        }
-                    } else if (key != kq) {
-                        if (first_open) {  // the first origin ends inside the run: keep it in H
-#pragma unroll
-                            for (int t = 0; t < TAPS; ++t) {
-                                hr[t] = cr[t];
-                                hi[t] = ci[t];
-                            }
-                            first_open = false;
-                        } else {  // an interior origin: unique to this lane
-                            add_origin(true, qd0, qkw, c, kz, cr, ci);
-                        }
-#pragma unroll
-                        for (int t = 0; t < TAPS; ++t) cr[t] = ci[t] = 0.0f;
-                        kq = key;
-#pragma unroll
-                        for (int d = 0; d < NDIM; ++d) {
-                            qd0[d] = rec[(1 + 2 * d) * B + s];
-                            qkw[d] = rec[(2 + 2 * d) * B + s];
-                        }
+                    for (int d = 0; d < NDIM; ++d) {
+                        hd0[d] = qd0[d] = rec[(1 + 2 * d) * B + s];
+                        hkw[d] = qkw[d] = rec[(2 + 2 * d) * B + s];
                     }
-                    const float2 sv = sval[c * B + s];
-                    float wx[WC], wy[WC];
+                } else if (key != kq) {
+                    if (first_open) {  // the first origin ends inside the run: keep it in H
 #pragma unroll
-                    for (int t = 0; t < WC; ++t) {
-                        wx[t] = __int_as_float(rec[(1 + 2 * NDIM + t) * B + s]);
-                        wy[t] = __int_as_float(rec[(1 + 2 * NDIM + WC + t) * B + s]);
-                    }
-                    const float wz = NDIM > 2 ? __int_as_float(rec[(1 + 2 * NDIM + 2 * WC + kz) * B + s]) : 1.0f;
-#pragma unroll
-                    for (int ky = 0; ky < WC; ++ky) {
-                        const float wzy = wz * wy[ky];
-                        const float fr = wzy * sv.x, fi = wzy * sv.y;
-#pragma unroll
-                        for (int kx = 0; kx < WC; ++kx) {
-                            cr[ky * WC + kx] += wx[kx] * fr;
-                            ci[ky * WC + kx] += wx[kx] * fi;
+                        for (int t = 0; t < TAPS; ++t) {
+                            hr[t] = cr[t];
+                            hi[t] = ci[t];
                         }
+                        first_open = false;
+                    } else {  // an interior origin: unique to this group
+                        add_origin(true, qd0, qkw, kz, cr, ci);
+                    }
+#pragma unroll
+                    for (int t = 0; t < TAPS; ++t) cr[t] = ci[t] = 0.0f;
+                    kq = key;
+#pragma unroll
+                    for (int d = 0; d < NDIM; ++d) {
+                        qd0[d] = rec[(1 + 2 * d) * B + s];
+                        qkw[d] = rec[(2 + 2 * d) * B + s];
+                    }
+                }
+                const float2 sv = sval[s * SV + cs];
+                float wx[WC], wy[WC];
+#pragma unroll
+                for (int t = 0; t < WC; ++t) {
+                    wx[t] = __int_as_float(rec[(1 + 2 * NDIM + t) * B + s]);
+                    wy[t] = __int_as_float(rec[(1 + 2 * NDIM + WC + t) * B + s]);
+                }
+                const float wz = NDIM > 2 ? __int_as_float(rec[(1 + 2 * NDIM + 2 * WC + kz) * B + s]) : 1.0f;
+#pragma unroll
+                for (int ky = 0; ky < WC; ++ky) {
+                    const float wzy = wz * wy[ky];
+                    const float fr = wzy * sv.x, fi = wzy * sv.y;
+#pragma unroll
+                    for (int kx = 0; kx < WC; ++kx) {
+                        cr[ky * WC + kx] += wx[kx] * fr;
+                        ci[ky * WC + kx] += wx[kx] * fi;
                     }
                 }
             }
@@ -764,34 +771,33 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
             // origin's sums only if the run has more than one origin.
             const int kb = kq;
             const bool has_h = kq >= 0 && !first_open;
-            const int kb_prev = __shfl_up(kb, 1, kWave);
-            const bool h_to_prev = has_h && lane > 0 && kb_prev == ka;
-            // 1. the first-origin sums of lane l+1 join lane l when lane l ends on that origin
+            const int kb_prev = __shfl_up(kb, cw, kWave);
+            const bool h_to_prev = has_h && grp > 0 && kb_prev == ka;
+            // 1. the first-origin sums of group g+1 join group g when g ends on that origin
             if (__ballot(h_to_prev) != 0) {
-                const int ka_next = __shfl_down(ka, 1, kWave);
-                const bool hn_next = __shfl_down(has_h ? 1 : 0, 1, kWave) != 0;
-                const bool take_next = lane < kWave - 1 && kb >= 0 && hn_next && ka_next == kb;
+                const int ka_next = __shfl_down(ka, cw, kWave);
+                const bool hn_next = __shfl_down(has_h ? 1 : 0, cw, kWave) != 0;
+                const bool take_next = grp < ng - 1 && kb >= 0 && hn_next && ka_next == kb;
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) {
-                    const float nr = __shfl_down(hr[t], 1, kWave);
-                    const float ni = __shfl_down(hi[t], 1, kWave);
+                    const float nr = __shfl_down(hr[t], cw, kWave);
+                    const float ni = __shfl_down(hi[t], cw, kWave);
                     if (take_next) {
                         cr[t] += nr;
                         ci[t] += ni;
                     }
                 }
             }
-            if (__ballot(has_h && !h_to_prev) != 0) add_origin(has_h && !h_to_prev, hd0, hkw, c, kz, hr, hi);
-            // 2. segmented inclusive scan of the last-origin sums over lanes
-            if (__ballot(kb >= 0 && lane > 0 && kb_prev == kb) != 0) {
-#pragma unroll
-                for (int d = 1; d < kWave; d <<= 1) {
-                    const int kp = __shfl_up(kb, d, kWave);
-                    const bool join = lane >= d && kp == kb && kb >= 0;
+            if (__ballot(has_h && !h_to_prev) != 0) add_origin(has_h && !h_to_prev, hd0, hkw, kz, hr, hi);
+            // 2. segmented inclusive scan of the last-origin sums over the groups
+            if (__ballot(kb >= 0 && grp > 0 && kb_prev == kb) != 0) {
+                for (int d = 1; d < ng; d <<= 1) {
+                    const int kp = __shfl_up(kb, d * cw, kWave);
+                    const bool join = grp >= d && kp == kb && kb >= 0;
 #pragma unroll
                     for (int t = 0; t < TAPS; ++t) {
-                        const float pr = __shfl_up(cr[t], d, kWave);
-                        const float pi = __shfl_up(ci[t], d, kWave);
+                        const float pr = __shfl_up(cr[t], d * cw, kWave);
+                        const float pi = __shfl_up(ci[t], d * cw, kWave);
                         if (join) {
                             cr[t] += pr;
                             ci[t] += pi;
@@ -799,9 +805,9 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                     }
                 }
             }
-            const int kb_next = __shfl_down(kb, 1, kWave);
-            const bool seg_end = kb >= 0 && (lane == kWave - 1 || kb_next != kb);
-            add_origin(seg_end, qd0, qkw, c, kz, cr, ci);
+            const int kb_next = __shfl_down(kb, cw, kWave);
+            const bool seg_end = kb >= 0 && (grp == ng - 1 || kb_next != kb);
+            add_origin(seg_end, qd0, qkw, kz, cr, ci);
         }
     }
     __syncthreads();

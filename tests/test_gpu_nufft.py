@@ -262,6 +262,7 @@ GRID_CASES = [
     (2, 40, 4, 2.0, 1, 0),    # one coil: waves own row bands
     (2, 40, 4, 2.0, 3, 3),    # tiny split: every tile in parts
     (2, 40, 4, 2.0, 8, 64),
+    (2, 24, 4, 2.0, 20, 0),   # 20 coils: 8-lane coil groups, idle lanes
     (2, 36, 6, 1.5, 2, 0),    # 36 taps
     (2, 32, 8, 2.0, 5, 128),  # 64 taps per plane (per-sample atomics only)
     (2, 30, 5, 2.0, 6, 0),    # 25 taps
