@@ -57,6 +57,9 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.bin_off = plan->d_bin_off + (int64_t)f * plan->ntiles_total;
         j.pentries = plan->d_pentries;
         j.pbin_off = plan->d_pbin_off + (int64_t)f * plan->nitiles_total;
+        j.mrow = plan->d_mrow ? plan->d_mrow + (int64_t)f * gvol : nullptr;
+        j.mnz = plan->d_mnz;
+        j.mlong = p.grid_split == 0 ? kMatrixLongRow : p.grid_split < 0 ? INT32_MAX : p.grid_split;
         // deepmr layout (batch, nframes, ...): frame f of coil b
         j.grid_off = grid_off + (int64_t)f * gvol;
         j.grid_bstride = (int64_t)p.nframes * gvol;
@@ -83,6 +86,8 @@ void free_tables(LaunchTables &t) {
     if (t.d_fold) (void)hipFree(t.d_fold);
     if (t.d_slot_base) (void)hipFree(t.d_slot_base);
     if (t.d_scratch) (void)hipFree(t.d_scratch);
+    if (t.d_tr) (void)hipFree(t.d_tr);
+    if (t.d_st) (void)hipFree(t.d_st);
     t = LaunchTables{};
 }
 
@@ -134,7 +139,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         if (pl->grid_mode != gmode) return fail(TN_E_ARG, "batched plans must share the gridding variant");
     t.grid_mode = gmode;
     const int gndim = t.kind_grid / 16;
-    if (grid_lds_bytes(1, max_plane, wc, gmode, gndim) > kGridLdsBudget)
+    if (gmode != kGridMatrix && grid_lds_bytes(1, max_plane, wc, gmode, gndim) > kGridLdsBudget)
         return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
                     grid_lds_bytes(1, max_plane, wc, gmode, gndim));
     int lds_coils = 1;
@@ -151,7 +156,9 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     t.interp_mode = mode;
 
     int max_gc = 1, max_ic = 1;
-    std::vector<Item> interp, grid, fold;
+    int st_cw = 0;
+    bool st_none = false;
+    std::vector<Item> interp, grid, fold, tr;
     std::vector<int64_t> slot_base;
     int64_t scratch = 0;
     for (size_t ji = 0; ji < jobs.size(); ++ji) {
@@ -184,6 +191,39 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                 for (int c0 = 0; c0 < j.nbatch; c0 += ic)
                     interp.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + ic), 0, 0, 0, 0});
         }
+        if (gmode == kGridMatrix) {
+            // rows longer than mlong: one workgroup each (part 1, unit = cell),
+            // queued first; then blocks of kBlock cells, one lane per cell
+            // (part 0, unit = block), which skip the long rows
+            const int mc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, kMatrixChunk), kMatrixCoils);
+            max_gc = std::max(max_gc, mc);
+            const int64_t ncell = (int64_t)j.G[0] * j.G[1] * j.G[2];
+            const int64_t f = (j.mrow - pl->d_mrow) / std::max<int64_t>(ncell, 1);
+            const int32_t *row = pl->h_mrow.data() + f * ncell;
+            for (int64_t c = 0; c < ncell; ++c)
+                if (row[c + 1] - row[c] > j.mlong)
+                    for (int c0 = 0; c0 < j.nbatch; c0 += mc)
+                        grid.push_back(Item{(int)ji, (int)c, c0, std::min(j.nbatch, c0 + mc), c0 / mc, 0, 1, -1});
+            // cells per item: kBlock for the lane-per-cell kernel; kMatrixPasses
+            // passes of kBlock / cw cells for the coil-minor kernel
+            const int cw = mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16;
+            const int64_t bcells = mc >= 2 ? (int64_t)kBlock / cw * kMatrixPasses : kBlock;
+            const int64_t nblk = (ncell + bcells - 1) / bcells;
+            for (int64_t b = 0; b < nblk; ++b)
+                for (int c0 = 0; c0 < j.nbatch; c0 += mc)
+                    grid.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + mc), c0 / mc, (int)bcells, 0, -1});
+            // coil-minor sample copy: chunks of mc coils padded to st_cw lanes
+            if (mc >= 2) {
+                st_cw = std::max(st_cw, cw);
+                const int64_t pblk = (j.npts + kBlock - 1) / kBlock;
+                for (int c0 = 0; c0 < j.nbatch; c0 += mc)
+                    for (int64_t b = 0; b < pblk; ++b)
+                        tr.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + mc), c0 / mc, 0, 0, -1});
+            } else {
+                st_none = true;  // a one-coil chunk: the lane-per-cell kernel reads in place
+            }
+            continue;
+        }
         const int64_t ntiles = (int64_t)j.NT[0] * j.NT[1] * j.NT[2];
         const int64_t fg = (j.bin_off - pl->d_bin_off) / std::max<int64_t>(pl->ntiles_total, 1);
         const int32_t *goff = pl->h_bin_off.data() + fg * pl->ntiles_total;
@@ -213,10 +253,25 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     }
     t.scratch_floats = scratch;
     t.grid_coils = max_gc;
-    t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc, gmode, gndim);
+    std::vector<Job> jobs_st(jobs);
+    if (gmode == kGridMatrix && st_cw > 0 && !st_none) {  // every job gets its copy at st_cw coils per record
+        int64_t off = 0;
+        for (size_t ji = 0; ji < jobs.size(); ++ji) {
+            const int mc = std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch, kMatrixChunk), kMatrixCoils);
+            jobs_st[ji].st_off = off;
+            off += (int64_t)((jobs[ji].nbatch + mc - 1) / mc) * jobs[ji].npts * st_cw;
+        }
+        t.st_cw = st_cw;
+        t.st_elems = off;
+        t.n_tr = (int64_t)tr.size();
+        TN_HIP(hipMalloc(&t.d_tr, sizeof(Item) * std::max<size_t>(tr.size(), 1)));
+        if (!tr.empty()) TN_HIP(hipMemcpy(t.d_tr, tr.data(), sizeof(Item) * tr.size(), hipMemcpyHostToDevice));
+        TN_HIP(hipMalloc(&t.d_st, sizeof(float2) * std::max<int64_t>(off, 1)));
+    }
+    t.grid_lds = gmode == kGridMatrix ? 0 : grid_lds_bytes(max_gc, max_plane, wc, gmode, gndim);
     t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
     t.interp_coils = max_ic;
-    return upload_tables(jobs, interp, grid, fold, slot_base, t);
+    return upload_tables(jobs_st, interp, grid, fold, slot_base, t);
 }
 
 }  // namespace tn
@@ -248,7 +303,7 @@ int validate(const tn_plan_params *p) {
     if ((int64_t)p->grid[0] * p->grid[1] * p->grid[2] >= INT32_MAX)
         return fail(TN_E_RANGE, "grid too large");
     if (p->interp_mode < 0 || p->interp_mode > 2) return fail(TN_E_ARG, "interp_mode must be 0, 1 or 2");
-    if (p->grid_mode < 0 || p->grid_mode > 2) return fail(TN_E_ARG, "grid_mode must be 0, 1 or 2");
+    if (p->grid_mode < 0 || p->grid_mode > 3) return fail(TN_E_ARG, "grid_mode must be 0, 1, 2 or 3");
     return TN_OK;
 }
 
@@ -294,6 +349,8 @@ void destroy(tn_plan *pl) {
     if (pl->d_bin_off) (void)hipFree(pl->d_bin_off);
     if (pl->d_pentries) (void)hipFree(pl->d_pentries);
     if (pl->d_pbin_off) (void)hipFree(pl->d_pbin_off);
+    if (pl->d_mrow) (void)hipFree(pl->d_mrow);
+    if (pl->d_mnz) (void)hipFree(pl->d_mnz);
     for (int d = 0; d < 3; ++d)
         if (pl->d_apod[d]) (void)hipFree(pl->d_apod[d]);
     (void)hipSetDevice(cur);
@@ -379,7 +436,10 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
             delete pl;
             return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per z plane (got %d)", kRunsMaxTaps, taps);
         }
-        pl->grid_mode = params->grid_mode == kGridSpread ? kGridSpread : runs_ok ? kGridRuns : kGridSpread;
+        pl->grid_mode = params->grid_mode == kGridSpread   ? kGridSpread
+                        : params->grid_mode == kGridMatrix ? kGridMatrix
+                        : runs_ok                          ? kGridRuns
+                                                           : kGridSpread;
     }
     const int64_t ncoord = (int64_t)params->nframes * params->npts * params->ndim;
     auto cleanup = [&](int code) {
@@ -421,6 +481,9 @@ int64_t tn_plan_device_bytes(const tn_plan *plan) {
     b += sizeof(int32_t) * (plan->ntiles_total * p.nframes + 1);
     b += sizeof(float4) * (int64_t)p.nframes * p.npts;
     b += sizeof(int32_t) * (plan->nitiles_total * p.nframes + 1);
+    if (plan->d_mrow)
+        b += sizeof(int32_t) * ((int64_t)p.grid[0] * p.grid[1] * p.grid[2] * p.nframes + 1) +
+             sizeof(int2) * plan->mnnz;
     for (int d = 0; d < 3; ++d)
         if (plan->d_apod[d]) b += sizeof(float) * p.shape[d];
     for (const auto &kv : plan->cache)

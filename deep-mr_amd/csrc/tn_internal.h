@@ -27,7 +27,14 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 // Gridding variants: one LDS atomic per tap and sample over entries spread
 // across footprint origins, or per-lane register runs over origin-sorted
 // entries (for at most kRunsMaxTaps taps per sample).
-enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2 };
+// Matrix: per-cell sums over the plan's sparse gridding matrix (CSR by grid
+// cell), one lane per cell.
+enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2, kGridMatrix = 3 };
+constexpr int kMatrixCoils = 16;  // coils summed per lane by the matrix gridding
+constexpr int kMatrixLongRow = 128;  // default: rows with more entries get a whole workgroup
+constexpr int kMatrixChunk = 8;      // default coils per item (record width of the coil-minor copy)
+constexpr int kMatrixPasses = 2;     // cell passes per item of the coil-minor matrix gridding
+constexpr int kXcds = 8;          // accelerator complex dies (each with its own L2)
 constexpr int kRunsMaxTaps = 36;  // register budget of the run sums of one z plane (W <= 6)
 constexpr int kRunsBatch = 256;                 // entries per staged batch (= kBlock)
 constexpr int kRunsMaxCoils = 64;               // coils per item: 4 waves x up to 16 lanes per group
@@ -63,7 +70,7 @@ struct alignas(16) Job {
     float inv_scale;
     int32_t ndim;
     int32_t nbatch;     // coils (batch) of this job
-    int32_t pad0;
+    int32_t mlong;      // matrix gridding: rows with more entries are summed by a whole workgroup
     int64_t npts;
     const float *coord;       // (npts, ndim) grid units
     const float4 *entries;    // (x, y, z, bits(sample)) sorted by tile
@@ -71,11 +78,14 @@ struct alignas(16) Job {
     const float4 *pentries;   // each sample once, sorted by the interpolation
                               // tile holding its footprint origin
     const int32_t *pbin_off;  // (interp tiles + 1) offsets into pentries
+    const int32_t *mrow;      // matrix gridding: (cells + 1) row offsets of this frame
+    const int2 *mnz;          // matrix gridding: (sample, weight bits) per entry
     int64_t grid_off;         // complex elements from the grid base
     int64_t grid_bstride;     // complex elements between coils
     int64_t samp_off;
     int64_t samp_bstride;
     int64_t w_off;            // offset into the weight base (-1: no weight)
+    int64_t st_off;           // matrix gridding: this job's coil-minor sample copy in the table's buffer
 };
 
 // Work item. Gathering interpolation: unit = sample block. Staged
@@ -118,6 +128,13 @@ struct LaunchTables {
     size_t interp_lds = 0;     // dynamic LDS bytes for the staged interpolation
     int interp_coils = 1;      // coils per LDS region load (staged interpolation)
     int grid_mode = kGridSpread;  // gridding variant (matches the plans' entry order)
+    // matrix gridding over a coil-minor copy of the samples: chunks of
+    // st_cw coils, [chunk][sample][st_cw] per job, built by the transpose items
+    Item *d_tr = nullptr;
+    int64_t n_tr = 0;
+    float2 *d_st = nullptr;
+    int64_t st_elems = 0;
+    int st_cw = 0;  // 0: the lane-per-cell matrix kernel reads the samples in place
 };
 
 }  // namespace tn
@@ -135,6 +152,7 @@ struct tn_plan {
     int32_t *d_pbin_off = nullptr;
     std::vector<int32_t> h_pbin_off;  // host copies (work-list construction)
     std::vector<int32_t> h_bin_off;
+    std::vector<int32_t> h_mrow;      // host copy of d_mrow (work-list construction)
     float *d_coord = nullptr;
     float4 *d_entries = nullptr;
     int32_t *d_bin_off = nullptr;
@@ -142,6 +160,9 @@ struct tn_plan {
     int64_t nentries = 0;
     int64_t footprint = -1;
     int grid_mode = tn::kGridSpread;  // effective gridding variant (entry order)
+    int32_t *d_mrow = nullptr;  // matrix gridding: row offsets, (nframes * cells + 1)
+    int2 *d_mnz = nullptr;      // matrix gridding: (sample, weight bits) sorted by (frame, cell)
+    int64_t mnnz = 0;
     std::map<int64_t, tn::LaunchTables> cache;
     std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank)
 };
@@ -176,6 +197,7 @@ int launch_apod_pad(const tn_plan *plan, const float2 *image, float2 *grid,
 int launch_crop_apod(const tn_plan *plan, const float2 *grid, float2 *image,
                      int64_t nimg, hipStream_t stream);
 int build_bins(tn_plan *plan, hipStream_t stream);
+int build_matrix(tn_plan *plan, hipStream_t stream);
 int count_footprint(tn_plan *plan, hipStream_t stream);
 
 // Variant selection shared by the table builder and the launchers.

@@ -60,6 +60,10 @@ struct Idx4 {
     }
 };
 
+// Clang vector types for the nontemporal load/store builtins.
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 template <int NDIM, int WC>
 struct Taps {
     int idx[3][WC];
@@ -815,6 +819,257 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                      scratch, slot_base);
 }
 
+// Gridding by the plan's sparse gridding matrix: per frame, every (cell,
+// sample, weight) triple of the kernel footprints, sorted by output cell
+// (CSR), samples ascending inside a cell. Two kinds of work item, both
+// summing weight * scaled sample for up to CM coils in registers and writing
+// each cell of every coil once (zeros included) with streaming stores:
+//   part 0: a block of it.e1 <= kBlock cells, one lane per cell, walking its
+//           row with the next entry prefetched; rows longer than J.mlong are
+//           skipped;
+//   part 1: one such long row (dense trajectory centres hold hundreds of
+//           entries per cell), its entries strided over the workgroup and
+//           the partial sums reduced by a fixed butterfly per wave, then
+//           across the waves in wave order.
+// No atomics, and a fixed summation order for every cell. Long rows are
+// queued first; items are dealt to the XCDs in contiguous ranges (the
+// hardware deals workgroups out round-robin), so the neighbouring rows of
+// one frame, which read mostly the same samples, share one XCD's L2.
+// Low rank (LR): accumulator coils are (coil, coeff) pairs; the sample of
+// frame f enters coefficient k times basis[k, f].
+template <int CM, bool LR>
+__global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restrict__ jobs,
+                                                             const Item *__restrict__ items, int64_t nitems,
+                                                             const float2 *__restrict__ samp,
+                                                             float2 *__restrict__ grid,
+                                                             const float *__restrict__ weight, LowRank lr) {
+    __shared__ float red[kBlock / kWave][2 * CM];
+    const int64_t per = (nitems + kXcds - 1) / kXcds;
+    const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (idx >= nitems) return;
+    const Item it = items[idx];
+    const Job &J = jobs[it.job];
+    const int ncoil = it.c1 - it.c0;
+    const bool dcf = weight != nullptr && J.w_off >= 0;
+    // per-coil sample rows (uniform); coils past the item's last repeat it
+    // (loads stay in bounds and unconditional, their sums are not stored)
+    const char *cb[CM];
+    const float2 *qb[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+        const int vc = it.c0 + min(c, ncoil - 1);
+        const int b = LR ? vc / lr.ncoeff : vc;
+        cb[c] = (const char *)(samp + J.samp_off + (int64_t)b * J.samp_bstride);
+        qb[c] = LR ? lr.coef + (int64_t)(vc - b * lr.ncoeff) * lr.nframes : nullptr;
+    }
+    float ar[CM], ai[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) ar[c] = ai[c] = 0.0f;
+    auto add_entry = [&](i32x2 m) {
+        const uint32_t boff = (uint32_t)m.x * (uint32_t)sizeof(float2);  // npts < 2^28
+        float w = __int_as_float(m.y) * J.inv_scale;
+        if (dcf) w *= weight[J.w_off + m.x];
+        float2 v[CM];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) v[c] = *(const float2 *)(cb[c] + boff);
+        if (!LR) {
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                ar[c] += w * v[c].x;
+                ai[c] += w * v[c].y;
+            }
+        } else {
+            const int f = m.x / lr.frame_npts;
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                const float2 q = qb[c][f];
+                ar[c] += w * (q.x * v[c].x - q.y * v[c].y);
+                ai[c] += w * (q.x * v[c].y + q.y * v[c].x);
+            }
+        }
+    };
+    const i32x2 *mnz = (const i32x2 *)J.mnz;
+    if (it.part == 0) {  // it.e1 (<= kBlock) cells per item
+        const int64_t ncell = (int64_t)J.G[0] * J.G[1] * J.G[2];
+        const int64_t cell = (int64_t)it.unit * it.e1 + threadIdx.x;
+        if ((int)threadIdx.x >= it.e1 || cell >= ncell) return;
+        const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+        if (r1 - r0 > J.mlong) return;  // written by its long-row item
+        if (r0 < r1) {
+            i32x2 m = __builtin_nontemporal_load(mnz + r0);
+            for (int e = r0; e < r1; ++e) {
+                const i32x2 mn = __builtin_nontemporal_load(mnz + min(e + 1, r1 - 1));
+                add_entry(m);
+                m = mn;
+            }
+        }
+        float2 *g = grid + J.grid_off + cell;
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+            if (c < ncoil) {
+                const f32x2 v = {ar[c], ai[c]};
+                __builtin_nontemporal_store(v, (f32x2 *)(g + (int64_t)(it.c0 + c) * J.grid_bstride));
+            }
+        }
+        return;
+    }
+    const int64_t cell = it.unit;
+    const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+    for (int e = r0 + (int)threadIdx.x; e < r1; e += kBlock) add_entry(__builtin_nontemporal_load(mnz + e));
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) {
+            ar[c] += __shfl_xor(ar[c], d, kWave);
+            ai[c] += __shfl_xor(ai[c], d, kWave);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+            red[wave][2 * c] = ar[c];
+            red[wave][2 * c + 1] = ai[c];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncoil) {
+        const int c = threadIdx.x;
+        float re = 0.0f, im = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            re += red[w][2 * c];
+            im += red[w][2 * c + 1];
+        }
+        grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + cell] = make_float2(re, im);
+    }
+}
+
+// Coil-minor copy of the samples for the matrix gridding: chunk q of a job
+// (coils c0 .. c1 - 1 of the item) becomes st[q][p][CW] = samp[c0 + j][p]
+// * inv_scale * weight[p], zeros for j >= c1 - c0. One item = kBlock
+// samples of one chunk, transposed through LDS: coalesced reads along the
+// samples of each coil, contiguous writes.
+template <int LCW>
+__global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__restrict__ jobs,
+                                                                const Item *__restrict__ items,
+                                                                const float2 *__restrict__ samp,
+                                                                const float *__restrict__ weight,
+                                                                float2 *__restrict__ st) {
+    constexpr int CW = 1 << LCW;
+    __shared__ float2 tile[kBlock * (CW + 1)];
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)it.unit * kBlock;
+    const int64_t p = p0 + t;
+    const int nc = it.c1 - it.c0;
+    float sc = J.inv_scale;
+    if (p < J.npts && weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (j < nc && p < J.npts) {
+            const float2 a = samp[J.samp_off + (int64_t)(it.c0 + j) * J.samp_bstride + p];
+            v = make_float2(a.x * sc, a.y * sc);
+        }
+        tile[t * (CW + 1) + j] = v;
+    }
+    __syncthreads();
+    float2 *dst = st + J.st_off + ((int64_t)it.e0 * J.npts + p0) * CW;
+    const int np = (int)min<int64_t>(kBlock, J.npts - p0);
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const int flat = k * kBlock + t;
+        const int pp = flat >> LCW, j = flat & (CW - 1);
+        if (pp < np) dst[flat] = tile[pp * (CW + 1) + j];
+    }
+}
+
+// Matrix gridding over the coil-minor sample copy: one lane per (cell,
+// coil), the CW lanes of a cell reading one contiguous CW-coil record per
+// entry (one cache line for 16 coils instead of one line per coil). Items
+// as for grid_matrix_kernel (it.e0 = chunk index): part 0 walks a block of
+// it.e1 cells, kBlock / CW cells per pass, each row four entries at a time
+// (independent loads in flight); part 1 strides one long row's entries over
+// the cell groups of the workgroup and reduces the group sums with a fixed
+// butterfly per wave, then across the waves in wave order.
+template <int LCW>
+__global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
+                                                               const Item *__restrict__ items, int64_t nitems,
+                                                               const float2 *__restrict__ st,
+                                                               float2 *__restrict__ grid) {
+    constexpr int CW = 1 << LCW;
+    constexpr int CPP = kBlock / CW;  // cells per pass
+    constexpr int U = 4;              // entries in flight per lane
+    __shared__ float2 red[kBlock / kWave][CW];
+    const int64_t per = (nitems + kXcds - 1) / kXcds;
+    const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (idx >= nitems) return;
+    const Item it = items[idx];
+    const Job &J = jobs[it.job];
+    const int j = threadIdx.x & (CW - 1), g = threadIdx.x >> LCW;
+    const float2 *sq = st + J.st_off + (int64_t)it.e0 * J.npts * CW + j;
+    const bool store = it.c0 + j < it.c1;
+    float2 *gout = grid + J.grid_off + (int64_t)(it.c0 + (store ? j : 0)) * J.grid_bstride;
+    const i32x2 *mnz = (const i32x2 *)J.mnz;
+    // entries e0, e0 + stride, ... below r1 of one row, U at a time
+    auto row_sum = [&](int e0, int r1, int stride, float &re, float &im) {
+        for (int e = e0; e < r1; e += U * stride) {
+            i32x2 m[U];
+            float2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) m[u] = __builtin_nontemporal_load(mnz + min(e + u * stride, r1 - 1));
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = sq[(int64_t)m[u].x * CW];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float w = e + u * stride < r1 ? __int_as_float(m[u].y) : 0.0f;
+                re += w * v[u].x;
+                im += w * v[u].y;
+            }
+        }
+    };
+    if (it.part == 0) {
+        const int64_t ncell = (int64_t)J.G[0] * J.G[1] * J.G[2];
+        const int npass = it.e1 / CPP;
+        for (int pass = 0; pass < npass; ++pass) {
+            const int64_t cell = (int64_t)it.unit * it.e1 + pass * CPP + g;
+            if (cell >= ncell) break;
+            const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+            if (r1 - r0 > J.mlong) continue;  // written by its long-row item
+            float re = 0.0f, im = 0.0f;
+            row_sum(r0, r1, 1, re, im);
+            if (store) {
+                const f32x2 v = {re, im};
+                __builtin_nontemporal_store(v, (f32x2 *)(gout + cell));
+            }
+        }
+        return;
+    }
+    const int64_t cell = it.unit;
+    const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+    float re = 0.0f, im = 0.0f;
+    row_sum(r0 + g, r1, CPP, re, im);
+#pragma unroll
+    for (int d = CW; d < kWave; d <<= 1) {
+        re += __shfl_xor(re, d, kWave);
+        im += __shfl_xor(im, d, kWave);
+    }
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane < CW) red[wave][lane] = make_float2(re, im);
+    __syncthreads();
+    if ((int)threadIdx.x < CW && store) {
+        float2 a = red[0][j];
+#pragma unroll
+        for (int w = 1; w < kBlock / kWave; ++w) {
+            a.x += red[w][j].x;
+            a.y += red[w][j].y;
+        }
+        gout[cell] = a;
+    }
+}
+
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
 __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jobs,
                                                       const Item *__restrict__ items,
@@ -1032,9 +1287,52 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
     return TN_OK;
 }
 
+using MatrixFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, LowRank);
+
+template <bool LR>
+MatrixFn pick_matrix(int coils) {
+    if (coils <= 1) return grid_matrix_kernel<1, LR>;
+    if (coils <= 2) return grid_matrix_kernel<2, LR>;
+    if (coils <= 4) return grid_matrix_kernel<4, LR>;
+    if (coils <= 8) return grid_matrix_kernel<8, LR>;
+    return grid_matrix_kernel<kMatrixCoils, LR>;
+}
+
 int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const float *weight,
                 hipStream_t stream, LowRank lr) {
     if (t.n_grid == 0) return TN_OK;
+    if (t.grid_mode == kGridMatrix && lr.ncoeff == 0 && t.st_cw > 0) {
+        using TrFn = void (*)(const Job *, const Item *, const float2 *, const float *, float2 *);
+        using MtFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *);
+        TrFn tr = nullptr;
+        MtFn mt = nullptr;
+        switch (t.st_cw) {
+            case 2: tr = samp_transpose_kernel<1>; mt = grid_matrix_t_kernel<1>; break;
+            case 4: tr = samp_transpose_kernel<2>; mt = grid_matrix_t_kernel<2>; break;
+            case 8: tr = samp_transpose_kernel<3>; mt = grid_matrix_t_kernel<3>; break;
+            case 16: tr = samp_transpose_kernel<4>; mt = grid_matrix_t_kernel<4>; break;
+            default: return fail(TN_E_ARG, "matrix gridding: %d coils per record", t.st_cw);
+        }
+        if (t.n_tr > 0) {
+            hipLaunchKernelGGL(tr, dim3((unsigned)t.n_tr), dim3(kBlock), 0, stream, t.d_jobs, t.d_tr, samp, weight,
+                               t.d_st);
+            TN_HIP(hipGetLastError());
+        }
+        const int64_t nblk = (t.n_grid + kXcds - 1) / kXcds * kXcds;
+        hipLaunchKernelGGL(mt, dim3((unsigned)nblk), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid,
+                           (const float2 *)t.d_st, grid);
+        TN_HIP(hipGetLastError());
+        return TN_OK;
+    }
+    if (t.grid_mode == kGridMatrix) {
+        if (t.grid_coils > kMatrixCoils) return fail(TN_E_ARG, "matrix gridding: %d coils per item", t.grid_coils);
+        MatrixFn fn = lr.ncoeff > 0 ? pick_matrix<true>(t.grid_coils) : pick_matrix<false>(t.grid_coils);
+        const int64_t nblk = (t.n_grid + kXcds - 1) / kXcds * kXcds;
+        hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid, samp,
+                           grid, weight, lr);
+        TN_HIP(hipGetLastError());
+        return TN_OK;
+    }
     const int ndim = t.kind_grid / 16, wc = t.kind_grid % 16;
     GridFn fn = nullptr;
     if (t.grid_mode == kGridRuns)

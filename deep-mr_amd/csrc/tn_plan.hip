@@ -24,6 +24,7 @@ struct AxisGeom {
 };
 
 struct BinGeom {
+    float beta[3];       // Kaiser-Bessel beta (x, y, z)
     int ndim;
     int64_t npts;
     int nframes;
@@ -46,6 +47,8 @@ __device__ __forceinline__ int axis_tiles(const AxisGeom &a, int base, int (&tl)
     }
     return n;
 }
+
+__device__ __forceinline__ float beta_of(const BinGeom &g, int d) { return g.beta[d]; }
 
 __device__ __forceinline__ int axis_base(const AxisGeom &a, float g) {
     if (a.W == 1) return wrap_mod((int)g + a.shift, a.G);
@@ -122,6 +125,43 @@ __global__ void bin_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
         const int64_t k = e < n ? (int64_t)keys[e] : nbins;
         const int64_t kp = e > 0 ? (int64_t)keys[e - 1] : -1;
         for (int64_t b = kp + 1; b <= k; ++b) bin_off[b] = (int32_t)e;
+    }
+}
+
+// Gridding matrix entries of every sample, in (sample, tap) order, tap x
+// fastest: key = frame * cells + output cell, value = (sample within its
+// frame, weight bits) packed low/high in 64 bits. The weight is the product
+// of the per-axis Kaiser-Bessel taps (z * y first, then x), the same taps
+// the other gridding variants evaluate.
+__global__ void matrix_emit_kernel(BinGeom geo, const float *__restrict__ coord, int taps,
+                                   uint32_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+    const int64_t total = geo.npts * geo.nframes;
+    const int64_t ncell = (int64_t)geo.ax[0].G * geo.ax[1].G * geo.ax[2].G;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frame = i / geo.npts;
+        const int64_t p = i - frame * geo.npts;
+        int base[3] = {0, 0, 0};
+        float w[3][8];
+        for (int d = 0; d < 3; ++d) {
+            if (d < geo.ndim) {
+                axis_taps<8>(coord[i * geo.ndim + d], geo.ax[d].W, geo.ax[d].G, geo.ax[d].shift, beta_of(geo, d),
+                             base[d], w[d]);
+            } else {
+                w[d][0] = 1.0f;
+            }
+        }
+        const int Wx = geo.ax[0].W, Wy = geo.ax[1].W;
+        const int64_t e0 = i * taps;
+        for (int t = 0; t < taps; ++t) {
+            const int kx = t % Wx, ky = (t / Wx) % Wy, kz = t / (Wx * Wy);
+            const int64_t cell = ((int64_t)wrap_add(base[2], kz, geo.ax[2].G) * geo.ax[1].G +
+                                  wrap_add(base[1], ky, geo.ax[1].G)) * geo.ax[0].G +
+                                 wrap_add(base[0], kx, geo.ax[0].G);
+            const float wt = (w[2][kz] * w[1][ky]) * w[0][kx];
+            keys[e0 + t] = (uint32_t)(frame * ncell + cell);
+            vals[e0 + t] = ((uint64_t)(uint32_t)__float_as_int(wt) << 32) | (uint64_t)(uint32_t)p;
+        }
     }
 }
 
@@ -242,6 +282,7 @@ BinGeom make_geom(const tn_plan *plan, bool primary = false) {
         g.ax[d].G = plan->p.grid[zyx];
         g.ax[d].W = plan->p.width[zyx];
         g.ax[d].shift = plan->p.shift[zyx];
+        g.beta[d] = plan->p.beta[zyx];
         g.ax[d].T = primary ? plan->itile[zyx] : plan->tile[zyx];
         g.NT[d] = primary ? plan->nitiles[zyx] : plan->ntiles[zyx];
     }
@@ -388,7 +429,12 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
     }
 
     // ---- gridding bins: every tile a footprint touches --------------------
-    {
+    if (plan->grid_mode == kGridMatrix) {  // gridding reads the matrix instead
+        TN_HIP(hipMemsetAsync(plan->d_bin_off, 0, sizeof(int32_t) * (nbins + 1), stream));
+        plan->nentries = 0;
+        int rc = build_matrix(plan, stream);
+        if (rc) return rc;
+    } else {
         DevBuf counts, offsets, scan_tmp;
         TN_HIP(hipMalloc(&counts.p, sizeof(int32_t) * total));
         TN_HIP(hipMalloc(&offsets.p, sizeof(int32_t) * (total + 1)));
@@ -439,6 +485,55 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
         TN_HIP(hipMemcpy(plan->h_pbin_off.data(), plan->d_pbin_off, sizeof(int32_t) * (npbins + 1),
                          hipMemcpyDeviceToHost));
     }
+    return TN_OK;
+}
+
+// Sparse gridding matrix of every frame, CSR by output cell: the entries of
+// every sample are sorted by (frame, cell) with a stable radix sort, so the
+// samples of one cell stay in ascending order (a fixed summation order).
+int build_matrix(tn_plan *plan, hipStream_t stream) {
+    const BinGeom geo = make_geom(plan);
+    const int64_t total = geo.npts * geo.nframes;
+    const int64_t ncell = (int64_t)plan->p.grid[0] * plan->p.grid[1] * plan->p.grid[2];
+    const int64_t nrows = ncell * geo.nframes;
+    const int taps = plan->p.width[0] * plan->p.width[1] * plan->p.width[2];
+    const int64_t n = total * taps;
+    if (nrows >= (int64_t)UINT32_MAX) return fail(TN_E_RANGE, "gridding matrix: too many rows (%lld)", (long long)nrows);
+    if (n >= INT32_MAX) return fail(TN_E_RANGE, "gridding matrix: too many entries (%lld)", (long long)n);
+    if (geo.npts > ((int64_t)1 << 28)) return fail(TN_E_RANGE, "gridding matrix: too many samples per frame");
+    TN_HIP(hipMalloc(&plan->d_mrow, sizeof(int32_t) * (nrows + 1)));
+    TN_HIP(hipMalloc(&plan->d_mnz, sizeof(int2) * std::max<int64_t>(n, 1)));
+    plan->mnnz = n;
+    if (n == 0) {
+        TN_HIP(hipMemsetAsync(plan->d_mrow, 0, sizeof(int32_t) * (nrows + 1), stream));
+        TN_HIP(hipStreamSynchronize(stream));
+        plan->h_mrow.assign(nrows + 1, 0);
+        return TN_OK;
+    }
+    DevBuf keys, vals, keys_sorted, tmp;
+    TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * n));
+    TN_HIP(hipMalloc(&vals.p, sizeof(uint64_t) * n));
+    TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(matrix_emit_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo, plan->d_coord,
+                       taps, (uint32_t *)keys.p, (uint64_t *)vals.p);
+    TN_HIP(hipGetLastError());
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)nrows) ++end_bit;
+    size_t bytes = 0;
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t *)keys.p, (uint32_t *)keys_sorted.p,
+                                              (uint64_t *)vals.p, (uint64_t *)plan->d_mnz, (int)n, 0, end_bit,
+                                              stream));
+    TN_HIP(hipMalloc(&tmp.p, bytes));
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, (uint32_t *)keys.p, (uint32_t *)keys_sorted.p,
+                                              (uint64_t *)vals.p, (uint64_t *)plan->d_mnz, (int)n, 0, end_bit,
+                                              stream));
+    hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(n + 1)), dim3(kBlock), 0, stream,
+                       (const uint32_t *)keys_sorted.p, n, nrows, plan->d_mrow);
+    TN_HIP(hipGetLastError());
+    plan->h_mrow.resize(nrows + 1);
+    TN_HIP(hipMemcpyAsync(plan->h_mrow.data(), plan->d_mrow, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost,
+                          stream));
+    TN_HIP(hipStreamSynchronize(stream));  // temporaries are freed on scope exit
     return TN_OK;
 }
 

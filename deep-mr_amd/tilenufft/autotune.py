@@ -27,9 +27,9 @@ def default_candidates(ndim):
     else:
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
         itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
-    return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8], interp_chunks=[1, 2, 4, 8, 16],
+    return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8, 16], interp_chunks=[1, 2, 4, 8, 16],
                 itiles=itiles, staged_chunks=[2, 4, 8, 16], grid_splits=[512, 1024, 2048, 4096],
-                grid_modes=[2, 1])
+                matrix_splits=[64, 128, 256], grid_modes=[3, 2, 1])
 
 
 def _cache_path():
@@ -94,7 +94,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         # 1) gridding variant x tile, automatic chunks
         best_tile, best_t, best_gm = None, float("inf"), 0
         for gm in cand.get("grid_modes", [0]):
-            for tile in cand["tiles"]:
+            # the matrix gridding does not use the gridding tile
+            for tile in (cand["tiles"][:1] if gm == 3 else cand["tiles"]):
                 try:
                     _apply(plans, tile, 0, 0, gmode=gm)
                     b = BatchedNUFFT(plans, ncoils)
@@ -120,7 +121,9 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
                 best_gc, best_gt = gc, t
         # 2b) hot-tile split threshold
         best_split = 0
-        for sp in cand.get("grid_splits", []):
+        # (for the matrix gridding: the row length above which a whole
+        # workgroup sums one cell)
+        for sp in cand.get("matrix_splits" if best_gm == 3 else "grid_splits", []):
             _apply(plans, best_tile, 0, best_gc, split=sp, gmode=best_gm)
             b = BatchedNUFFT(plans, ncoils)
             t = _time(lambda: b.gridding(b.grid), reps, dev)

@@ -273,12 +273,13 @@ GRID_CASES = [
 ]
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("ndim, n, width, osf, ncoil, split", GRID_CASES)
 def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf, ncoil, split):
-    """Both gridding variants (per-sample LDS atomics over origin-spread
-    entries; register runs over origin-sorted entries) on dense-centre
-    trajectories, split tiles, coil counts below and above the wave count."""
+    """The gridding variants (per-sample LDS atomics over origin-spread
+    entries; register runs over origin-sorted entries; per-cell sums over the
+    sparse gridding matrix) on dense-centre trajectories, split tiles, coil
+    counts below and above the wave count and the coils per lane."""
     if mode == 2 and width * width > 36:
         pytest.skip("register-run gridding holds at most 36 taps per z plane")
     print(f"grid mode={mode} ndim={ndim} N={n} W={width} osf={osf} coils={ncoil} split={split}", flush=True)
@@ -296,6 +297,38 @@ def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf
     torch.cuda.synchronize()
     assert torch.equal(a.view(torch.float32), b.view(torch.float32))
     ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, width, osf), weight=w)
+    assert rel(a.cpu().numpy(), ref) < RTOL
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 8, 16])
+def test_grid_matrix_coil_chunks(chunk):
+    """Matrix gridding with 1..16 coils per lane over 21 coils (partial last
+    items repeat their last coil's loads without storing them) and a density
+    weight, against the oracle."""
+    n = 32
+    print(f"grid matrix chunk={chunk}", flush=True)
+    coord = testing.vd_spiral(8, 200, n)
+    k = cnormal((21,) + coord.shape[:-1], 53)
+    w = np.random.default_rng(54).uniform(0.5, 1.5, coord.shape[:-1]).astype(np.float32)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV, grid_mode=3, grid_chunk=chunk)
+    a = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan, weight=torch.from_numpy(w))
+    ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0), weight=w)
+    assert rel(a.cpu().numpy(), ref) < RTOL
+
+
+def test_grid_matrix_multiframe_plan_and_empty_frame_rows():
+    """One plan of several frames (per-frame row offsets into one matrix),
+    with repeated samples on the grid edge (taps that wrap), against the
+    oracle."""
+    n = 24
+    print("grid matrix multi-frame", flush=True)
+    rng = np.random.default_rng(55)
+    coord = rng.uniform(-n / 2, n / 2, (3, 5, 40, 2)).astype(np.float32)
+    coord[1, 0, :5] = [[-n / 2, -n / 2]] * 5  # a duplicated edge sample
+    k = cnormal((2, 3, 5, 40), 56)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 1.5, device=DEV, grid_mode=3)
+    a = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan)
+    ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 1.5))
     assert rel(a.cpu().numpy(), ref) < RTOL
 
 
