@@ -351,6 +351,8 @@ void destroy(tn_plan *pl) {
     if (pl->d_pbin_off) (void)hipFree(pl->d_pbin_off);
     if (pl->d_mrow) (void)hipFree(pl->d_mrow);
     if (pl->d_mnz) (void)hipFree(pl->d_mnz);
+    for (int a = 0; a < 2; ++a)
+        if (pl->d_fft_tw[a]) (void)hipFree(pl->d_fft_tw[a]);
     for (int d = 0; d < 3; ++d)
         if (pl->d_apod[d]) (void)hipFree(pl->d_apod[d]);
     (void)hipSetDevice(cur);
@@ -462,6 +464,8 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
             return cleanup(fail(TN_E_HIP, "apodization upload failed"));
     }
     rc = build_bins(pl, s);
+    if (rc) return cleanup(rc);
+    rc = build_fft(pl);
     if (rc) return cleanup(rc);
     *out = pl;
     return TN_OK;
@@ -602,6 +606,28 @@ int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg, tn_
     int rc = check_device(plan->device);
     if (rc) return rc;
     return launch_crop_apod(plan, (const float2 *)grid, (float2 *)image, nimg, (hipStream_t)stream);
+}
+
+int tn_plan_has_fft(const tn_plan *plan) { return plan && plan->d_fft_tw[0] ? 1 : 0; }
+
+int tn_fft_forward(tn_plan *plan, const void *image, void *grid, int64_t nimg, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nimg < 0) return fail(TN_E_ARG, "nimg must be >= 0");
+    if (nimg > 0 && (!image || !grid)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    return launch_fft_forward(plan, (const float2 *)image, (float2 *)grid, nimg, (hipStream_t)stream);
+}
+
+int tn_fft_adjoint(tn_plan *plan, void *grid, void *image, int64_t nimg, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nimg < 0) return fail(TN_E_ARG, "nimg must be >= 0");
+    if (nimg > 0 && (!image || !grid)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    return launch_fft_adjoint(plan, (float2 *)grid, (float2 *)image, nimg, (hipStream_t)stream);
 }
 
 int tn_batch_create(tn_plan *const *plans, const int64_t *nbatch, const int64_t *grid_offset,

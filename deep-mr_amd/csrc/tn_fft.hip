@@ -1,0 +1,448 @@
+// Oversampled-grid FFT stages of the 2D NUFFT for gfx950, pruned to the
+// lines that carry data and fused with the zero padding, the apodization
+// and the crop (src/deepmr/fft/nufft.py:499-505 forward: apodize, resize,
+// fft with norm=None; :587-593 adjoint: ifft, resize, apodize).
+//
+// Index maps: image pixel m of an axis sits at grid index (m - N/2) mod G
+// (the centring shifts folded into the placement, as in the other stages),
+// so a grid line along y holds data only at the Ny rows with
+// (y + Ny/2) mod Gy < Ny until the y transform has run, and likewise along x.
+//
+//   forward  columns: image columns -> apodize -> DFT along y -> grid columns
+//                     (only the Nx columns that carry data)
+//            rows:    DFT along x of every grid row, reading only its Nx
+//                     data columns
+//   adjoint  rows:    inverse DFT along x of every grid row, writing only
+//                     the Nx columns the crop keeps
+//            columns: inverse DFT along y of those columns, keeping the Ny
+//                     image rows, scaled by 1 / (Gx Gy), apodized
+//
+// Each line transform is a mixed-radix (8, 4, 2, 3, 5) Stockham FFT run by
+// one wave: the radix-R butterflies of a pass are evaluated in registers
+// with twiddles from a table computed in double precision on the host, and
+// the passes exchange data through that wave's LDS line. All reads of a pass
+// are issued before its writes, and a wave's LDS operations complete in
+// order, so a line is transformed in place without barriers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tn_internal.h"
+
+namespace tn {
+namespace {
+
+constexpr int kFftPad = 8;  // LDS column padding in float2: a column stride of 8 mod 16
+                            // spreads the row-major tile loads over the banks
+
+// Radix of pass p of a length-L transform: radix 8 while possible, then 4,
+// 2, 3 and 5; 0 past the last pass.
+constexpr int fft_radix_at(int L, int p) {
+    const int order[5] = {8, 4, 2, 3, 5};
+    int i = 0;
+    for (int o = 0; o < 5; ++o) {
+        while (L > 1 && L % order[o] == 0) {
+            if (i == p) return order[o];
+            L /= order[o];
+            ++i;
+        }
+    }
+    return 0;
+}
+
+constexpr int fft_npass(int L) {
+    int n = 0;
+    while (fft_radix_at(L, n) != 0) ++n;
+    return n;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (s i) with s = -1 (forward) or +1 (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_j(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// In-register DFTs of length R (sign -1 forward, +1 inverse).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+    const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_j<INV>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a1 = cadd(t1, t3);
+    a2 = csub(t0, t2);
+    a3 = csub(t1, t3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[R]) {
+    if constexpr (R == 2) {
+        const float2 t = v[0];
+        v[0] = cadd(t, v[1]);
+        v[1] = csub(t, v[1]);
+    } else if constexpr (R == 4) {
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4<INV>(e0, e1, e2, e3);
+        dft4<INV>(o0, o1, o2, o3);
+        const float h = 0.70710678118654752f;
+        const float2 w1 = cscale(cadd(o1, mul_j<INV>(o1)), h);   // exp(s i pi/4) o1
+        const float2 w2 = mul_j<INV>(o2);                        // exp(s i pi/2) o2
+        const float2 w3 = cscale(csub(mul_j<INV>(o3), o3), h);   // exp(s 3i pi/4) o3
+        v[0] = cadd(e0, o0);
+        v[4] = csub(e0, o0);
+        v[1] = cadd(e1, w1);
+        v[5] = csub(e1, w1);
+        v[2] = cadd(e2, w2);
+        v[6] = csub(e2, w2);
+        v[3] = cadd(e3, w3);
+        v[7] = csub(e3, w3);
+    } else if constexpr (R == 3) {
+        const float c = -0.5f, sn = 0.86602540378443865f;
+        const float2 t1 = cadd(v[1], v[2]), t2 = cscale(mul_j<INV>(csub(v[1], v[2])), sn);
+        const float2 m = cadd(v[0], cscale(t1, c));
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, t2);
+        v[2] = csub(m, t2);
+    } else if constexpr (R == 5) {
+        const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+        const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+        const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        const float2 m1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+        const float2 m2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+        const float2 n1 = mul_j<INV>(cadd(cscale(t3, s1), cscale(t4, s2)));
+        const float2 n2 = mul_j<INV>(csub(cscale(t3, s2), cscale(t4, s1)));
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+}
+
+// Stockham passes P.. of a length-L line held by one wave. Pass P, radix R,
+// NS = product of the earlier radices: butterfly j < L/R reads elements
+// j + r L/R, multiplies element r by W_{NS R}^{r (j mod NS)}, runs the
+// R-point DFT and stores element r at (j / NS) NS R + (j mod NS) + r NS.
+// Pass 0 reads through `src`, the last pass writes through `dst`, the others
+// use `buf` (the wave's LDS line; src / dst may address it too).
+template <int L, int P, int NS, bool INV, class Src, class Dst>
+__device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ tw, float2 *buf, Src &src, Dst &dst) {
+    constexpr int NP = fft_npass(L);
+    constexpr int R = fft_radix_at(L, P);
+    constexpr int M = L / R;
+    constexpr int Q = (M + kWave - 1) / kWave;
+    constexpr int TS = L / (NS * R);  // twiddle table stride
+    float2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = lane + q * kWave;
+        if (M % kWave == 0 || j < M) {
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (P == 0)
+                    v[q][r] = src(j + r * M);
+                else
+                    v[q][r] = buf[j + r * M];
+            }
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[r * k * TS];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<R, INV>(v[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = lane + q * kWave;
+        if (M % kWave == 0 || j < M) {
+            const int k = j % NS;
+            const int d = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (P == NP - 1)
+                    dst(d + r * NS, v[q][r]);
+                else
+                    buf[d + r * NS] = v[q][r];
+            }
+        }
+    }
+    if constexpr (P + 1 < NP) fft_passes<L, P + 1, NS * R, INV>(lane, tw, buf, src, dst);
+}
+
+template <int L, bool INV, class Src, class Dst>
+__device__ __forceinline__ void fft_line(int lane, const float2 *__restrict__ tw, float2 *buf, Src src, Dst dst) {
+    fft_passes<L, 0, 1, INV>(lane, tw, buf, src, dst);
+}
+
+// data index of pixel-space position i of an axis: kept iff (i + N/2) mod L < N
+__device__ __forceinline__ bool kept(int i, int N, int L) {
+    int m = i + N / 2;
+    if (m >= L) m -= L;
+    return m < N;
+}
+
+// Row transforms along x, in place, one wave per grid row of length L = Gx.
+// Forward: reads only the Nx data columns (the others are zeros). Inverse:
+// writes only the Nx columns the crop keeps.
+template <int L, bool INV>
+__global__ __launch_bounds__(kBlock) void fft_rows_kernel(float2 *__restrict__ data, int64_t nlines, int Nx,
+                                                          const float2 *__restrict__ tw) {
+    __shared__ float2 lds[kBlock / kWave][L];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t line = (int64_t)blockIdx.x * (kBlock / kWave) + wave;
+    if (line >= nlines) return;  // no barriers below
+    float2 *row = data + line * L;
+    if (!INV) {
+        fft_line<L, false>(
+            lane, tw, lds[wave], [&](int i) { return kept(i, Nx, L) ? row[i] : make_float2(0.0f, 0.0f); },
+            [&](int i, float2 v) { row[i] = v; });
+    } else {
+        fft_line<L, true>(
+            lane, tw, lds[wave], [&](int i) { return row[i]; },
+            [&](int i, float2 v) {
+                if (kept(i, Nx, L)) row[i] = v;
+            });
+    }
+}
+
+// Block of C adjacent image columns of one image, dealt to the XCDs in
+// contiguous ranges (neighbouring column groups share cache lines).
+__device__ __forceinline__ bool column_block(int64_t nimg, int Nx, int C, int64_t &b, int &mx0) {
+    const int ngrp = (Nx + C - 1) / C;
+    const int64_t nblk = nimg * ngrp;
+    const int64_t per = (nblk + kXcds - 1) / kXcds;
+    const int64_t blk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (blk >= nblk) return false;
+    b = blk / ngrp;
+    mx0 = (int)(blk - b * ngrp) * C;
+    return true;
+}
+
+// Forward column stage: image columns mx0 .. mx0 + C - 1 of image b are
+// apodized (x then y, as the separate pre-pass does), placed at their grid
+// rows in LDS, transformed along y (length L = Gy; rows without data are
+// zeros) and written to grid columns (mx - Nx/2) mod Gx.
+template <int L, int C>
+__global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__restrict__ img,
+                                                              float2 *__restrict__ grid, int64_t nimg, int Nx,
+                                                              int Ny, int Gx, const float *__restrict__ apx,
+                                                              const float *__restrict__ apy,
+                                                              const float2 *__restrict__ tw) {
+    constexpr int LP = L + kFftPad;
+    __shared__ float2 lds[C * LP];
+    int64_t b;
+    int mx0;
+    if (!column_block(nimg, Nx, C, b, mx0)) return;
+    const int hy = Ny / 2, hx = Nx / 2;
+    for (int i = threadIdx.x; i < Ny * C; i += kBlock) {
+        const int my = i / C, c = i % C;
+        const int mx = mx0 + c;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (mx < Nx) {
+            v = img[(b * Ny + my) * (int64_t)Nx + mx];
+            if (apx) v = cscale(v, apx[mx]);
+            if (apy) v = cscale(v, apy[my]);
+        }
+        int y = my - hy;
+        if (y < 0) y += L;
+        lds[c * LP + y] = v;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    for (int c = wave; c < C; c += kBlock / kWave) {
+        float2 *col = lds + c * LP;
+        fft_line<L, false>(
+            lane, tw, col, [&](int i) { return kept(i, Ny, L) ? col[i] : make_float2(0.0f, 0.0f); },
+            [&](int i, float2 v) { col[i] = v; });
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < L * C; i += kBlock) {
+        const int y = i / C, c = i % C;
+        const int mx = mx0 + c;
+        if (mx < Nx) {
+            int x = mx - hx;
+            if (x < 0) x += Gx;
+            grid[(b * L + y) * (int64_t)Gx + x] = lds[c * LP + y];
+        }
+    }
+}
+
+// Adjoint column stage: grid columns (mx - Nx/2) mod Gx for image columns
+// mx0 .. mx0 + C - 1, inverse-transformed along y (L = Gy); the Ny image rows
+// are scaled by 1 / (Gx Gy), apodized (x then y) and written to the image.
+template <int L, int C>
+__global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__restrict__ grid,
+                                                              float2 *__restrict__ img, int64_t nimg, int Nx,
+                                                              int Ny, int Gx, const float *__restrict__ apx,
+                                                              const float *__restrict__ apy,
+                                                              const float2 *__restrict__ tw, float scale) {
+    constexpr int LP = L + kFftPad;
+    __shared__ float2 lds[C * LP];
+    int64_t b;
+    int mx0;
+    if (!column_block(nimg, Nx, C, b, mx0)) return;
+    const int hy = Ny / 2, hx = Nx / 2;
+    for (int i = threadIdx.x; i < L * C; i += kBlock) {
+        const int y = i / C, c = i % C;
+        const int mx = mx0 + c;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (mx < Nx) {
+            int x = mx - hx;
+            if (x < 0) x += Gx;
+            v = grid[(b * L + y) * (int64_t)Gx + x];
+        }
+        lds[c * LP + y] = v;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    for (int c = wave; c < C; c += kBlock / kWave) {
+        float2 *col = lds + c * LP;
+        fft_line<L, true>(
+            lane, tw, col, [&](int i) { return col[i]; }, [&](int i, float2 v) { col[i] = v; });
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < Ny * C; i += kBlock) {
+        const int my = i / C, c = i % C;
+        const int mx = mx0 + c;
+        if (mx < Nx) {
+            int y = my - hy;
+            if (y < 0) y += L;
+            float2 v = cscale(lds[c * LP + y], scale);
+            if (apx) v = cscale(v, apx[mx]);
+            if (apy) v = cscale(v, apy[my]);
+            img[(b * Ny + my) * (int64_t)Nx + mx] = v;
+        }
+    }
+}
+
+constexpr int col_group(int L) { return L <= 256 ? 16 : L <= 512 ? 8 : 4; }
+
+using RowFn = void (*)(float2 *, int64_t, int, const float2 *);
+using ColFwdFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, const float *, const float *,
+                          const float2 *);
+using ColInvFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, const float *, const float *,
+                          const float2 *, float);
+
+struct FftFns {
+    int L = 0;
+    RowFn row_fwd = nullptr, row_inv = nullptr;
+    ColFwdFn col_fwd = nullptr;
+    ColInvFn col_inv = nullptr;
+    int C = 0;
+};
+
+template <int L>
+FftFns fns_for() {
+    FftFns f;
+    f.L = L;
+    f.C = col_group(L);
+    f.row_fwd = fft_rows_kernel<L, false>;
+    f.row_inv = fft_rows_kernel<L, true>;
+    f.col_fwd = fft_cols_fwd_kernel<L, col_group(L)>;
+    f.col_inv = fft_cols_inv_kernel<L, col_group(L)>;
+    return f;
+}
+
+// Transform lengths with kernels: 2^a 3^b 5^c from 64 to 1024 (the common
+// oversampled sizes of 64..512-pixel images at oversampling 1.25 .. 2).
+FftFns fft_fns(int L) {
+    switch (L) {
+        case 64: return fns_for<64>();
+        case 80: return fns_for<80>();
+        case 96: return fns_for<96>();
+        case 128: return fns_for<128>();
+        case 160: return fns_for<160>();
+        case 192: return fns_for<192>();
+        case 256: return fns_for<256>();
+        case 320: return fns_for<320>();
+        case 384: return fns_for<384>();
+        case 512: return fns_for<512>();
+        case 640: return fns_for<640>();
+        case 768: return fns_for<768>();
+        case 1024: return fns_for<1024>();
+        default: return FftFns{};
+    }
+}
+
+unsigned col_blocks(int64_t nimg, int Nx, int C) {
+    const int64_t n = nimg * ((Nx + C - 1) / C);
+    return (unsigned)((n + kXcds - 1) / kXcds * kXcds);
+}
+
+}  // namespace
+
+bool fft_supported(const tn_plan *plan) {
+    const tn_plan_params &p = plan->p;
+    if (p.ndim != 2) return false;
+    return fft_fns(p.grid[2]).L != 0 && fft_fns(p.grid[1]).L != 0;
+}
+
+int build_fft(tn_plan *plan) {
+    if (!fft_supported(plan)) return TN_OK;
+    for (int a = 0; a < 2; ++a) {  // 0: x (grid[2]), 1: y (grid[1])
+        const int L = plan->p.grid[2 - a];
+        std::vector<float2> tw(L);
+        const double pi = 3.14159265358979323846;
+        for (int m = 0; m < L; ++m) {
+            const double ang = -2.0 * pi * (double)m / (double)L;
+            tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        TN_HIP(hipMalloc(&plan->d_fft_tw[a], sizeof(float2) * L));
+        TN_HIP(hipMemcpy(plan->d_fft_tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+    }
+    return TN_OK;
+}
+
+int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, int64_t nimg, hipStream_t stream) {
+    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D, lengths 2^a 3^b 5^c in 64..1024)");
+    if (nimg == 0) return TN_OK;
+    const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
+    const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
+    const FftFns fy = fft_fns(Gy), fx = fft_fns(Gx);
+    hipLaunchKernelGGL(fy.col_fwd, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), 0, stream, image, grid, nimg, Nx,
+                       Ny, Gx, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
+                       (const float2 *)plan->d_fft_tw[1]);
+    TN_HIP(hipGetLastError());
+    const int64_t nlines = nimg * Gy;
+    const int64_t rb = (nlines + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL(fx.row_fwd, dim3((unsigned)rb), dim3(kBlock), 0, stream, grid, nlines, Nx,
+                       (const float2 *)plan->d_fft_tw[0]);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t nimg, hipStream_t stream) {
+    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D, lengths 2^a 3^b 5^c in 64..1024)");
+    if (nimg == 0) return TN_OK;
+    const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
+    const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
+    const FftFns fy = fft_fns(Gy), fx = fft_fns(Gx);
+    const int64_t nlines = nimg * Gy;
+    const int64_t rb = (nlines + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL(fx.row_inv, dim3((unsigned)rb), dim3(kBlock), 0, stream, grid, nlines, Nx,
+                       (const float2 *)plan->d_fft_tw[0]);
+    TN_HIP(hipGetLastError());
+    const float scale = (float)(1.0 / ((double)Gx * (double)Gy));
+    hipLaunchKernelGGL(fy.col_inv, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), 0, stream, grid, image, nimg, Nx,
+                       Ny, Gx, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
+                       (const float2 *)plan->d_fft_tw[1], scale);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+}  // namespace tn

@@ -162,6 +162,7 @@ struct tn_plan {
     int grid_mode = tn::kGridSpread;  // effective gridding variant (entry order)
     int32_t *d_mrow = nullptr;  // matrix gridding: row offsets, (nframes * cells + 1)
     int2 *d_mnz = nullptr;      // matrix gridding: (sample, weight bits) sorted by (frame, cell)
+    float2 *d_fft_tw[2] = {nullptr, nullptr};  // pruned FFT twiddles exp(-2 pi i m / G), (x, y); null: no pruned FFT
     int64_t mnnz = 0;
     std::map<int64_t, tn::LaunchTables> cache;
     std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank)
@@ -198,6 +199,11 @@ int launch_crop_apod(const tn_plan *plan, const float2 *grid, float2 *image,
                      int64_t nimg, hipStream_t stream);
 int build_bins(tn_plan *plan, hipStream_t stream);
 int build_matrix(tn_plan *plan, hipStream_t stream);
+// Pruned, fused oversampled-grid FFT stages (2D plans with supported lengths).
+bool fft_supported(const tn_plan *plan);
+int build_fft(tn_plan *plan);
+int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, int64_t nimg, hipStream_t stream);
+int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t nimg, hipStream_t stream);
 int count_footprint(tn_plan *plan, hipStream_t stream);
 
 // Variant selection shared by the table builder and the launchers.

@@ -267,6 +267,25 @@ class NativePlan:
                                             stream_ptr(self.device)), "tn_crop_apod")
         return out
 
+    @property
+    def has_fft(self):
+        """True when the plan has the pruned, fused FFT stages (2D, grid
+        lengths 2^a 3^b 5^c in 64..1024)."""
+        return bool(_lib.load().tn_plan_has_fft(self.handle))
+
+    def fft_forward(self, image, grid, nimg):
+        """image (nimg, *N) -> grid (nimg, *G): apodize, pad and forward DFT."""
+        _lib.check(_lib.load().tn_fft_forward(self.handle, ptr(image), ptr(grid), int(nimg),
+                                              stream_ptr(self.device)), "tn_fft_forward")
+        return grid
+
+    def fft_adjoint(self, grid, image, nimg):
+        """grid (nimg, *G) -> image (nimg, *N): inverse DFT (1 / prod(G)),
+        crop and apodize; ``grid`` is overwritten."""
+        _lib.check(_lib.load().tn_fft_adjoint(self.handle, ptr(grid), ptr(image), int(nimg),
+                                              stream_ptr(self.device)), "tn_fft_adjoint")
+        return image
+
 
 def _destroy(handle, device):
     try:

@@ -57,6 +57,9 @@ _SIGNATURES = {
                                   c_void_p, c_void_p]),
     "tn_apod_pad": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "tn_crop_apod": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "tn_plan_has_fft": (c_int32, [c_void_p]),
+    "tn_fft_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "tn_fft_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "tn_batch_create": (c_int32, [ctypes.POINTER(c_void_p), ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
                                   ctypes.POINTER(c_int64), ctypes.POINTER(c_int64), c_int32,
                                   ctypes.POINTER(c_void_p)]),

@@ -107,16 +107,23 @@ class BatchedNUFFT:
     def forward(self):
         """image arena -> k-space arena (deepmr apply_nufft per plan)."""
         native0 = self.plans[0].interpolator
-        native0.apod_pad(self.image, self.grid, self.nimg)
-        k = torch.fft.fftn(self.grid, dim=tuple(range(-self.ndim, 0)))
+        if native0.has_fft:
+            k = native0.fft_forward(self.image, self.grid, self.nimg)
+        else:
+            native0.apod_pad(self.image, self.grid, self.nimg)
+            k = torch.fft.fftn(self.grid, dim=tuple(range(-self.ndim, 0)))
         self.interp(k)
         return self.kspace
 
     def adjoint(self):
         """k-space arena -> adjoint image arena (deepmr apply_nufft_adj per plan)."""
         self.gridding(self.grid)
-        img = torch.fft.ifftn(self.grid, dim=tuple(range(-self.ndim, 0)))
-        self.plans[0].interpolator.crop_apod(img, self.image_adj, self.nimg)
+        native0 = self.plans[0].interpolator
+        if native0.has_fft:
+            native0.fft_adjoint(self.grid, self.image_adj, self.nimg)
+        else:
+            img = torch.fft.ifftn(self.grid, dim=tuple(range(-self.ndim, 0)))
+            native0.crop_apod(img, self.image_adj, self.nimg)
         return self.image_adj
 
     def interp(self, grid):

@@ -5,8 +5,10 @@ Mirrors ``src/deepmr/fft/nufft.py``: ``plan_nufft``, ``apply_nufft``,
 argument meaning, tensor layouts, dtype casts and numpy-in / numpy-out
 behaviour. The compute chain per direction is three device steps:
 
-  forward  tn_apod_pad -> torch.fft.fftn (rocFFT) -> tn_interp
-  adjoint  tn_grid -> torch.fft.ifftn (rocFFT) -> tn_crop_apod
+  forward  tn_fft_forward (apodize, pad, pruned 2D DFT) -> tn_interp
+  adjoint  tn_grid -> tn_fft_adjoint (pruned inverse 2D DFT, crop, apodize)
+  (plans without the pruned FFT stages, e.g. 3D: tn_apod_pad / tn_crop_apod
+  around torch.fft on the device)
 
 The centring shifts of deepmr's ``fft``/``ifft`` and the zero-pad / crop are
 folded into the index maps of the native kernels, and the ``1/scale`` of the
@@ -399,8 +401,11 @@ def _apply_nufft(image, nufft_plan, basis_adjoint, weight, device, threadsperblo
     native = plan.interpolator
     with torch.cuda.device(dev):
         grid = torch.empty((nimg, *plan.os_shape), dtype=torch.complex64, device=dev)
-        native.apod_pad(x, grid, nimg)
-        k = torch.fft.fftn(grid, dim=tuple(range(-ndim, 0)))
+        if native.has_fft:
+            k = native.fft_forward(x, grid, nimg)
+        else:
+            native.apod_pad(x, grid, nimg)
+            k = torch.fft.fftn(grid, dim=tuple(range(-ndim, 0)))
         del grid
         ksp_shape = (*batch_shape, nframes, *plan.ishape) if nframes > 1 else (*batch_shape, *plan.ishape)
         w_fused, w_bcast = _fused_weight(weight, plan, dev, (nframes, *plan.ishape) if nframes > 1 else plan.ishape)
@@ -466,11 +471,15 @@ def _apply_nufft_adj(kspace, nufft_plan, basis, weight, device, threadsperblock)
     grid, batch_shape, nimgax = _grid_step(y, plan, basis, weight, dev)
     nbatch = int(np.prod(batch_shape))
     with torch.cuda.device(dev):
-        img_g = torch.fft.ifftn(grid, dim=tuple(range(-ndim, 0)))
-        del grid
         nimg = nbatch * nimgax
         img = torch.empty((nimg, *plan.shape), dtype=torch.complex64, device=dev)
-        plan.interpolator.crop_apod(img_g, img, nimg)
+        if plan.interpolator.has_fft:
+            plan.interpolator.fft_adjoint(grid, img, nimg)
+        else:
+            img_g = torch.fft.ifftn(grid, dim=tuple(range(-ndim, 0)))
+            plan.interpolator.crop_apod(img_g, img, nimg)
+            del img_g
+        del grid
         shape = (*batch_shape, nimgax, *plan.shape) if nframes > 1 else (*batch_shape, *plan.shape)
         img = img.reshape(shape)
     img = img.to(odevice)

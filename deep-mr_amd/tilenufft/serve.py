@@ -325,9 +325,13 @@ class ReconServer:
                     g = grp["g"]
                     if grp["any_cg"]:
                         torch.mul(grp["smaps"], grp["P"].index_select(0, grp["row2req"]), out=grp["images"])
-                    ga = lay["gridA"][a:b].view(-1, g, g)
-                    grp["members"][0].plan.interpolator.apod_pad(grp["images"], ga, grp["rows"])
-                    torch.fft.fftn(ga, dim=(-2, -1), out=lay["gridB"][a:b].view(-1, g, g))
+                    nat = grp["members"][0].plan.interpolator
+                    if nat.has_fft:
+                        nat.fft_forward(grp["images"], lay["gridB"][a:b].view(-1, g, g), grp["rows"])
+                    else:
+                        ga = lay["gridA"][a:b].view(-1, g, g)
+                        nat.apod_pad(grp["images"], ga, grp["rows"])
+                        torch.fft.fftn(ga, dim=(-2, -1), out=lay["gridB"][a:b].view(-1, g, g))
                 lay["interp_batch"].interp(lay["gridB"], lay["kspace"])
             # measured k-space of requests starting this step
             for grp in lay["groups"]:
@@ -341,9 +345,13 @@ class ReconServer:
             for grp in lay["groups"]:
                 a, b = grp["gslice"]
                 g = grp["g"]
-                gb = lay["gridB"][a:b].view(-1, g, g)
-                torch.fft.ifftn(lay["gridA"][a:b].view(-1, g, g), dim=(-2, -1), out=gb)
-                grp["members"][0].plan.interpolator.crop_apod(gb, grp["images"], grp["rows"])
+                nat = grp["members"][0].plan.interpolator
+                if nat.has_fft:  # gridA is scratch from here on
+                    nat.fft_adjoint(lay["gridA"][a:b].view(-1, g, g), grp["images"], grp["rows"])
+                else:
+                    gb = lay["gridB"][a:b].view(-1, g, g)
+                    torch.fft.ifftn(lay["gridA"][a:b].view(-1, g, g), dim=(-2, -1), out=gb)
+                    nat.crop_apod(gb, grp["images"], grp["rows"])
                 self._cg_update(grp)
             finished = []
             for grp in lay["groups"]:

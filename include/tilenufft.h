@@ -159,6 +159,28 @@ int tn_apod_pad(tn_plan *plan, const void *image, void *grid, int64_t nimg,
 int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg,
                  tn_stream stream);
 
+/* 1 when the plan has the pruned, fused oversampled FFT stages below
+ * (2D plans whose grid lengths are 2^a 3^b 5^c between 64 and 1024), else 0. */
+int tn_plan_has_fft(const tn_plan *plan);
+
+/* Whole forward FFT stage of the NUFFT on the device: apodization, centred
+ * zero padding, ifftshift and the unnormalised 2D DFT, image (nimg, Ny, Nx)
+ * -> grid (nimg, Gy, Gx) in the layout tn_interp reads (every cell written).
+ * Transforms only the Nx image columns along y, then every row along x
+ * from its Nx data columns. Same result as tn_apod_pad followed by an
+ * unnormalised FFT. Replaces _apodize + resize + fft.fft(norm=None)
+ * (src/deepmr/fft/nufft.py:499-505). TN_E_ARG when tn_plan_has_fft is 0. */
+int tn_fft_forward(tn_plan *plan, const void *image, void *grid, int64_t nimg,
+                   tn_stream stream);
+
+/* Whole adjoint FFT stage: inverse 2D DFT scaled by 1 / (Gx Gy), fftshift,
+ * centred crop and apodization, grid (nimg, Gy, Gx) -> image (nimg, Ny, Nx).
+ * The grid is used as scratch (overwritten). Same result as an inverse FFT
+ * followed by tn_crop_apod. Replaces fft.ifft(norm=None) + resize + _apodize
+ * (src/deepmr/fft/nufft.py:587-593). TN_E_ARG when tn_plan_has_fft is 0. */
+int tn_fft_adjoint(tn_plan *plan, void *grid, void *image, int64_t nimg,
+                   tn_stream stream);
+
 /* ---- batched launches over several plans (ragged work queue) ----------
  * A batch binds `njobs` (plan, nbatch, grid offset, sample offset) tuples to
  * single launches: one interpolation launch and one gridding launch cover
