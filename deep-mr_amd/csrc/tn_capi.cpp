@@ -59,7 +59,7 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.pbin_off = plan->d_pbin_off + (int64_t)f * plan->nitiles_total;
         j.mrow = plan->d_mrow ? plan->d_mrow + (int64_t)f * gvol : nullptr;
         j.mnz = plan->d_mnz;
-        j.mlong = p.grid_split == 0 ? kMatrixLongRow : p.grid_split < 0 ? INT32_MAX : p.grid_split;
+        j.mlong = p.grid_split == 0 ? kMatrixLongRow : p.grid_split < 0 ? kMatrixStage : std::min(p.grid_split, kMatrixStage);
         // deepmr layout (batch, nframes, ...): frame f of coil b
         j.grid_off = grid_off + (int64_t)f * gvol;
         j.grid_bstride = (int64_t)p.nframes * gvol;
@@ -192,27 +192,50 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                     interp.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + ic), 0, 0, 0, 0});
         }
         if (gmode == kGridMatrix) {
-            // rows longer than mlong: one workgroup each (part 1, unit = cell),
-            // queued first; then blocks of kBlock cells, one lane per cell
-            // (part 0, unit = block), which skip the long rows
+            // Items are runs of consecutive cells of one frame (unit = first
+            // cell, e1 = cell count | entry count << 16, slot = first entry,
+            // e0 = coil chunk): at most kBlock / cw * kMatrixPasses cells
+            // (kBlock for the lane-per-cell kernel) and at most kMatrixStage
+            // entries, so that an item's entries (contiguous in the CSR order)
+            // fit the kernel's LDS stage. Rows longer than mlong end the run and
+            // get a workgroup of their own (part 1, unit = cell); those are
+            // queued first.
             const int mc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, kMatrixChunk), kMatrixCoils);
             max_gc = std::max(max_gc, mc);
+            const int cw = mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16;
+            const int cpi = mc >= 2 ? std::min(kBlock, kBlock / cw * kMatrixPasses) : kBlock;
             const int64_t ncell = (int64_t)j.G[0] * j.G[1] * j.G[2];
             const int64_t f = (j.mrow - pl->d_mrow) / std::max<int64_t>(ncell, 1);
             const int32_t *row = pl->h_mrow.data() + f * ncell;
-            for (int64_t c = 0; c < ncell; ++c)
-                if (row[c + 1] - row[c] > j.mlong)
+            std::vector<std::pair<int64_t, int>> runs;
+            int64_t start = 0;
+            int ncells = 0, nent = 0;
+            auto close = [&]() {
+                if (ncells > 0) runs.emplace_back(start, ncells);
+                ncells = 0;
+                nent = 0;
+            };
+            for (int64_t c = 0; c < ncell; ++c) {
+                const int len = row[c + 1] - row[c];
+                if (len > j.mlong) {
+                    close();
                     for (int c0 = 0; c0 < j.nbatch; c0 += mc)
                         grid.push_back(Item{(int)ji, (int)c, c0, std::min(j.nbatch, c0 + mc), c0 / mc, 0, 1, -1});
-            // cells per item: kBlock for the lane-per-cell kernel; kMatrixPasses
-            // passes of kBlock / cw cells for the coil-minor kernel
-            const int cw = mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16;
-            const int64_t bcells = mc >= 2 ? (int64_t)kBlock / cw * kMatrixPasses : kBlock;
-            const int64_t nblk = (ncell + bcells - 1) / bcells;
-            for (int64_t b = 0; b < nblk; ++b)
+                    continue;
+                }
+                if (ncells == cpi || nent + len > kMatrixStage) close();
+                if (ncells == 0) start = c;
+                ++ncells;
+                nent += len;
+            }
+            close();
+            for (const auto &r : runs) {
+                const int32_t e0 = row[r.first], ne = row[r.first + r.second] - e0;
                 for (int c0 = 0; c0 < j.nbatch; c0 += mc)
-                    grid.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + mc), c0 / mc, (int)bcells, 0, -1});
-            // coil-minor sample copy: chunks of mc coils padded to st_cw lanes
+                    grid.push_back(Item{(int)ji, (int)r.first, c0, std::min(j.nbatch, c0 + mc), c0 / mc,
+                                        r.second | (ne << 16), 0, e0});
+            }
+            // coil-minor sample copy: chunks of mc coils padded to cw lanes
             if (mc >= 2) {
                 st_cw = std::max(st_cw, cw);
                 const int64_t pblk = (j.npts + kBlock - 1) / kBlock;

@@ -824,9 +824,9 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
 // (CSR), samples ascending inside a cell. Two kinds of work item, both
 // summing weight * scaled sample for up to CM coils in registers and writing
 // each cell of every coil once (zeros included) with streaming stores:
-//   part 0: a block of it.e1 <= kBlock cells, one lane per cell, walking its
-//           row with the next entry prefetched; rows longer than J.mlong are
-//           skipped;
+//   part 0: a run of (it.e1 & 0xffff) <= kBlock consecutive cells from
+//           it.unit, one lane per cell, walking its row with the next entry
+//           prefetched; rows longer than J.mlong are skipped;
 //   part 1: one such long row (dense trajectory centres hold hundreds of
 //           entries per cell), its entries strided over the workgroup and
 //           the partial sums reduced by a fixed butterfly per wave, then
@@ -889,10 +889,9 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restri
         }
     };
     const i32x2 *mnz = (const i32x2 *)J.mnz;
-    if (it.part == 0) {  // it.e1 (<= kBlock) cells per item
-        const int64_t ncell = (int64_t)J.G[0] * J.G[1] * J.G[2];
-        const int64_t cell = (int64_t)it.unit * it.e1 + threadIdx.x;
-        if ((int)threadIdx.x >= it.e1 || cell >= ncell) return;
+    if (it.part == 0) {  // cells it.unit .. it.unit + (it.e1 & 0xffff) - 1 (<= kBlock)
+        const int64_t cell = (int64_t)it.unit + threadIdx.x;
+        if ((int)threadIdx.x >= (it.e1 & 0xffff)) return;
         const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
         if (r1 - r0 > J.mlong) return;  // written by its long-row item
         if (r0 < r1) {
@@ -989,11 +988,14 @@ __global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__res
 // Matrix gridding over the coil-minor sample copy: one lane per (cell,
 // coil), the CW lanes of a cell reading one contiguous CW-coil record per
 // entry (one cache line for 16 coils instead of one line per coil). Items
-// as for grid_matrix_kernel (it.e0 = chunk index): part 0 walks a block of
-// it.e1 cells, kBlock / CW cells per pass, each row four entries at a time
-// (independent loads in flight); part 1 strides one long row's entries over
-// the cell groups of the workgroup and reduces the group sums with a fixed
-// butterfly per wave, then across the waves in wave order.
+// as for grid_matrix_kernel (it.e0 = chunk index). Part 0, a run of
+// (it.e1 & 0xffff) cells from it.unit whose entries (it.e1 >> 16 of them,
+// at most kMatrixStage, contiguous in the CSR order from it.slot) and row
+// offsets are first staged in LDS with independent coalesced loads, so that
+// the only dependent global loads left are the sample records; kBlock / CW
+// cells per pass, each row four entries at a time. Part 1 strides one long row's entries over the cell groups of the
+// workgroup and reduces the group sums with a fixed butterfly per wave,
+// then across the waves in wave order.
 template <int LCW>
 __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,
@@ -1003,6 +1005,8 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     constexpr int CPP = kBlock / CW;  // cells per pass
     constexpr int U = 4;              // entries in flight per lane
     __shared__ float2 red[kBlock / kWave][CW];
+    __shared__ i32x2 sent[kMatrixStage];
+    __shared__ int srow[kBlock + 1];
     const int64_t per = (nitems + kXcds - 1) / kXcds;
     const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     if (idx >= nitems) return;
@@ -1013,13 +1017,16 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     const bool store = it.c0 + j < it.c1;
     float2 *gout = grid + J.grid_off + (int64_t)(it.c0 + (store ? j : 0)) * J.grid_bstride;
     const i32x2 *mnz = (const i32x2 *)J.mnz;
-    // entries e0, e0 + stride, ... below r1 of one row, U at a time
-    auto row_sum = [&](int e0, int r1, int stride, float &re, float &im) {
+    // entries e0, e0 + stride, ... below r1 of one row (from `ent`), U at a time
+    auto row_sum = [&](const i32x2 *ent, bool global, int e0, int r1, int stride, float &re, float &im) {
         for (int e = e0; e < r1; e += U * stride) {
             i32x2 m[U];
             float2 v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) m[u] = __builtin_nontemporal_load(mnz + min(e + u * stride, r1 - 1));
+            for (int u = 0; u < U; ++u) {
+                const int ei = min(e + u * stride, r1 - 1);
+                m[u] = global ? __builtin_nontemporal_load(ent + ei) : ent[ei];
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = sq[(int64_t)m[u].x * CW];
 #pragma unroll
@@ -1031,18 +1038,19 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
         }
     };
     if (it.part == 0) {
-        const int64_t ncell = (int64_t)J.G[0] * J.G[1] * J.G[2];
-        const int npass = it.e1 / CPP;
-        for (int pass = 0; pass < npass; ++pass) {
-            const int64_t cell = (int64_t)it.unit * it.e1 + pass * CPP + g;
-            if (cell >= ncell) break;
-            const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
-            if (r1 - r0 > J.mlong) continue;  // written by its long-row item
+        const int64_t cell0 = it.unit;
+        const int ncells = min(it.e1 & 0xffff, kBlock);
+        const int rb = it.slot;
+        const int nent = min(it.e1 >> 16, kMatrixStage);  // <= kMatrixStage by construction
+        for (int i = threadIdx.x; i <= ncells; i += kBlock) srow[i] = min(max(J.mrow[cell0 + i] - rb, 0), nent);
+        for (int i = threadIdx.x; i < nent; i += kBlock) sent[i] = __builtin_nontemporal_load(mnz + rb + i);
+        __syncthreads();
+        for (int c = g; c < ncells; c += CPP) {
             float re = 0.0f, im = 0.0f;
-            row_sum(r0, r1, 1, re, im);
+            row_sum(sent, false, srow[c], srow[c + 1], 1, re, im);
             if (store) {
                 const f32x2 v = {re, im};
-                __builtin_nontemporal_store(v, (f32x2 *)(gout + cell));
+                __builtin_nontemporal_store(v, (f32x2 *)(gout + cell0 + c));
             }
         }
         return;
@@ -1050,7 +1058,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     const int64_t cell = it.unit;
     const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
     float re = 0.0f, im = 0.0f;
-    row_sum(r0 + g, r1, CPP, re, im);
+    row_sum(mnz, true, r0 + g, r1, CPP, re, im);
 #pragma unroll
     for (int d = CW; d < kWave; d <<= 1) {
         re += __shfl_xor(re, d, kWave);
