@@ -217,10 +217,20 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             };
             for (int64_t c = 0; c < ncell; ++c) {
                 const int len = row[c + 1] - row[c];
-                if (len > j.mlong) {
+                if (len > j.mlong) {  // parts of kMatrixPart entries; several parts leave records to fold
                     close();
-                    for (int c0 = 0; c0 < j.nbatch; c0 += mc)
-                        grid.push_back(Item{(int)ji, (int)c, c0, std::min(j.nbatch, c0 + mc), c0 / mc, 0, 1, -1});
+                    const int nparts = (len + kMatrixPart - 1) / kMatrixPart;
+                    for (int c0 = 0; c0 < j.nbatch; c0 += mc) {
+                        const int c1 = std::min(j.nbatch, c0 + mc);
+                        if (nparts == 1) {
+                            grid.push_back(Item{(int)ji, (int)c, c0, c1, c0 / mc, 0, 1, -1});
+                            continue;
+                        }
+                        const int32_t rec = (int32_t)(scratch / (2 * kMatrixCoils));
+                        scratch += (int64_t)nparts * 2 * kMatrixCoils;
+                        for (int k = 0; k < nparts; ++k) grid.push_back(Item{(int)ji, (int)c, c0, c1, c0 / mc, k, 1, rec});
+                        fold.push_back(Item{(int)ji, (int)c, c0, c1, c0 / mc, nparts, 0, rec});
+                    }
                     continue;
                 }
                 if (ncells == cpi || nent + len > kMatrixStage) close();

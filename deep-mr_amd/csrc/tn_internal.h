@@ -35,6 +35,8 @@ constexpr int kMatrixLongRow = 128;  // default: rows with more entries get a wh
 constexpr int kMatrixChunk = 8;      // default coils per item (record width of the coil-minor copy)
 constexpr int kMatrixPasses = 4;     // cell passes per item of the coil-minor matrix gridding
 constexpr int kMatrixStage = 2048;   // entries per item staged in LDS (also the longest short row)
+constexpr int kMatrixPart = 2048;    // entries per workgroup of a long row; longer rows are summed
+                                     // in parts (kMatrixCoils-wide partial records, folded in order)
 constexpr int kXcds = 8;          // accelerator complex dies (each with its own L2)
 constexpr int kRunsMaxTaps = 36;  // register budget of the run sums of one z plane (W <= 6)
 constexpr int kRunsBatch = 256;                 // entries per staged batch (= kBlock)

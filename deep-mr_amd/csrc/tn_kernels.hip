@@ -827,10 +827,13 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
 //   part 0: a run of (it.e1 & 0xffff) <= kBlock consecutive cells from
 //           it.unit, one lane per cell, walking its row with the next entry
 //           prefetched; rows longer than J.mlong are skipped;
-//   part 1: one such long row (dense trajectory centres hold hundreds of
-//           entries per cell), its entries strided over the workgroup and
-//           the partial sums reduced by a fixed butterfly per wave, then
-//           across the waves in wave order.
+//   part 1: entries [it.e1 * kMatrixPart, + kMatrixPart) of one such long
+//           row (dense trajectory centres hold hundreds to 10^5 entries per
+//           cell), strided over the workgroup and reduced by a fixed
+//           butterfly per wave, then across the waves in wave order; a row
+//           of several parts (it.slot >= 0) leaves partial record
+//           it.slot + it.e1 for matrix_fold_kernel, else the cell is
+//           written.
 // No atomics, and a fixed summation order for every cell. Long rows are
 // queued first; items are dealt to the XCDs in contiguous ranges (the
 // hardware deals workgroups out round-robin), so the neighbouring rows of
@@ -842,7 +845,8 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restri
                                                              const Item *__restrict__ items, int64_t nitems,
                                                              const float2 *__restrict__ samp,
                                                              float2 *__restrict__ grid,
-                                                             const float *__restrict__ weight, LowRank lr) {
+                                                             const float *__restrict__ weight, LowRank lr,
+                                                             float2 *__restrict__ parts) {
     __shared__ float red[kBlock / kWave][2 * CM];
     const int64_t per = (nitems + kXcds - 1) / kXcds;
     const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
@@ -913,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restri
         return;
     }
     const int64_t cell = it.unit;
-    const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+    const int r0 = J.mrow[cell] + it.e1 * kMatrixPart, r1 = min(J.mrow[cell + 1], r0 + kMatrixPart);
     for (int e = r0 + (int)threadIdx.x; e < r1; e += kBlock) add_entry(__builtin_nontemporal_load(mnz + e));
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 #pragma unroll
@@ -940,7 +944,10 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restri
             re += red[w][2 * c];
             im += red[w][2 * c + 1];
         }
-        grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + cell] = make_float2(re, im);
+        if (it.slot < 0)
+            grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + cell] = make_float2(re, im);
+        else
+            parts[((int64_t)it.slot + it.e1) * kMatrixCoils + c] = make_float2(re, im);
     }
 }
 
@@ -1000,7 +1007,8 @@ template <int LCW>
 __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,
                                                                const float2 *__restrict__ st,
-                                                               float2 *__restrict__ grid) {
+                                                               float2 *__restrict__ grid,
+                                                               float2 *__restrict__ parts) {
     constexpr int CW = 1 << LCW;
     constexpr int CPP = kBlock / CW;  // cells per pass
     constexpr int U = 4;              // entries in flight per lane
@@ -1056,7 +1064,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
         return;
     }
     const int64_t cell = it.unit;
-    const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+    const int r0 = J.mrow[cell] + it.e1 * kMatrixPart, r1 = min(J.mrow[cell + 1], r0 + kMatrixPart);
     float re = 0.0f, im = 0.0f;
     row_sum(mnz, true, r0 + g, r1, CPP, re, im);
 #pragma unroll
@@ -1074,8 +1082,30 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
             a.x += red[w][j].x;
             a.y += red[w][j].y;
         }
-        gout[cell] = a;
+        if (it.slot < 0)
+            gout[cell] = a;
+        else
+            parts[((int64_t)it.slot + it.e1) * kMatrixCoils + j] = a;
     }
+}
+
+// Cells of long rows summed in parts: cell = part 0 + part 1 + ... in part
+// order (item: unit = cell, e1 = number of parts, slot = first record).
+__global__ __launch_bounds__(kWave) void matrix_fold_kernel(const Job *__restrict__ jobs,
+                                                            const Item *__restrict__ items,
+                                                            const float2 *__restrict__ parts,
+                                                            float2 *__restrict__ grid) {
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int c = threadIdx.x;
+    if (c >= it.c1 - it.c0) return;
+    float2 a = parts[(int64_t)it.slot * kMatrixCoils + c];
+    for (int k = 1; k < it.e1; ++k) {
+        const float2 p = parts[((int64_t)it.slot + k) * kMatrixCoils + c];
+        a.x += p.x;
+        a.y += p.y;
+    }
+    grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + it.unit] = a;
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
@@ -1295,7 +1325,16 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
     return TN_OK;
 }
 
-using MatrixFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, LowRank);
+using MatrixFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, LowRank,
+                          float2 *);
+
+int launch_matrix_fold(const LaunchTables &t, float2 *grid, hipStream_t stream) {
+    if (t.n_fold == 0) return TN_OK;
+    hipLaunchKernelGGL(matrix_fold_kernel, dim3((unsigned)t.n_fold), dim3(kWave), 0, stream, t.d_jobs, t.d_fold,
+                       (const float2 *)t.d_scratch, grid);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
 
 template <bool LR>
 MatrixFn pick_matrix(int coils) {
@@ -1311,7 +1350,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
     if (t.n_grid == 0) return TN_OK;
     if (t.grid_mode == kGridMatrix && lr.ncoeff == 0 && t.st_cw > 0) {
         using TrFn = void (*)(const Job *, const Item *, const float2 *, const float *, float2 *);
-        using MtFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *);
+        using MtFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, float2 *);
         TrFn tr = nullptr;
         MtFn mt = nullptr;
         switch (t.st_cw) {
@@ -1328,18 +1367,18 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         }
         const int64_t nblk = (t.n_grid + kXcds - 1) / kXcds * kXcds;
         hipLaunchKernelGGL(mt, dim3((unsigned)nblk), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid,
-                           (const float2 *)t.d_st, grid);
+                           (const float2 *)t.d_st, grid, (float2 *)t.d_scratch);
         TN_HIP(hipGetLastError());
-        return TN_OK;
+        return launch_matrix_fold(t, grid, stream);
     }
     if (t.grid_mode == kGridMatrix) {
         if (t.grid_coils > kMatrixCoils) return fail(TN_E_ARG, "matrix gridding: %d coils per item", t.grid_coils);
         MatrixFn fn = lr.ncoeff > 0 ? pick_matrix<true>(t.grid_coils) : pick_matrix<false>(t.grid_coils);
         const int64_t nblk = (t.n_grid + kXcds - 1) / kXcds * kXcds;
         hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid, samp,
-                           grid, weight, lr);
+                           grid, weight, lr, (float2 *)t.d_scratch);
         TN_HIP(hipGetLastError());
-        return TN_OK;
+        return launch_matrix_fold(t, grid, stream);
     }
     const int ndim = t.kind_grid / 16, wc = t.kind_grid % 16;
     GridFn fn = nullptr;

@@ -332,6 +332,29 @@ def test_grid_matrix_multiframe_plan_and_empty_frame_rows():
     assert rel(a.cpu().numpy(), ref) < RTOL
 
 
+@pytest.mark.parametrize("ncoil, chunk", [(8, 0), (3, 0), (2, 1)])
+def test_grid_matrix_long_rows_in_parts(ncoil, chunk):
+    """Cells with more entries than one workgroup part (3000 samples on one
+    spot next to a uniform background): the rows are summed in parts whose
+    partials are folded in order; coil-minor (chunks of 8 and 3 coils) and
+    lane-per-cell (1 coil per item) kernels, against the oracle, twice."""
+    n = 16
+    print(f"grid matrix long rows coils={ncoil} chunk={chunk}", flush=True)
+    rng = np.random.default_rng(57)
+    bg = rng.uniform(-n / 2, n / 2, (1, 400, 2))
+    hot = np.tile(np.array([[[0.3, -0.2]]]), (1, 3000, 1)) + rng.normal(0, 0.05, (1, 3000, 2))
+    coord = np.concatenate([bg, hot], 1).astype(np.float32)
+    k = cnormal((ncoil,) + coord.shape[:-1], 58)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV, grid_mode=3, grid_chunk=chunk)
+    kt = torch.from_numpy(k).to(DEV)
+    a = tilenufft.fft.apply_nufft_adj(kt, plan)
+    b = tilenufft.fft.apply_nufft_adj(kt, plan)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+    ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0))
+    assert rel(a.cpu().numpy(), ref) < RTOL
+
+
 def test_grid_runs_refuses_wide_kernels():
     coord = testing.golden_angle_radial(12, 64, 24)
     with pytest.raises(RuntimeError, match="taps"):
