@@ -60,6 +60,17 @@ struct Idx4 {
     }
 };
 
+// Work item of this workgroup, items dealt to the XCDs in contiguous ranges:
+// the hardware deals workgroups out round-robin over the XCDs, so without
+// the remap neighbouring items (which share grid cache lines, halos and
+// samples) would land in eight different L2s. Launches are rounded up to a
+// multiple of kXcds; the padding workgroups get -1.
+__device__ __forceinline__ int64_t xcd_item(int64_t nitems) {
+    const int64_t per = (nitems + kXcds - 1) / kXcds;
+    const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    return idx < nitems ? idx : -1;
+}
+
 // Clang vector types for the nontemporal load/store builtins.
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -96,11 +107,13 @@ __device__ __forceinline__ void compute_taps(const Job &J, float cx, float cy, f
 
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ jobs,
-                                                        const Item *__restrict__ items,
+                                                        const Item *__restrict__ items, int64_t nitems,
                                                         const float2 *__restrict__ grid,
                                                         float2 *__restrict__ samp,
                                                         const float *__restrict__ weight) {
-    const Item it = items[blockIdx.x];
+    const int64_t item = xcd_item(nitems);
+    if (item < 0) return;
+    const Item it = items[item];
     const Job &J = jobs[it.job];
     const int64_t p = (int64_t)it.unit * kInterpBlockPts + threadIdx.x;
     if (p >= J.npts) return;
@@ -153,12 +166,14 @@ __global__ __launch_bounds__(kBlock) void interp_kernel(const Job *__restrict__ 
 // evaluated once per sample and reused for every coil and coefficient.
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void interp_lowrank_kernel(const Job *__restrict__ jobs,
-                                                                const Item *__restrict__ items,
+                                                                const Item *__restrict__ items, int64_t nitems,
                                                                 const float2 *__restrict__ grid,
                                                                 float2 *__restrict__ samp,
                                                                 const float *__restrict__ weight,
                                                                 LowRank lr) {
-    const Item it = items[blockIdx.x];
+    const int64_t item = xcd_item(nitems);
+    if (item < 0) return;
+    const Item it = items[item];
     const Job &J = jobs[it.job];
     const int64_t p = (int64_t)it.unit * kInterpBlockPts + threadIdx.x;
     if (p >= J.npts) return;
@@ -217,13 +232,15 @@ __global__ __launch_bounds__(kBlock) void interp_lowrank_kernel(const Job *__res
 // Samples come from the plan's primary bins (each sample exactly once).
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__restrict__ jobs,
-                                                               const Item *__restrict__ items,
+                                                               const Item *__restrict__ items, int64_t nitems,
                                                                const float2 *__restrict__ grid,
                                                                float2 *__restrict__ samp,
                                                                const float *__restrict__ weight,
                                                                int chunk) {
     extern __shared__ float2 region[];
-    const Item it = items[blockIdx.x];
+    const int64_t item = xcd_item(nitems);
+    if (item < 0) return;
+    const Item it = items[item];
     const Job &J = jobs[it.job];
     const int tile = it.unit;
     const int Tx = J.IT[0], Ty = J.IT[1], Tz = J.IT[2];
@@ -393,7 +410,7 @@ __device__ __forceinline__ float2 grid_sample(const Job &J, const float2 *__rest
 // stored as a partial of a split tile, folded by fold_kernel).
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jobs,
-                                                      const Item *__restrict__ items,
+                                                      const Item *__restrict__ items, int64_t nitems,
                                                       const float2 *__restrict__ samp,
                                                       float2 *__restrict__ grid,
                                                       const float *__restrict__ weight,
@@ -402,7 +419,9 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
                                                       LowRank lr) {
     extern __shared__ float smem[];
     constexpr int B = kGridBatch;
-    const Item it = items[blockIdx.x];
+    const int64_t item = xcd_item(nitems);
+    if (item < 0) return;
+    const Item it = items[item];
     const Job &J = jobs[it.job];
     const int tile = it.unit;
     const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
@@ -576,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void grid_kernel(const Job *__restrict__ jo
 // deterministic.
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict__ jobs,
-                                                           const Item *__restrict__ items,
+                                                           const Item *__restrict__ items, int64_t nitems,
                                                            const float2 *__restrict__ samp,
                                                            float2 *__restrict__ grid,
                                                            const float *__restrict__ weight,
@@ -588,7 +607,9 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
     constexpr int NREC = runs_record_words(NDIM, WC);  // key, (d0, kw) per axis, weights
     static_assert(B == kBlock, "one staged entry per thread");
     extern __shared__ float smem[];
-    const Item it = items[blockIdx.x];
+    const int64_t item = xcd_item(nitems);
+    if (item < 0) return;
+    const Item it = items[item];
     const Job &J = jobs[it.job];
     const int tile = it.unit;
     const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
@@ -848,9 +869,8 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restri
                                                              const float *__restrict__ weight, LowRank lr,
                                                              float2 *__restrict__ parts) {
     __shared__ float red[kBlock / kWave][2 * CM];
-    const int64_t per = (nitems + kXcds - 1) / kXcds;
-    const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-    if (idx >= nitems) return;
+    const int64_t idx = xcd_item(nitems);
+    if (idx < 0) return;
     const Item it = items[idx];
     const Job &J = jobs[it.job];
     const int ncoil = it.c1 - it.c0;
@@ -1015,9 +1035,8 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     __shared__ float2 red[kBlock / kWave][CW];
     __shared__ i32x2 sent[kMatrixStage];
     __shared__ int srow[kBlock + 1];
-    const int64_t per = (nitems + kXcds - 1) / kXcds;
-    const int64_t idx = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-    if (idx >= nitems) return;
+    const int64_t idx = xcd_item(nitems);
+    if (idx < 0) return;
     const Item it = items[idx];
     const Job &J = jobs[it.job];
     const int j = threadIdx.x & (CW - 1), g = threadIdx.x >> LCW;
@@ -1209,17 +1228,19 @@ __global__ __launch_bounds__(kBlock) void crop_apod_kernel(PrePost pp, const flo
     }
 }
 
+unsigned xcd_blocks(int64_t n) { return (unsigned)((n + kXcds - 1) / kXcds * kXcds); }
+
 unsigned stride_blocks(int64_t n) {
     const int64_t b = (n + kBlock - 1) / kBlock;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 256 * 64));
 }
 
 // ---- dispatch ----------------------------------------------------------------
-using InterpFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *);
-using StagedFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, int);
-using GridFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, float *,
+using InterpFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *);
+using StagedFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, int);
+using GridFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, float *,
                         const int64_t *, LowRank);
-using InterpLrFn = void (*)(const Job *, const Item *, const float2 *, float2 *, const float *, LowRank);
+using InterpLrFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, LowRank);
 
 template <int NDIM>
 InterpLrFn pick_interp_lowrank(int wc) {
@@ -1313,13 +1334,13 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
     if (t.interp_mode == kInterpStaged) {
         StagedFn fn = ndim == 2 ? pick_staged<2>(wc) : ndim == 3 ? pick_staged<3>(wc) : nullptr;
         if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
-        hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), t.interp_lds, stream, t.d_jobs,
-                           t.d_interp, grid, samp, weight, t.interp_coils);
+        hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_interp)), dim3(kBlock), t.interp_lds, stream, t.d_jobs,
+                           t.d_interp, t.n_interp, grid, samp, weight, t.interp_coils);
     } else {
         InterpFn fn = ndim == 2 ? pick_interp<2>(wc) : ndim == 3 ? pick_interp<3>(wc) : nullptr;
         if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
-        hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), 0, stream, t.d_jobs,
-                           t.d_interp, grid, samp, weight);
+        hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_interp)), dim3(kBlock), 0, stream, t.d_jobs,
+                           t.d_interp, t.n_interp, grid, samp, weight);
     }
     TN_HIP(hipGetLastError());
     return TN_OK;
@@ -1365,8 +1386,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                                t.d_st);
             TN_HIP(hipGetLastError());
         }
-        const int64_t nblk = (t.n_grid + kXcds - 1) / kXcds * kXcds;
-        hipLaunchKernelGGL(mt, dim3((unsigned)nblk), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid,
+        hipLaunchKernelGGL(mt, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid,
                            (const float2 *)t.d_st, grid, (float2 *)t.d_scratch);
         TN_HIP(hipGetLastError());
         return launch_matrix_fold(t, grid, stream);
@@ -1374,8 +1394,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
     if (t.grid_mode == kGridMatrix) {
         if (t.grid_coils > kMatrixCoils) return fail(TN_E_ARG, "matrix gridding: %d coils per item", t.grid_coils);
         MatrixFn fn = lr.ncoeff > 0 ? pick_matrix<true>(t.grid_coils) : pick_matrix<false>(t.grid_coils);
-        const int64_t nblk = (t.n_grid + kXcds - 1) / kXcds * kXcds;
-        hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid, samp,
+        hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid, samp,
                            grid, weight, lr, (float2 *)t.d_scratch);
         TN_HIP(hipGetLastError());
         return launch_matrix_fold(t, grid, stream);
@@ -1387,8 +1406,8 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
     else
         fn = ndim == 2 ? pick_grid<2>(wc) : ndim == 3 ? pick_grid<3>(wc) : nullptr;
     if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d, mode=%d)", ndim, wc, t.grid_mode);
-    hipLaunchKernelGGL(fn, dim3((unsigned)t.n_grid), dim3(kBlock), t.grid_lds, stream, t.d_jobs,
-                       t.d_grid, samp, grid, weight, t.d_scratch, t.d_slot_base, lr);
+    hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), t.grid_lds, stream, t.d_jobs,
+                       t.d_grid, t.n_grid, samp, grid, weight, t.d_scratch, t.d_slot_base, lr);
     TN_HIP(hipGetLastError());
     if (t.n_fold > 0) {
         hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
@@ -1405,8 +1424,8 @@ int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *sam
     const int ndim = t.kind_interp / 16, wc = t.kind_interp % 16;
     InterpLrFn fn = ndim == 2 ? pick_interp_lowrank<2>(wc) : ndim == 3 ? pick_interp_lowrank<3>(wc) : nullptr;
     if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
-    hipLaunchKernelGGL(fn, dim3((unsigned)t.n_interp), dim3(kBlock), 0, stream, t.d_jobs, t.d_interp,
-                       grid, samp, weight, lr);
+    hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_interp)), dim3(kBlock), 0, stream, t.d_jobs, t.d_interp,
+                       t.n_interp, grid, samp, weight, lr);
     TN_HIP(hipGetLastError());
     return TN_OK;
 }
