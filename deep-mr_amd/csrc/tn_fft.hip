@@ -34,8 +34,10 @@
 namespace tn {
 namespace {
 
-constexpr int kFftPad = 8;  // LDS column padding in float2: a column stride of 8 mod 16
-                            // spreads the row-major tile loads over the banks
+// LDS column padding of a C-column tile, in float2: a column stride of
+// 64 / C mod 16 float2 puts the 64 lanes of a row-major tile load (C columns
+// x 64 / C rows) on 16 distinct bank pairs, four lanes each.
+constexpr int fft_pad(int C) { return 64 / C; }
 
 // Radix of pass p of a length-L transform: radix 8 while possible, then 4,
 // 2, 3 and 5; 0 past the last pass.
@@ -129,41 +131,54 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
     }
 }
 
-// Stockham passes P.. of a length-L line held by one wave. Pass P, radix R,
-// NS = product of the earlier radices: butterfly j < L/R reads elements
-// j + r L/R, multiplies element r by W_{NS R}^{r (j mod NS)}, runs the
-// R-point DFT and stores element r at (j / NS) NS R + (j mod NS) + r NS.
-// Pass 0 reads through `src`, the last pass writes through `dst`, the others
-// use `buf` (the wave's LDS line; src / dst may address it too).
-template <int L, int P, int NS, bool INV, class Src, class Dst>
-__device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ tw, float2 *buf, Src &src, Dst &dst) {
+// Stockham passes P.. of NL length-L lines held by one wave (the lines are
+// transformed together: their loads are independent and share the
+// twiddles). Pass P, radix R, NS = product of the earlier radices:
+// butterfly j < L/R reads elements j + r L/R, multiplies element r by
+// W_{NS R}^{r (j mod NS)}, runs the R-point DFT and stores element r at
+// (j / NS) NS R + (j mod NS) + r NS. Pass 0 reads line l through
+// src(l, i), the last pass writes through dst(l, i, v), the others use
+// buf[l] (the wave's LDS lines; src / dst may address them too).
+template <int L, int NL, int P, int NS, bool INV, class Src, class Dst>
+__device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ tw, float2 *const *buf, Src &src,
+                                           Dst &dst) {
     constexpr int NP = fft_npass(L);
     constexpr int R = fft_radix_at(L, P);
     constexpr int M = L / R;
     constexpr int Q = (M + kWave - 1) / kWave;
     constexpr int TS = L / (NS * R);  // twiddle table stride
-    float2 v[Q][R];
+    float2 v[NL][Q][R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int j = lane + q * kWave;
         if (M % kWave == 0 || j < M) {
             const int k = j % NS;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (P == 0)
-                    v[q][r] = src(j + r * M);
-                else
-                    v[q][r] = buf[j + r * M];
-            }
+            float2 w[R];
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[r * k * TS];
-                    if (INV) w.y = -w.y;
-                    v[q][r] = cmul(v[q][r], w);
+                    w[r] = tw[r * k * TS];
+                    if (INV) w[r].y = -w[r].y;
                 }
             }
-            dft<R, INV>(v[q]);
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (P == 0)
+                        v[l][q][r] = src(l, j + r * M);
+                    else
+                        v[l][q][r] = buf[l][j + r * M];
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[l][q][r] = cmul(v[l][q][r], w[r]);
+                }
+                dft<R, INV>(v[l][q]);
+            }
         }
     }
 #pragma unroll
@@ -173,20 +188,24 @@ __device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ 
             const int k = j % NS;
             const int d = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if constexpr (P == NP - 1)
-                    dst(d + r * NS, v[q][r]);
-                else
-                    buf[d + r * NS] = v[q][r];
+            for (int l = 0; l < NL; ++l) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (P == NP - 1)
+                        dst(l, d + r * NS, v[l][q][r]);
+                    else
+                        buf[l][d + r * NS] = v[l][q][r];
+                }
             }
         }
     }
-    if constexpr (P + 1 < NP) fft_passes<L, P + 1, NS * R, INV>(lane, tw, buf, src, dst);
+    if constexpr (P + 1 < NP) fft_passes<L, NL, P + 1, NS * R, INV>(lane, tw, buf, src, dst);
 }
 
-template <int L, bool INV, class Src, class Dst>
-__device__ __forceinline__ void fft_line(int lane, const float2 *__restrict__ tw, float2 *buf, Src src, Dst dst) {
-    fft_passes<L, 0, 1, INV>(lane, tw, buf, src, dst);
+template <int L, int NL, bool INV, class Src, class Dst>
+__device__ __forceinline__ void fft_lines(int lane, const float2 *__restrict__ tw, float2 *const *buf, Src src,
+                                          Dst dst) {
+    fft_passes<L, NL, 0, 1, INV>(lane, tw, buf, src, dst);
 }
 
 // data index of pixel-space position i of an axis: kept iff (i + N/2) mod L < N
@@ -207,14 +226,15 @@ __global__ __launch_bounds__(kBlock) void fft_rows_kernel(float2 *__restrict__ d
     const int64_t line = (int64_t)blockIdx.x * (kBlock / kWave) + wave;
     if (line >= nlines) return;  // no barriers below
     float2 *row = data + line * L;
+    float2 *const buf[1] = {lds[wave]};
     if (!INV) {
-        fft_line<L, false>(
-            lane, tw, lds[wave], [&](int i) { return kept(i, Nx, L) ? row[i] : make_float2(0.0f, 0.0f); },
-            [&](int i, float2 v) { row[i] = v; });
+        fft_lines<L, 1, false>(
+            lane, tw, buf, [&](int, int i) { return kept(i, Nx, L) ? row[i] : make_float2(0.0f, 0.0f); },
+            [&](int, int i, float2 v) { row[i] = v; });
     } else {
-        fft_line<L, true>(
-            lane, tw, lds[wave], [&](int i) { return row[i]; },
-            [&](int i, float2 v) {
+        fft_lines<L, 1, true>(
+            lane, tw, buf, [&](int, int i) { return row[i]; },
+            [&](int, int i, float2 v) {
                 if (kept(i, Nx, L)) row[i] = v;
             });
     }
@@ -243,8 +263,8 @@ __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__re
                                                               int Ny, int Gx, const float *__restrict__ apx,
                                                               const float *__restrict__ apy,
                                                               const float2 *__restrict__ tw) {
-    constexpr int LP = L + kFftPad;
-    __shared__ float2 lds[C * LP];
+    constexpr int LP = L + fft_pad(C);
+    extern __shared__ float2 lds[];  // C * LP
     int64_t b;
     int mx0;
     if (!column_block(nimg, Nx, C, b, mx0)) return;
@@ -264,11 +284,15 @@ __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__re
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    for (int c = wave; c < C; c += kBlock / kWave) {
-        float2 *col = lds + c * LP;
-        fft_line<L, false>(
-            lane, tw, col, [&](int i) { return kept(i, Ny, L) ? col[i] : make_float2(0.0f, 0.0f); },
-            [&](int i, float2 v) { col[i] = v; });
+    {   // wave w transforms columns w, w + 4, ... together
+        constexpr int NL = C / (kBlock / kWave);
+        float2 *col[NL];
+#pragma unroll
+        for (int l = 0; l < NL; ++l) col[l] = lds + (wave + l * (kBlock / kWave)) * LP;
+        fft_lines<L, NL, false>(
+            lane, tw, col,
+            [&](int l, int i) { return kept(i, Ny, L) ? col[l][i] : make_float2(0.0f, 0.0f); },
+            [&](int l, int i, float2 v) { col[l][i] = v; });
     }
     __syncthreads();
     for (int i = threadIdx.x; i < L * C; i += kBlock) {
@@ -291,8 +315,8 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
                                                               int Ny, int Gx, const float *__restrict__ apx,
                                                               const float *__restrict__ apy,
                                                               const float2 *__restrict__ tw, float scale) {
-    constexpr int LP = L + kFftPad;
-    __shared__ float2 lds[C * LP];
+    constexpr int LP = L + fft_pad(C);
+    extern __shared__ float2 lds[];  // C * LP
     int64_t b;
     int mx0;
     if (!column_block(nimg, Nx, C, b, mx0)) return;
@@ -310,10 +334,14 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    for (int c = wave; c < C; c += kBlock / kWave) {
-        float2 *col = lds + c * LP;
-        fft_line<L, true>(
-            lane, tw, col, [&](int i) { return col[i]; }, [&](int i, float2 v) { col[i] = v; });
+    {   // wave w transforms columns w, w + 4, ... together
+        constexpr int NL = C / (kBlock / kWave);
+        float2 *col[NL];
+#pragma unroll
+        for (int l = 0; l < NL; ++l) col[l] = lds + (wave + l * (kBlock / kWave)) * LP;
+        fft_lines<L, NL, true>(
+            lane, tw, col, [&](int l, int i) { return col[l][i]; },
+            [&](int l, int i, float2 v) { col[l][i] = v; });
     }
     __syncthreads();
     for (int i = threadIdx.x; i < Ny * C; i += kBlock) {
@@ -330,7 +358,12 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
     }
 }
 
+// Columns per column-stage workgroup. The strided row segments, not the
+// transform, bound these stages; still, wider tiles (16 columns at L = 512,
+// 66 KiB of LDS, two workgroups per CU) measured slower than 8 columns at
+// four workgroups per CU.
 constexpr int col_group(int L) { return L <= 256 ? 16 : L <= 512 ? 8 : 4; }
+constexpr size_t col_lds_bytes(int L) { return sizeof(float2) * (size_t)col_group(L) * (L + fft_pad(col_group(L))); }
 
 using RowFn = void (*)(float2 *, int64_t, int, const float2 *);
 using ColFwdFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, const float *, const float *,
@@ -344,6 +377,7 @@ struct FftFns {
     ColFwdFn col_fwd = nullptr;
     ColInvFn col_inv = nullptr;
     int C = 0;
+    size_t lds = 0;  // dynamic LDS bytes of the column kernels
 };
 
 template <int L>
@@ -355,6 +389,7 @@ FftFns fns_for() {
     f.row_inv = fft_rows_kernel<L, true>;
     f.col_fwd = fft_cols_fwd_kernel<L, col_group(L)>;
     f.col_inv = fft_cols_inv_kernel<L, col_group(L)>;
+    f.lds = col_lds_bytes(L);
     return f;
 }
 
@@ -394,6 +429,11 @@ bool fft_supported(const tn_plan *plan) {
 
 int build_fft(tn_plan *plan) {
     if (!fft_supported(plan)) return TN_OK;
+    {   // dynamic LDS of the column tiles (up to 64 KiB with these column counts)
+        const FftFns fy = fft_fns(plan->p.grid[1]);
+        TN_HIP(hipFuncSetAttribute((const void *)fy.col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fy.lds));
+        TN_HIP(hipFuncSetAttribute((const void *)fy.col_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fy.lds));
+    }
     for (int a = 0; a < 2; ++a) {  // 0: x (grid[2]), 1: y (grid[1])
         const int L = plan->p.grid[2 - a];
         std::vector<float2> tw(L);
@@ -414,7 +454,7 @@ int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, i
     const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
     const FftFns fy = fft_fns(Gy), fx = fft_fns(Gx);
-    hipLaunchKernelGGL(fy.col_fwd, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), 0, stream, image, grid, nimg, Nx,
+    hipLaunchKernelGGL(fy.col_fwd, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), fy.lds, stream, image, grid, nimg, Nx,
                        Ny, Gx, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
                        (const float2 *)plan->d_fft_tw[1]);
     TN_HIP(hipGetLastError());
@@ -438,7 +478,7 @@ int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t
                        (const float2 *)plan->d_fft_tw[0]);
     TN_HIP(hipGetLastError());
     const float scale = (float)(1.0 / ((double)Gx * (double)Gy));
-    hipLaunchKernelGGL(fy.col_inv, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), 0, stream, grid, image, nimg, Nx,
+    hipLaunchKernelGGL(fy.col_inv, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), fy.lds, stream, grid, image, nimg, Nx,
                        Ny, Gx, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
                        (const float2 *)plan->d_fft_tw[1], scale);
     TN_HIP(hipGetLastError());
