@@ -393,8 +393,9 @@ FftFns fns_for() {
     return f;
 }
 
-// Transform lengths with kernels: 2^a 3^b 5^c from 64 to 1024 (the common
-// oversampled sizes of 64..512-pixel images at oversampling 1.25 .. 2).
+// Transform lengths with kernels: the oversampled sizes ceil(osf N) of
+// N = 64, 128, 192, 256, 320, 384, 512 at osf 1.25, 1.5 and 2 (all of the
+// form 2^a 3^b 5^c), 64 .. 1024.
 FftFns fft_fns(int L) {
     switch (L) {
         case 64: return fns_for<64>();
@@ -403,10 +404,15 @@ FftFns fft_fns(int L) {
         case 128: return fns_for<128>();
         case 160: return fns_for<160>();
         case 192: return fns_for<192>();
+        case 240: return fns_for<240>();
         case 256: return fns_for<256>();
+        case 288: return fns_for<288>();
         case 320: return fns_for<320>();
         case 384: return fns_for<384>();
+        case 400: return fns_for<400>();
+        case 480: return fns_for<480>();
         case 512: return fns_for<512>();
+        case 576: return fns_for<576>();
         case 640: return fns_for<640>();
         case 768: return fns_for<768>();
         case 1024: return fns_for<1024>();
@@ -449,7 +455,7 @@ int build_fft(tn_plan *plan) {
 }
 
 int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, int64_t nimg, hipStream_t stream) {
-    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D, lengths 2^a 3^b 5^c in 64..1024)");
+    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D plans with common oversampled grid lengths, 64..1024)");
     if (nimg == 0) return TN_OK;
     const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
@@ -467,7 +473,7 @@ int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, i
 }
 
 int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t nimg, hipStream_t stream) {
-    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D, lengths 2^a 3^b 5^c in 64..1024)");
+    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D plans with common oversampled grid lengths, 64..1024)");
     if (nimg == 0) return TN_OK;
     const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];

@@ -159,8 +159,9 @@ int tn_apod_pad(tn_plan *plan, const void *image, void *grid, int64_t nimg,
 int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg,
                  tn_stream stream);
 
-/* 1 when the plan has the pruned, fused oversampled FFT stages below
- * (2D plans whose grid lengths are 2^a 3^b 5^c between 64 and 1024), else 0. */
+/* 1 when the plan has the pruned, fused oversampled FFT stages below, else
+ * 0: 2D plans whose grid lengths are among the oversampled sizes ceil(osf N)
+ * of N = 64..512 at osf 1.25, 1.5 and 2 (64, 80, 96, 128, 160, 192, 240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024). */
 int tn_plan_has_fft(const tn_plan *plan);
 
 /* Whole forward FFT stage of the NUFFT on the device: apodization, centred

@@ -43,6 +43,9 @@ CASES = [
     ((320, 160), 2.0, False),         # 640 x 320
     ((512, 384), (2.0, 2.0), False),  # 1024 x 768
     ((64, 48), 2.0, True),            # Cartesian y: 64 (no padding) x 96
+    ((192, 192), 1.25, False),        # 240 = 8 * 2 * 3 * 5
+    ((320, 320), 1.25, False),        # 400 = 8 * 2 * 5 * 5
+    ((384, 192), (1.5, 1.25), False),  # 576 x 240
 ]
 
 
