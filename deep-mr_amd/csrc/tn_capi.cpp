@@ -471,8 +471,18 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
             delete pl;
             return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per z plane (got %d)", kRunsMaxTaps, taps);
         }
+        // automatic: the sparse gridding matrix when it fits kMatrixAutoBytes
+        // (8 bytes per sample and tap), else register runs when the taps of a
+        // z plane fit the registers, else per-sample LDS atomics
+        const int64_t nnz = (int64_t)params->nframes * params->npts * params->width[0] * params->width[1] *
+                            params->width[2];
+        const int64_t nrows = (int64_t)params->nframes * params->grid[0] * params->grid[1] * params->grid[2];
+        const bool matrix_ok = nnz < INT32_MAX && (uint64_t)nnz * sizeof(int2) <= kMatrixAutoBytes &&
+                               nrows < (int64_t)UINT32_MAX && params->npts <= ((int64_t)1 << 28);
         pl->grid_mode = params->grid_mode == kGridSpread   ? kGridSpread
                         : params->grid_mode == kGridMatrix ? kGridMatrix
+                        : params->grid_mode == kGridRuns   ? kGridRuns
+                        : matrix_ok                        ? kGridMatrix
                         : runs_ok                          ? kGridRuns
                                                            : kGridSpread;
     }

@@ -65,7 +65,9 @@ typedef struct tn_plan_params {
     int32_t grid_split;   /* max samples per gridding workgroup before a hot tile
                              is split across workgroups (partials folded in a
                              fixed order); 0 = automatic, < 0 = never split   */
-    int32_t grid_mode;    /* gridding variant: 0 automatic, 1 one LDS atomic per
+    int32_t grid_mode;    /* gridding variant: 0 automatic (3 when its matrix
+                             fits in 4 GiB, else 2 when W_y * W_x <= 36,
+                             else 1), 1 one LDS atomic per
                              tap and sample (entries spread over footprint
                              origins), 2 per-lane runs of origin-sorted
                              entries summed in registers, one z plane of the

@@ -359,7 +359,7 @@ def test_grid_runs_refuses_wide_kernels():
     coord = testing.golden_angle_radial(12, 64, 24)
     with pytest.raises(RuntimeError, match="taps"):
         tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV, grid_mode=2)
-    plan = tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV)  # automatic: per-sample atomics
+    plan = tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV)  # automatic: the gridding matrix (49 taps)
     k = torch.from_numpy(cnormal((1,) + coord.shape[:-1], 52)).to(DEV)
     ref = orc.apply_nufft_adj(k.cpu().numpy(), orc.plan_nufft(coord, 24, 7, 2.0))
     assert rel(tilenufft.fft.apply_nufft_adj(k, plan).cpu().numpy(), ref) < RTOL
