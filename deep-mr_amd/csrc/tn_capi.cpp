@@ -120,6 +120,42 @@ static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &
     return TN_OK;
 }
 
+// Matrix-gridding runs of frame f (cached in the plan per cells-per-item
+// limit): runs of consecutive cells with at most cpi cells and at most
+// kMatrixStage entries, so that an item's entries (contiguous in the CSR
+// order) fit the kernel's LDS stage; rows longer than mlong end the run and
+// are summed in ceil(len / kMatrixPart) parts (listed first).
+static const std::vector<MatrixRun> &matrix_runs(const tn_plan *pl, int64_t f, int64_t ncell, int cpi, int mlong) {
+    auto &per_frame = pl->mruns[cpi];
+    if (per_frame.empty()) per_frame.resize(pl->p.nframes);
+    std::vector<MatrixRun> &out = per_frame[f];
+    if (!out.empty() || ncell == 0) return out;
+    const int32_t *row = pl->h_mrow.data() + f * ncell;
+    std::vector<MatrixRun> shorts;
+    int64_t start = 0;
+    int ncells = 0, nent = 0;
+    auto close = [&]() {
+        if (ncells > 0) shorts.push_back(MatrixRun{(int32_t)start, ncells, row[start], row[start + ncells] - row[start]});
+        ncells = 0;
+        nent = 0;
+    };
+    for (int64_t c = 0; c < ncell; ++c) {
+        const int len = row[c + 1] - row[c];
+        if (len > mlong) {
+            close();
+            out.push_back(MatrixRun{(int32_t)c, -((len + kMatrixPart - 1) / kMatrixPart), row[c], len});
+            continue;
+        }
+        if (ncells == cpi || nent + len > kMatrixStage) close();
+        if (ncells == 0) start = c;
+        ++ncells;
+        nent += len;
+    }
+    close();
+    out.insert(out.end(), shorts.begin(), shorts.end());
+    return out;
+}
+
 // Work lists. Interpolation items are (job, sample block, coil range);
 // gridding items are (job, tile, coil range). `plan_of[j]` is the plan job j
 // came from (for its chunk preferences).
@@ -169,7 +205,6 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         // the latency of its LDS read-add-write chains)
         const int gc_auto = gmode == kGridRuns ? std::min(lds_coils, kBlock / kWave) : lds_coils;
         const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, gc_auto), lds_coils);
-        max_gc = std::max(max_gc, gc);
         if (mode == kInterpStaged) {
             // one item per (tile, segment) covers every coil; the region is
             // loaded `ic` coils at a time
@@ -206,44 +241,24 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             const int cpi = mc >= 2 ? std::min(kBlock, kBlock / cw * kMatrixPasses) : kBlock;
             const int64_t ncell = (int64_t)j.G[0] * j.G[1] * j.G[2];
             const int64_t f = (j.mrow - pl->d_mrow) / std::max<int64_t>(ncell, 1);
-            const int32_t *row = pl->h_mrow.data() + f * ncell;
-            std::vector<std::pair<int64_t, int>> runs;
-            int64_t start = 0;
-            int ncells = 0, nent = 0;
-            auto close = [&]() {
-                if (ncells > 0) runs.emplace_back(start, ncells);
-                ncells = 0;
-                nent = 0;
-            };
-            for (int64_t c = 0; c < ncell; ++c) {
-                const int len = row[c + 1] - row[c];
-                if (len > j.mlong) {  // parts of kMatrixPart entries; several parts leave records to fold
-                    close();
-                    const int nparts = (len + kMatrixPart - 1) / kMatrixPart;
-                    for (int c0 = 0; c0 < j.nbatch; c0 += mc) {
-                        const int c1 = std::min(j.nbatch, c0 + mc);
-                        if (nparts == 1) {
-                            grid.push_back(Item{(int)ji, (int)c, c0, c1, c0 / mc, 0, 1, -1});
-                            continue;
-                        }
-                        const int32_t rec = (int32_t)(scratch / (2 * kMatrixCoils));
-                        scratch += (int64_t)nparts * 2 * kMatrixCoils;
-                        for (int k = 0; k < nparts; ++k) grid.push_back(Item{(int)ji, (int)c, c0, c1, c0 / mc, k, 1, rec});
-                        fold.push_back(Item{(int)ji, (int)c, c0, c1, c0 / mc, nparts, 0, rec});
+            const std::vector<MatrixRun> &runs = matrix_runs(pl, f, ncell, cpi, j.mlong);
+            for (const MatrixRun &r : runs) {
+                for (int c0 = 0; c0 < j.nbatch; c0 += mc) {
+                    const int c1 = std::min(j.nbatch, c0 + mc);
+                    if (r.ncells > 0) {
+                        grid.push_back(Item{(int)ji, r.cell, c0, c1, c0 / mc, r.ncells | (r.ne << 16), 0, r.e0});
+                        continue;
                     }
-                    continue;
+                    const int nparts = -r.ncells;
+                    if (nparts == 1) {
+                        grid.push_back(Item{(int)ji, r.cell, c0, c1, c0 / mc, 0, 1, -1});
+                        continue;
+                    }
+                    const int32_t rec = (int32_t)(scratch / (2 * kMatrixCoils));
+                    scratch += (int64_t)nparts * 2 * kMatrixCoils;
+                    for (int k = 0; k < nparts; ++k) grid.push_back(Item{(int)ji, r.cell, c0, c1, c0 / mc, k, 1, rec});
+                    fold.push_back(Item{(int)ji, r.cell, c0, c1, c0 / mc, nparts, 0, rec});
                 }
-                if (ncells == cpi || nent + len > kMatrixStage) close();
-                if (ncells == 0) start = c;
-                ++ncells;
-                nent += len;
-            }
-            close();
-            for (const auto &r : runs) {
-                const int32_t e0 = row[r.first], ne = row[r.first + r.second] - e0;
-                for (int c0 = 0; c0 < j.nbatch; c0 += mc)
-                    grid.push_back(Item{(int)ji, (int)r.first, c0, std::min(j.nbatch, c0 + mc), c0 / mc,
-                                        r.second | (ne << 16), 0, e0});
             }
             // coil-minor sample copy: chunks of mc coils padded to cw lanes
             if (mc >= 2) {
@@ -257,6 +272,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             }
             continue;
         }
+        max_gc = std::max(max_gc, gc);
         const int64_t ntiles = (int64_t)j.NT[0] * j.NT[1] * j.NT[2];
         const int64_t fg = (j.bin_off - pl->d_bin_off) / std::max<int64_t>(pl->ntiles_total, 1);
         const int32_t *goff = pl->h_bin_off.data() + fg * pl->ntiles_total;

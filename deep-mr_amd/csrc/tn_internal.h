@@ -143,6 +143,15 @@ struct LaunchTables {
 
 }  // namespace tn
 
+namespace tn {
+// Matrix-gridding work of one frame, independent of the coils: a run of
+// ncells consecutive cells from `cell` whose entries are [e0, e0 + ne), or,
+// with ncells < 0, a long row `cell` summed in -ncells parts.
+struct MatrixRun {
+    int32_t cell, ncells, e0, ne;
+};
+}  // namespace tn
+
 struct tn_plan {
     int device = 0;
     tn_plan_params p{};        // (z, y, x) as given; apod pointers cleared
@@ -157,6 +166,9 @@ struct tn_plan {
     std::vector<int32_t> h_pbin_off;  // host copies (work-list construction)
     std::vector<int32_t> h_bin_off;
     std::vector<int32_t> h_mrow;      // host copy of d_mrow (work-list construction)
+    // matrix-gridding runs per frame, keyed by the cells-per-item limit
+    // (built once per plan; launch tables of any coil count reuse them)
+    mutable std::map<int, std::vector<std::vector<tn::MatrixRun>>> mruns;
     float *d_coord = nullptr;
     float4 *d_entries = nullptr;
     int32_t *d_bin_off = nullptr;
