@@ -194,6 +194,19 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     int max_gc = 1, max_ic = 1;
     int st_cw = 0;
     bool st_none = false;
+    // matrix gridding: record width of the coil-minor copy (the widest chunk
+    // of the table) and the cells per item the kernel assumes for it
+    int mat_cpi = kBlock;
+    if (gmode == kGridMatrix) {
+        int cwmax = 0;
+        bool inplace = false;
+        for (size_t ji = 0; ji < jobs.size(); ++ji) {
+            const int mc = std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch, kMatrixChunk), kMatrixCoils);
+            if (mc < 2) inplace = true;
+            cwmax = std::max(cwmax, mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16);
+        }
+        if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / cwmax * kMatrixPasses);
+    }
     std::vector<Item> interp, grid, fold, tr;
     std::vector<int64_t> slot_base;
     int64_t scratch = 0;
@@ -238,7 +251,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             const int mc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, kMatrixChunk), kMatrixCoils);
             max_gc = std::max(max_gc, mc);
             const int cw = mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16;
-            const int cpi = mc >= 2 ? std::min(kBlock, kBlock / cw * kMatrixPasses) : kBlock;
+            const int cpi = mat_cpi;
             const int64_t ncell = (int64_t)j.G[0] * j.G[1] * j.G[2];
             const int64_t f = (j.mrow - pl->d_mrow) / std::max<int64_t>(ncell, 1);
             const std::vector<MatrixRun> &runs = matrix_runs(pl, f, ncell, cpi, j.mlong);

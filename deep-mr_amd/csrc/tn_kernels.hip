@@ -1032,7 +1032,12 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     constexpr int CW = 1 << LCW;
     constexpr int CPP = kBlock / CW;  // cells per pass
     constexpr int U = 4;              // entries in flight per lane
+    // cells per item (the host builds items of at most this many cells for
+    // this record width); the item's sums are staged per coil plane in LDS
+    // and stored as contiguous runs (row pitch CPI + 1 against bank conflicts)
+    constexpr int CPI = CPP * kMatrixPasses < kBlock ? CPP * kMatrixPasses : kBlock;
     __shared__ float2 red[kBlock / kWave][CW];
+    __shared__ float2 sout[CW * (CPI + 1)];
     __shared__ i32x2 sent[kMatrixStage];
     __shared__ int srow[kBlock + 1];
     const int64_t idx = xcd_item(nitems);
@@ -1066,7 +1071,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     };
     if (it.part == 0) {
         const int64_t cell0 = it.unit;
-        const int ncells = min(it.e1 & 0xffff, kBlock);
+        const int ncells = min(it.e1 & 0xffff, CPI);
         const int rb = it.slot;
         const int nent = min(it.e1 >> 16, kMatrixStage);  // <= kMatrixStage by construction
         for (int i = threadIdx.x; i <= ncells; i += kBlock) srow[i] = min(max(J.mrow[cell0 + i] - rb, 0), nent);
@@ -1075,9 +1080,16 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
         for (int c = g; c < ncells; c += CPP) {
             float re = 0.0f, im = 0.0f;
             row_sum(sent, false, srow[c], srow[c + 1], 1, re, im);
-            if (store) {
-                const f32x2 v = {re, im};
-                __builtin_nontemporal_store(v, (f32x2 *)(gout + cell0 + c));
+            sout[j * (CPI + 1) + c] = make_float2(re, im);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < CW * CPI; i += kBlock) {
+            const int jj = i / CPI, c = i - jj * CPI;
+            if (c < ncells && it.c0 + jj < it.c1) {
+                const float2 a = sout[jj * (CPI + 1) + c];
+                const f32x2 v = {a.x, a.y};
+                __builtin_nontemporal_store(
+                    v, (f32x2 *)(grid + J.grid_off + (int64_t)(it.c0 + jj) * J.grid_bstride + cell0 + c));
             }
         }
         return;
