@@ -162,6 +162,53 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
     )
 
 
+def kooshball_bench():
+    """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
+    256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3); gridding
+    with register runs (tile 16^3) and with the gridding matrix (long rows above
+    256 entries), staged interpolation (tile 8x8x16). Durations from HIP events
+    on the launch stream; algorithmic bytes as for the 2D workload."""
+    import tilenufft
+    from tilenufft import testing
+    from tilenufft.batch import BatchedNUFFT
+    from tilenufft.fft.nufft import retune
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    coord = testing.kooshball(20000, 256, 128)
+    out = {"workload": "3D kooshball 128^3, 8 coils, 20000 spokes x 256 samples, KB width 4", "cases": []}
+    gen = torch.Generator().manual_seed(0)
+    for osf in (2.0, 1.5):
+        log(f"bench: kooshball osf {osf}")
+        plan = tilenufft.fft.plan_nufft(coord, 128, 4, osf, device=dev, tile=(16, 16, 16), itile=(8, 8, 16),
+                                        interp_mode=2, grid_mode=2)
+        case = {"osf": osf, "grid": list(plan.os_shape)}
+        for name, gm, split in (("runs", 2, 0), ("matrix", 3, 256)):
+            if gm != 2:
+                t0 = time.perf_counter()
+                retune(plan, grid_mode=gm, grid_split=split)
+                torch.cuda.synchronize(dev)
+                case["matrix_plan_s"] = time.perf_counter() - t0
+                case["matrix_plan_bytes"] = plan.interpolator.device_bytes()
+            b = BatchedNUFFT([plan], 8)
+            b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(dev))
+            t = time_kernel(lambda: b.gridding(b.grid), 3, dev)
+            b_interp, b_grid = b.algorithmic_bytes()
+            case[f"grid_{name}_ms"] = t
+            case[f"grid_{name}_frac"] = b_grid / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
+            if gm == 2:
+                ti = time_kernel(lambda: b.interp(b.grid), 3, dev)
+                case["interp_ms"] = ti
+                case["interp_frac"] = b_interp / (ti * 1e-3) / 1e9 / HBM_PEAK_GBS
+                case["grid_alg_bytes"], case["interp_alg_bytes"] = b_grid, b_interp
+            b = None
+            torch.cuda.empty_cache()
+        out["cases"].append(case)
+        plan = None
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,7 +225,13 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after measuring, run 3 plain device copies of this many MiB (a known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--kooshball", action="store_true",
+                    help="instead: time 3D kooshball gridding / interpolation (128^3, 8 coils, 20000 x 256, "
+                         "W 4, osf 2.0 and 1.5) with register runs and the gridding matrix; one JSON line")
     args = ap.parse_args()
+    if args.kooshball:
+        kooshball_bench()
+        return
     if args.tune_cache:
         os.environ["TILENUFFT_CACHE"] = args.tune_cache
     if args.gpus != 1:
