@@ -224,6 +224,37 @@ __global__ __launch_bounds__(kBlock) void interp_lowrank_kernel(const Job *__res
     }
 }
 
+// Tap sum of one sample over an LDS region (origin r, row pitch Rx, plane
+// pitch Rx * Ry); FULL: every axis has all WC taps (no per-tap tests).
+template <int NDIM, int WC, bool FULL>
+__device__ __forceinline__ float2 staged_taps(const float2 *r, const float (&w)[3][WC], int nx, int ny, int nz,
+                                              int Rx, int Ry) {
+    constexpr int KZ = NDIM > 2 ? WC : 1;
+    float re = 0.0f, im = 0.0f;
+#pragma unroll
+    for (int kz = 0; kz < KZ; ++kz) {
+        if (FULL || kz < nz) {
+#pragma unroll
+            for (int ky = 0; ky < WC; ++ky) {
+                if (FULL || ky < ny) {
+                    const float vzy = w[2][kz] * w[1][ky];
+                    const float2 *row = r + (kz * Ry + ky) * Rx;
+#pragma unroll
+                    for (int kx = 0; kx < WC; ++kx) {
+                        if (FULL || kx < nx) {
+                            const float v = vzy * w[0][kx];
+                            const float2 sv = row[kx];
+                            re += v * sv.x;
+                            im += v * sv.y;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return make_float2(re, im);
+}
+
 // Staged interpolation: a workgroup takes one segment (<= kBlock * kStagedEpt
 // samples) of one interpolation tile. Each lane evaluates the taps of its
 // samples once and keeps them in registers; then, coil chunk by coil chunk,
@@ -290,17 +321,34 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
         }
     }
 
+    // regions no larger than the grid wrap at most once per axis, and the
+    // offsets of the item's coils fit 32 bits: the cheap load path
+    const bool near = Rx <= Gx && Ry <= Gy && Rz <= Gz && (int64_t)it.c1 * J.grid_bstride < INT32_MAX;
+    // every axis at the full compiled width: branch-free tap sums
+    const bool full = nx == WC && (NDIM < 2 || ny == WC) && (NDIM < 3 || nz == WC);
     for (int cc = it.c0; cc < it.c1; cc += chunk) {
         const int ncoil = min(chunk, it.c1 - cc);
         __syncthreads();  // previous chunk consumed
         Idx4 q(threadIdx.x, kBlock, Rx, Ry, Rz);
-        for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, q.next()) {
-            int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
-            while (gx >= Gx) gx -= Gx;
-            while (gy >= Gy) gy -= Gy;
-            while (gz >= Gz) gz -= Gz;
-            region[i] = grid[J.grid_off + (int64_t)(cc + q.b) * J.grid_bstride +
-                             ((int64_t)gz * Gy + gy) * Gx + gx];
+        if (near) {  // one wrap per axis at most, 32-bit offsets from the job's grid
+            const float2 *gb = grid + J.grid_off;
+            const int bs = (int)J.grid_bstride;
+            for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, q.next()) {
+                int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
+                gx -= gx >= Gx ? Gx : 0;
+                gy -= gy >= Gy ? Gy : 0;
+                gz -= gz >= Gz ? Gz : 0;
+                region[i] = gb[(cc + (int)q.b) * bs + (gz * Gy + gy) * Gx + gx];
+            }
+        } else {
+            for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, q.next()) {
+                int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
+                while (gx >= Gx) gx -= Gx;
+                while (gy >= Gy) gy -= Gy;
+                while (gz >= Gz) gz -= Gz;
+                region[i] = grid[J.grid_off + (int64_t)(cc + q.b) * J.grid_bstride +
+                                 ((int64_t)gz * Gy + gy) * Gx + gx];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -308,28 +356,9 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
             if (loc[q] < 0) continue;
             for (int b = 0; b < ncoil; ++b) {
                 const float2 *r = region + (size_t)b * cells + loc[q];
-                float re = 0.0f, im = 0.0f;
-#pragma unroll
-                for (int kz = 0; kz < WC; ++kz) {
-                    if (kz < nz) {
-#pragma unroll
-                        for (int ky = 0; ky < WC; ++ky) {
-                            if (ky < ny) {
-                                const float vzy = w[q][2][kz] * w[q][1][ky];
-                                const float2 *row = r + (kz * Ry + ky) * Rx;
-#pragma unroll
-                                for (int kx = 0; kx < WC; ++kx) {
-                                    if (kx < nx) {
-                                        const float v = vzy * w[q][0][kx];
-                                        const float2 s = row[kx];
-                                        re += v * s.x;
-                                        im += v * s.y;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
+                const float2 t = full ? staged_taps<NDIM, WC, true>(r, w[q], nx, ny, nz, Rx, Ry)
+                                      : staged_taps<NDIM, WC, false>(r, w[q], nx, ny, nz, Rx, Ry);
+                const float re = t.x, im = t.y;
                 samp[J.samp_off + (int64_t)(cc + b) * J.samp_bstride + pnt[q]] =
                     make_float2(re * scl[q], im * scl[q]);
             }
