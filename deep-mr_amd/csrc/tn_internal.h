@@ -35,7 +35,7 @@ constexpr int kMatrixLongRow = 128;  // default: rows with more entries get a wh
 constexpr int kMatrixChunk = 8;      // default coils per item (record width of the coil-minor copy)
 constexpr uint64_t kMatrixAutoBytes = 4ull << 30;  // largest gridding matrix the automatic mode builds
 constexpr int kMatrixPasses = 4;     // cell passes per item of the coil-minor matrix gridding
-constexpr int kMatrixStage = 2048;   // entries per item staged in LDS (also the longest short row)
+constexpr int kMatrixStage = 1024;   // entries per item staged in LDS (also the longest short row)
 constexpr int kMatrixPart = 2048;    // entries per workgroup of a long row; longer rows are summed
                                      // in parts (kMatrixCoils-wide partial records, folded in order)
 constexpr int kXcds = 8;          // accelerator complex dies (each with its own L2)
