@@ -215,27 +215,42 @@ __device__ __forceinline__ bool kept(int i, int N, int L) {
     return m < N;
 }
 
-// Row transforms along x, in place, one wave per grid row of length L = Gx.
-// Forward: reads only the Nx data columns (the others are zeros). Inverse:
-// writes only the Nx columns the crop keeps.
+// Row transforms along x, in place, rows_per_wave(L) grid rows (length
+// L = Gx) per wave, transformed together (independent loads in flight,
+// shared twiddles; two rows up to L = 512, one above, where the LDS lines
+// would cost occupancy). Forward: reads only the Nx data columns (the others
+// are zeros). Inverse: writes only the Nx columns the crop keeps.
+constexpr int rows_per_wave(int L) { return L <= 512 ? 2 : 1; }
+
 template <int L, bool INV>
 __global__ __launch_bounds__(kBlock) void fft_rows_kernel(float2 *__restrict__ data, int64_t nlines, int Nx,
                                                           const float2 *__restrict__ tw) {
-    __shared__ float2 lds[kBlock / kWave][L];
+    constexpr int RL = rows_per_wave(L);
+    __shared__ float2 lds[kBlock / kWave * RL][L];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int64_t line = (int64_t)blockIdx.x * (kBlock / kWave) + wave;
-    if (line >= nlines) return;  // no barriers below
-    float2 *row = data + line * L;
-    float2 *const buf[1] = {lds[wave]};
+    const int64_t line0 = ((int64_t)blockIdx.x * (kBlock / kWave) + wave) * RL;
+    if (line0 >= nlines) return;  // no barriers below
+    // a row past the last one reads the wave's first row and writes nothing
+    float2 *row[RL];
+    bool ok[RL];
+    float2 *buf[RL];
+#pragma unroll
+    for (int l = 0; l < RL; ++l) {
+        ok[l] = line0 + l < nlines;
+        row[l] = data + (ok[l] ? line0 + l : line0) * L;
+        buf[l] = lds[wave * RL + l];
+    }
     if (!INV) {
-        fft_lines<L, 1, false>(
-            lane, tw, buf, [&](int, int i) { return kept(i, Nx, L) ? row[i] : make_float2(0.0f, 0.0f); },
-            [&](int, int i, float2 v) { row[i] = v; });
+        fft_lines<L, RL, false>(
+            lane, tw, buf, [&](int l, int i) { return kept(i, Nx, L) ? row[l][i] : make_float2(0.0f, 0.0f); },
+            [&](int l, int i, float2 v) {
+                if (ok[l]) row[l][i] = v;
+            });
     } else {
-        fft_lines<L, 1, true>(
-            lane, tw, buf, [&](int, int i) { return row[i]; },
-            [&](int, int i, float2 v) {
-                if (kept(i, Nx, L)) row[i] = v;
+        fft_lines<L, RL, true>(
+            lane, tw, buf, [&](int l, int i) { return row[l][i]; },
+            [&](int l, int i, float2 v) {
+                if (ok[l] && kept(i, Nx, L)) row[l][i] = v;
             });
     }
 }
@@ -465,7 +480,8 @@ int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, i
                        (const float2 *)plan->d_fft_tw[1]);
     TN_HIP(hipGetLastError());
     const int64_t nlines = nimg * Gy;
-    const int64_t rb = (nlines + kBlock / kWave - 1) / (kBlock / kWave);
+    const int64_t rpb = (int64_t)rows_per_wave(Gx) * (kBlock / kWave);  // rows per block
+    const int64_t rb = (nlines + rpb - 1) / rpb;
     hipLaunchKernelGGL(fx.row_fwd, dim3((unsigned)rb), dim3(kBlock), 0, stream, grid, nlines, Nx,
                        (const float2 *)plan->d_fft_tw[0]);
     TN_HIP(hipGetLastError());
@@ -479,7 +495,8 @@ int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
     const FftFns fy = fft_fns(Gy), fx = fft_fns(Gx);
     const int64_t nlines = nimg * Gy;
-    const int64_t rb = (nlines + kBlock / kWave - 1) / (kBlock / kWave);
+    const int64_t rpb = (int64_t)rows_per_wave(Gx) * (kBlock / kWave);  // rows per block
+    const int64_t rb = (nlines + rpb - 1) / rpb;
     hipLaunchKernelGGL(fx.row_inv, dim3((unsigned)rb), dim3(kBlock), 0, stream, grid, nlines, Nx,
                        (const float2 *)plan->d_fft_tw[0]);
     TN_HIP(hipGetLastError());
