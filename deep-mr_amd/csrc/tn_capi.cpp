@@ -242,12 +242,14 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         if (gmode == kGridMatrix) {
             // Items are runs of consecutive cells of one frame (unit = first
             // cell, e1 = cell count | entry count << 16, slot = first entry,
-            // e0 = coil chunk): at most kBlock / cw * kMatrixPasses cells
-            // (kBlock for the lane-per-cell kernel) and at most kMatrixStage
-            // entries, so that an item's entries (contiguous in the CSR order)
-            // fit the kernel's LDS stage. Rows longer than mlong end the run and
-            // get a workgroup of their own (part 1, unit = cell); those are
-            // queued first.
+            // e0 = coil chunk): at most mat_cpi cells (what the coil-minor
+            // kernel of the table's record width assumes; kBlock for the
+            // lane-per-cell kernel) and at most kMatrixStage entries, so that
+            // an item's entries (contiguous in the CSR order) fit the kernel's
+            // LDS stage. Rows longer than mlong end the run and are summed by
+            // workgroups of their own in parts of kMatrixPart entries (part 1,
+            // unit = cell, e1 = part); rows of several parts leave partial
+            // records folded in part order. Long rows are queued first.
             const int mc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, kMatrixChunk), kMatrixCoils);
             max_gc = std::max(max_gc, mc);
             const int cw = mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16;
