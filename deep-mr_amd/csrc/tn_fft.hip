@@ -284,18 +284,31 @@ __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__re
     int mx0;
     if (!column_block(nimg, Nx, C, b, mx0)) return;
     const int hy = Ny / 2, hx = Nx / 2;
-    for (int i = threadIdx.x; i < Ny * C; i += kBlock) {
-        const int my = i / C, c = i % C;
-        const int mx = mx0 + c;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (mx < Nx) {
-            v = img[(b * Ny + my) * (int64_t)Nx + mx];
-            if (apx) v = cscale(v, apx[mx]);
-            if (apy) v = cscale(v, apy[my]);
+    for (int i0 = threadIdx.x; i0 < Ny * C; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
+        float2 v[kLoadBatch];
+        float ax[kLoadBatch], ay[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int i = i0 + u * kBlock;
+            const int my = i / C, mx = mx0 + i % C;
+            v[u] = make_float2(0.0f, 0.0f);
+            ax[u] = ay[u] = 1.0f;
+            if (i < Ny * C && mx < Nx) {
+                v[u] = img[(b * Ny + my) * (int64_t)Nx + mx];
+                if (apx) ax[u] = apx[mx];
+                if (apy) ay[u] = apy[my];
+            }
         }
-        int y = my - hy;
-        if (y < 0) y += L;
-        lds[c * LP + y] = v;
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i >= Ny * C) continue;
+            const int my = i / C, c = i % C;
+            const float2 w = cscale(cscale(v[u], ax[u]), ay[u]);  // x then y, as the separate pre-pass
+            int y = my - hy;
+            if (y < 0) y += L;
+            lds[c * LP + y] = w;
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -336,16 +349,24 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
     int mx0;
     if (!column_block(nimg, Nx, C, b, mx0)) return;
     const int hy = Ny / 2, hx = Nx / 2;
-    for (int i = threadIdx.x; i < L * C; i += kBlock) {
-        const int y = i / C, c = i % C;
-        const int mx = mx0 + c;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (mx < Nx) {
-            int x = mx - hx;
-            if (x < 0) x += Gx;
-            v = grid[(b * L + y) * (int64_t)Gx + x];
+    for (int i0 = threadIdx.x; i0 < L * C; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int i = i0 + u * kBlock;
+            const int y = i / C, mx = mx0 + i % C;
+            v[u] = make_float2(0.0f, 0.0f);
+            if (i < L * C && mx < Nx) {
+                int x = mx - hx;
+                if (x < 0) x += Gx;
+                v[u] = grid[(b * L + y) * (int64_t)Gx + x];
+            }
         }
-        lds[c * LP + y] = v;
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i < L * C) lds[(i % C) * LP + i / C] = v[u];
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;

@@ -19,6 +19,7 @@ constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
 constexpr size_t kGridLdsBudget = 80 * 1024;  // LDS per gridding workgroup (two per CU)
 constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation workgroup
 constexpr int kStagedEpt = 2;  // samples per lane in the staged interpolation
+constexpr int kLoadBatch = 4;  // global loads per lane issued before their LDS writes in staging loops
 constexpr int kInterpSegment = 256 * kStagedEpt;  // samples per staged work item
 constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
 constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch

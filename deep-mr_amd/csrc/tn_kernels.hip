@@ -329,26 +329,30 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
     for (int cc = it.c0; cc < it.c1; cc += chunk) {
         const int ncoil = min(chunk, it.c1 - cc);
         __syncthreads();  // previous chunk consumed
+        // kLoadBatch independent loads per lane in flight before their LDS writes
         Idx4 q(threadIdx.x, kBlock, Rx, Ry, Rz);
-        if (near) {  // one wrap per axis at most, 32-bit offsets from the job's grid
-            const float2 *gb = grid + J.grid_off;
-            const int bs = (int)J.grid_bstride;
-            for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, q.next()) {
+        const int total = ncoil * cells;
+        for (int i0 = threadIdx.x; i0 < total; i0 += kLoadBatch * kBlock) {
+            float2 v[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u, q.next()) {
+                if (i0 + u * kBlock >= total) continue;
                 int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
-                gx -= gx >= Gx ? Gx : 0;
-                gy -= gy >= Gy ? Gy : 0;
-                gz -= gz >= Gz ? Gz : 0;
-                region[i] = gb[(cc + (int)q.b) * bs + (gz * Gy + gy) * Gx + gx];
+                if (near) {  // one wrap per axis at most, 32-bit offsets from the job's grid
+                    gx -= gx >= Gx ? Gx : 0;
+                    gy -= gy >= Gy ? Gy : 0;
+                    gz -= gz >= Gz ? Gz : 0;
+                    v[u] = grid[J.grid_off + ((cc + (int)q.b) * (int)J.grid_bstride + (gz * Gy + gy) * Gx + gx)];
+                } else {
+                    while (gx >= Gx) gx -= Gx;
+                    while (gy >= Gy) gy -= Gy;
+                    while (gz >= Gz) gz -= Gz;
+                    v[u] = grid[J.grid_off + (int64_t)(cc + q.b) * J.grid_bstride + ((int64_t)gz * Gy + gy) * Gx + gx];
+                }
             }
-        } else {
-            for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, q.next()) {
-                int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
-                while (gx >= Gx) gx -= Gx;
-                while (gy >= Gy) gy -= Gy;
-                while (gz >= Gz) gz -= Gz;
-                region[i] = grid[J.grid_off + (int64_t)(cc + q.b) * J.grid_bstride +
-                                 ((int64_t)gz * Gy + gy) * Gx + gx];
-            }
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (i0 + u * kBlock < total) region[i0 + u * kBlock] = v[u];
         }
         __syncthreads();
 #pragma unroll
@@ -1104,7 +1108,15 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
         const int rb = it.slot;
         const int nent = min(it.e1 >> 16, kMatrixStage);  // <= kMatrixStage by construction
         for (int i = threadIdx.x; i <= ncells; i += kBlock) srow[i] = min(max(J.mrow[cell0 + i] - rb, 0), nent);
-        for (int i = threadIdx.x; i < nent; i += kBlock) sent[i] = __builtin_nontemporal_load(mnz + rb + i);
+        for (int i0 = threadIdx.x; i0 < nent; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
+            i32x2 m[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (i0 + u * kBlock < nent) m[u] = __builtin_nontemporal_load(mnz + rb + i0 + u * kBlock);
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (i0 + u * kBlock < nent) sent[i0 + u * kBlock] = m[u];
+        }
         __syncthreads();
         for (int c = g; c < ncells; c += CPP) {
             float re = 0.0f, im = 0.0f;
