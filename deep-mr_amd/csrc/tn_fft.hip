@@ -380,16 +380,28 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
             [&](int l, int i, float2 v) { col[l][i] = v; });
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < Ny * C; i += kBlock) {
-        const int my = i / C, c = i % C;
-        const int mx = mx0 + c;
-        if (mx < Nx) {
-            int y = my - hy;
-            if (y < 0) y += L;
-            float2 v = cscale(lds[c * LP + y], scale);
-            if (apx) v = cscale(v, apx[mx]);
-            if (apy) v = cscale(v, apy[my]);
-            img[(b * Ny + my) * (int64_t)Nx + mx] = v;
+    for (int i0 = threadIdx.x; i0 < Ny * C; i0 += kLoadBatch * kBlock) {  // apodization loads first
+        float ax[kLoadBatch], ay[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int i = i0 + u * kBlock;
+            const int my = i / C, mx = mx0 + i % C;
+            ax[u] = ay[u] = 1.0f;
+            if (i < Ny * C && mx < Nx) {
+                if (apx) ax[u] = apx[mx];
+                if (apy) ay[u] = apy[my];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int i = i0 + u * kBlock;
+            const int my = i / C, c = i % C, mx = mx0 + c;
+            if (i < Ny * C && mx < Nx) {
+                int y = my - hy;
+                if (y < 0) y += L;
+                const float2 v = cscale(cscale(cscale(lds[c * LP + y], scale), ax[u]), ay[u]);
+                img[(b * Ny + my) * (int64_t)Nx + mx] = v;
+            }
         }
     }
 }
