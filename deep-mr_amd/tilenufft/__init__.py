@@ -5,10 +5,12 @@ Modules mirror deepmr's layout for the NUFFT path:
   tilenufft._interp   plan_interpolator, apply_interpolation, apply_gridding
   tilenufft.linops    NUFFTOp, NUFFTAdjointOp, SenseOp, SenseAdjointOp, algebra
   tilenufft.optim     CGStep, cg_solve
+  tilenufft.calib     find_acs (non-Cartesian calibration through nufft_adj)
+  tilenufft.dcf       voronoi, analytical_dcf, angular_compensation
 The compute path is the native library libtilenufft.so (HIP, gfx950).
 """
 
-from . import _interp, _signal, autotune, batch, fft, linops, optim, recon, serve, testing  # noqa: F401
+from . import _interp, _signal, autotune, batch, calib, dcf, fft, linops, optim, recon, serve, testing  # noqa: F401
 from ._lib import version  # noqa: F401
 from ._signal import resize  # noqa: F401
 
