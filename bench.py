@@ -15,6 +15,7 @@ C/OpenMP restatement of deepmr's Numba CPU loops) on a bounded sample.
 """
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -53,16 +54,26 @@ def make_spiral_workload(device, nframes=8, ncoils=16, n=256, nsamp=2048, osf=2.
 
 
 def time_kernel(fn, reps, device):
-    """Average duration of fn() from HIP events recorded on the launch stream."""
+    """Average duration of fn() from HIP events recorded on the launch stream.
+    Python's cyclic garbage collector is run before and held off during the
+    timed launches: freeing device objects of earlier work inside the timed
+    region would stall the host between an event and its launch."""
     stream = torch.cuda.current_stream(device)
     fn()
     torch.cuda.synchronize(device)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-    for s, e in ev:
-        s.record(stream)
-        fn()
-        e.record(stream)
-    torch.cuda.synchronize(device)
+    gc.collect()
+    was_enabled = gc.isenabled()
+    gc.disable()
+    try:
+        for s, e in ev:
+            s.record(stream)
+            fn()
+            e.record(stream)
+        torch.cuda.synchronize(device)
+    finally:
+        if was_enabled:
+            gc.enable()
     return float(np.mean([s.elapsed_time(e) for s, e in ev]))
 
 
@@ -270,11 +281,14 @@ def main():
         run()
     torch.cuda.synchronize(dev)
     log(f"bench: timing {args.steps} steps")
+    gc.collect()
+    gc.disable()  # no collector pauses between the timed replays
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run()
     torch.cuda.synchronize(dev)
     elapsed = time.perf_counter() - t0
+    gc.enable()
     ms_step = elapsed / args.steps * 1e3
 
     # per-kernel durations on the launch stream (the forward output in the

@@ -168,7 +168,8 @@ struct tn_plan {
     std::vector<int32_t> h_bin_off;
     std::vector<int32_t> h_mrow;      // host copy of d_mrow (work-list construction)
     // matrix-gridding runs per frame, keyed by the cells-per-item limit
-    // (built once per plan; launch tables of any coil count reuse them)
+    // (built once per plan; launch tables of any coil count reuse them; the
+    // long-row threshold is the plan's grid_split, fixed for its lifetime)
     mutable std::map<int, std::vector<std::vector<tn::MatrixRun>>> mruns;
     float *d_coord = nullptr;
     float4 *d_entries = nullptr;
