@@ -205,7 +205,9 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             if (mc < 2) inplace = true;
             cwmax = std::max(cwmax, mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16);
         }
-        if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / cwmax * kMatrixPasses);
+        // lanes per cell: one per coil, except 16-coil records (8 lanes of 2 coils)
+        const int lanes = cwmax == 16 ? 8 : cwmax;
+        if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / lanes * kMatrixPasses);
     }
     std::vector<Item> interp, grid, fold, tr;
     std::vector<int64_t> slot_base;

@@ -33,7 +33,7 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2, kGridMatrix = 3 };
 constexpr int kMatrixCoils = 16;  // coils summed per lane by the matrix gridding
 constexpr int kMatrixLongRow = 128;  // default: rows with more entries get a whole workgroup
-constexpr int kMatrixChunk = 8;      // default coils per item (record width of the coil-minor copy)
+constexpr int kMatrixChunk = 16;     // default coils per item (record width of the coil-minor copy)
 constexpr uint64_t kMatrixAutoBytes = 4ull << 30;  // largest gridding matrix the automatic mode builds
 constexpr int kMatrixPasses = 4;     // cell passes per item of the coil-minor matrix gridding
 constexpr int kMatrixStage = 1024;   // entries per item staged in LDS (also the longest short row)

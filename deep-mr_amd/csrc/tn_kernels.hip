@@ -1045,32 +1045,35 @@ __global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__res
     }
 }
 
-// Matrix gridding over the coil-minor sample copy: one lane per (cell,
-// coil), the CW lanes of a cell reading one contiguous CW-coil record per
-// entry (one cache line for 16 coils instead of one line per coil). Items
-// as for grid_matrix_kernel (it.e0 = chunk index). Part 0, a run of
+// Matrix gridding over the coil-minor sample copy: CW lanes per cell, each
+// lane summing CPL adjacent coils, so that the cell's lanes read one
+// contiguous RC = CW * CPL coil record per entry (one cache line for 16
+// coils instead of one line per coil; with CPL = 2 each lane reads 16 bytes).
+// Items as for grid_matrix_kernel (it.e0 = chunk index). Part 0, a run of
 // (it.e1 & 0xffff) cells from it.unit whose entries (it.e1 >> 16 of them,
 // at most kMatrixStage, contiguous in the CSR order from it.slot) and row
 // offsets are first staged in LDS with independent coalesced loads, so that
 // the only dependent global loads left are the sample records; kBlock / CW
-// cells per pass, each row four entries at a time. Part 1 strides one long row's entries over the cell groups of the
-// workgroup and reduces the group sums with a fixed butterfly per wave,
-// then across the waves in wave order.
-template <int LCW>
+// cells per pass, each row four entries at a time. Part 1 strides one long
+// row's entries over the cell groups of the workgroup and reduces the group
+// sums with a fixed butterfly per wave, then across the waves in wave order.
+template <int LCW, int CPL>
 __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,
                                                                const float2 *__restrict__ st,
                                                                float2 *__restrict__ grid,
                                                                float2 *__restrict__ parts) {
-    constexpr int CW = 1 << LCW;
+    static_assert(CPL == 1 || CPL == 2, "one or two coils per lane");
+    constexpr int CW = 1 << LCW;      // lanes per cell
+    constexpr int RC = CW * CPL;      // coils per record
     constexpr int CPP = kBlock / CW;  // cells per pass
     constexpr int U = 4;              // entries in flight per lane
     // cells per item (the host builds items of at most this many cells for
     // this record width); the item's sums are staged per coil plane in LDS
     // and stored as contiguous runs (row pitch CPI + 1 against bank conflicts)
     constexpr int CPI = CPP * kMatrixPasses < kBlock ? CPP * kMatrixPasses : kBlock;
-    __shared__ float2 red[kBlock / kWave][CW];
-    __shared__ float2 sout[CW * (CPI + 1)];
+    __shared__ float2 red[kBlock / kWave][RC];
+    __shared__ float2 sout[RC * (CPI + 1)];
     __shared__ i32x2 sent[kMatrixStage];
     __shared__ int srow[kBlock + 1];
     const int64_t idx = xcd_item(nitems);
@@ -1078,27 +1081,40 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     const Item it = items[idx];
     const Job &J = jobs[it.job];
     const int j = threadIdx.x & (CW - 1), g = threadIdx.x >> LCW;
-    const float2 *sq = st + J.st_off + (int64_t)it.e0 * J.npts * CW + j;
-    const bool store = it.c0 + j < it.c1;
-    float2 *gout = grid + J.grid_off + (int64_t)(it.c0 + (store ? j : 0)) * J.grid_bstride;
+    // this lane's coils j * CPL .. j * CPL + CPL - 1 of the record
+    const float2 *sq = st + J.st_off + (int64_t)it.e0 * J.npts * RC + j * CPL;
     const i32x2 *mnz = (const i32x2 *)J.mnz;
+    struct Acc {
+        float re[CPL], im[CPL];
+    };
     // entries e0, e0 + stride, ... below r1 of one row (from `ent`), U at a time
-    auto row_sum = [&](const i32x2 *ent, bool global, int e0, int r1, int stride, float &re, float &im) {
+    auto row_sum = [&](const i32x2 *ent, bool global, int e0, int r1, int stride, Acc &a) {
         for (int e = e0; e < r1; e += U * stride) {
             i32x2 m[U];
-            float2 v[U];
+            float2 v[U][CPL];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int ei = min(e + u * stride, r1 - 1);
                 m[u] = global ? __builtin_nontemporal_load(ent + ei) : ent[ei];
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = sq[(int64_t)m[u].x * CW];
+            for (int u = 0; u < U; ++u) {
+                if constexpr (CPL == 2) {
+                    const float4 r = *(const float4 *)(sq + (int64_t)m[u].x * RC);
+                    v[u][0] = make_float2(r.x, r.y);
+                    v[u][1] = make_float2(r.z, r.w);
+                } else {
+                    v[u][0] = sq[(int64_t)m[u].x * RC];
+                }
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const float w = e + u * stride < r1 ? __int_as_float(m[u].y) : 0.0f;
-                re += w * v[u].x;
-                im += w * v[u].y;
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) {
+                    a.re[k] += w * v[u][k].x;
+                    a.im[k] += w * v[u][k].y;
+                }
             }
         }
     };
@@ -1119,12 +1135,15 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
         }
         __syncthreads();
         for (int c = g; c < ncells; c += CPP) {
-            float re = 0.0f, im = 0.0f;
-            row_sum(sent, false, srow[c], srow[c + 1], 1, re, im);
-            sout[j * (CPI + 1) + c] = make_float2(re, im);
+            Acc a;
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) a.re[k] = a.im[k] = 0.0f;
+            row_sum(sent, false, srow[c], srow[c + 1], 1, a);
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) sout[(j * CPL + k) * (CPI + 1) + c] = make_float2(a.re[k], a.im[k]);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < CW * CPI; i += kBlock) {
+        for (int i = threadIdx.x; i < RC * CPI; i += kBlock) {
             const int jj = i / CPI, c = i - jj * CPI;
             if (c < ncells && it.c0 + jj < it.c1) {
                 const float2 a = sout[jj * (CPI + 1) + c];
@@ -1137,27 +1156,36 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     }
     const int64_t cell = it.unit;
     const int r0 = J.mrow[cell] + it.e1 * kMatrixPart, r1 = min(J.mrow[cell + 1], r0 + kMatrixPart);
-    float re = 0.0f, im = 0.0f;
-    row_sum(mnz, true, r0 + g, r1, CPP, re, im);
+    Acc a;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) a.re[k] = a.im[k] = 0.0f;
+    row_sum(mnz, true, r0 + g, r1, CPP, a);
 #pragma unroll
     for (int d = CW; d < kWave; d <<= 1) {
-        re += __shfl_xor(re, d, kWave);
-        im += __shfl_xor(im, d, kWave);
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+            a.re[k] += __shfl_xor(a.re[k], d, kWave);
+            a.im[k] += __shfl_xor(a.im[k], d, kWave);
+        }
     }
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    if (lane < CW) red[wave][lane] = make_float2(re, im);
+    if (lane < CW) {
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) red[wave][lane * CPL + k] = make_float2(a.re[k], a.im[k]);
+    }
     __syncthreads();
-    if ((int)threadIdx.x < CW && store) {
-        float2 a = red[0][j];
+    const int c = threadIdx.x;  // coil of the record
+    if (c < RC && it.c0 + c < it.c1) {
+        float2 s = red[0][c];
 #pragma unroll
         for (int w = 1; w < kBlock / kWave; ++w) {
-            a.x += red[w][j].x;
-            a.y += red[w][j].y;
+            s.x += red[w][c].x;
+            s.y += red[w][c].y;
         }
         if (it.slot < 0)
-            gout[cell] = a;
+            grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + cell] = s;
         else
-            parts[((int64_t)it.slot + it.e1) * kMatrixCoils + j] = a;
+            parts[((int64_t)it.slot + it.e1) * kMatrixCoils + c] = s;
     }
 }
 
@@ -1428,10 +1456,10 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         TrFn tr = nullptr;
         MtFn mt = nullptr;
         switch (t.st_cw) {
-            case 2: tr = samp_transpose_kernel<1>; mt = grid_matrix_t_kernel<1>; break;
-            case 4: tr = samp_transpose_kernel<2>; mt = grid_matrix_t_kernel<2>; break;
-            case 8: tr = samp_transpose_kernel<3>; mt = grid_matrix_t_kernel<3>; break;
-            case 16: tr = samp_transpose_kernel<4>; mt = grid_matrix_t_kernel<4>; break;
+            case 2: tr = samp_transpose_kernel<1>; mt = grid_matrix_t_kernel<1, 1>; break;
+            case 4: tr = samp_transpose_kernel<2>; mt = grid_matrix_t_kernel<2, 1>; break;
+            case 8: tr = samp_transpose_kernel<3>; mt = grid_matrix_t_kernel<3, 1>; break;
+            case 16: tr = samp_transpose_kernel<4>; mt = grid_matrix_t_kernel<3, 2>; break;  // 8 lanes x 2 coils
             default: return fail(TN_E_ARG, "matrix gridding: %d coils per record", t.st_cw);
         }
         if (t.n_tr > 0) {

@@ -71,10 +71,11 @@ typedef struct tn_plan_params {
                              tap and sample (entries spread over footprint
                              origins), 2 per-lane runs of origin-sorted
                              entries summed in registers, one z plane of the
-                             footprint per pass (W_y * W_x <= 36), 3 one lane
-                             per grid cell over a sparse gridding matrix built
-                             with the plan (CSR by cell, 8 bytes per sample
-                             and tap; grid_chunk = coils per lane, <= 16)   */
+                             footprint per pass (W_y * W_x <= 36), 3 per-cell
+                             sums over a sparse gridding matrix built with
+                             the plan (CSR by cell, 8 bytes per sample and
+                             tap) reading coil-minor sample records;
+                             grid_chunk = coils per record, <= 16 (0: 16)  */
     float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
     const float *apod[3]; /* host apodization vectors of length shape[d], or
                              NULL for ones (src/deepmr/fft/nufft.py:423-447)   */
