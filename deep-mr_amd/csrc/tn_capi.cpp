@@ -196,7 +196,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     bool st_none = false;
     // matrix gridding: record width of the coil-minor copy (the widest chunk
     // of the table) and the cells per item the kernel assumes for it
-    int mat_cpi = kBlock;
+    int mat_cpi = kBlock, mat_cpl = 1;
     if (gmode == kGridMatrix) {
         int cwmax = 0;
         bool inplace = false;
@@ -205,9 +205,11 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             if (mc < 2) inplace = true;
             cwmax = std::max(cwmax, mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16);
         }
-        // lanes per cell: one per coil, except 16-coil records (8 lanes of 2 coils)
-        const int lanes = cwmax == 16 ? 8 : cwmax;
-        if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / lanes * kMatrixPasses);
+        // coils per lane: two for 16-coil records, and for 4-8 coil records
+        // in 2D (half the gather instructions); one for 4-8 coil records in
+        // 3D, where twice the cells per wave measured slower (divergence)
+        mat_cpl = (cwmax == 16 || (cwmax >= 4 && t.kind_grid / 16 == 2)) ? 2 : 1;
+        if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / (cwmax / mat_cpl) * kMatrixPasses);
     }
     std::vector<Item> interp, grid, fold, tr;
     std::vector<int64_t> slot_base;
@@ -328,6 +330,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             off += (int64_t)((jobs[ji].nbatch + mc - 1) / mc) * jobs[ji].npts * st_cw;
         }
         t.st_cw = st_cw;
+        t.st_cpl = mat_cpl;
         t.st_elems = off;
         t.n_tr = (int64_t)tr.size();
         TN_HIP(hipMalloc(&t.d_tr, sizeof(Item) * std::max<size_t>(tr.size(), 1)));

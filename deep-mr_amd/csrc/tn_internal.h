@@ -140,6 +140,7 @@ struct LaunchTables {
     float2 *d_st = nullptr;
     int64_t st_elems = 0;
     int st_cw = 0;  // 0: the lane-per-cell matrix kernel reads the samples in place
+    int st_cpl = 1; // coils per lane of the matrix kernel over the copy (1 or 2)
 };
 
 }  // namespace tn

@@ -1455,13 +1455,25 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         using MtFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, float2 *);
         TrFn tr = nullptr;
         MtFn mt = nullptr;
+        const bool pair = t.st_cpl == 2;  // two coils per lane (the table's cells per item assume it)
         switch (t.st_cw) {
             case 2: tr = samp_transpose_kernel<1>; mt = grid_matrix_t_kernel<1, 1>; break;
-            case 4: tr = samp_transpose_kernel<2>; mt = grid_matrix_t_kernel<2, 1>; break;
-            case 8: tr = samp_transpose_kernel<3>; mt = grid_matrix_t_kernel<3, 1>; break;
-            case 16: tr = samp_transpose_kernel<4>; mt = grid_matrix_t_kernel<3, 2>; break;  // 8 lanes x 2 coils
+            case 4:
+                tr = samp_transpose_kernel<2>;
+                mt = pair ? grid_matrix_t_kernel<1, 2> : grid_matrix_t_kernel<2, 1>;
+                break;
+            case 8:
+                tr = samp_transpose_kernel<3>;
+                mt = pair ? grid_matrix_t_kernel<2, 2> : grid_matrix_t_kernel<3, 1>;
+                break;
+            case 16:
+                tr = samp_transpose_kernel<4>;
+                mt = grid_matrix_t_kernel<3, 2>;  // 8 lanes x 2 coils
+                break;
             default: return fail(TN_E_ARG, "matrix gridding: %d coils per record", t.st_cw);
         }
+        if (t.st_cw == 16 ? !pair : (pair && t.st_cw < 4))
+            return fail(TN_E_ARG, "matrix gridding: %d coils per lane for %d-coil records", t.st_cpl, t.st_cw);
         if (t.n_tr > 0) {
             hipLaunchKernelGGL(tr, dim3((unsigned)t.n_tr), dim3(kBlock), 0, stream, t.d_jobs, t.d_tr, samp, weight,
                                t.d_st);
