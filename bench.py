@@ -247,7 +247,8 @@ def main():
         os.environ["TILENUFFT_CACHE"] = args.tune_cache
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the NUFFT path "
-              "shards by frame/coil/request without communication, see DESIGN.md", flush=True)
+              "shards by frame, coil group or request with one process and one set of plans per GPU, and needs "
+              "no communication except a per-iteration image all-reduce for coil-split CG-SENSE", flush=True)
         sys.exit(2)
 
     import tilenufft
