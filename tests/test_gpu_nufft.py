@@ -302,9 +302,10 @@ def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf
 
 @pytest.mark.parametrize("chunk", [1, 3, 8, 16])
 def test_grid_matrix_coil_chunks(chunk):
-    """Matrix gridding with 1..16 coils per lane over 21 coils (partial last
-    items repeat their last coil's loads without storing them) and a density
-    weight, against the oracle."""
+    """Matrix gridding with 1..16 coils per item over 21 coils (records of 4,
+    8 and 16 coils summed two coils per lane in 2D; partial last items hold
+    zero coils that are not stored) and a density weight, against the
+    oracle."""
     n = 32
     print(f"grid matrix chunk={chunk}", flush=True)
     coord = testing.vd_spiral(8, 200, n)
@@ -332,12 +333,12 @@ def test_grid_matrix_multiframe_plan_and_empty_frame_rows():
     assert rel(a.cpu().numpy(), ref) < RTOL
 
 
-@pytest.mark.parametrize("ncoil, chunk", [(8, 0), (3, 0), (2, 1)])
+@pytest.mark.parametrize("ncoil, chunk", [(16, 0), (8, 0), (3, 0), (2, 1)])
 def test_grid_matrix_long_rows_in_parts(ncoil, chunk):
     """Cells with more entries than one workgroup part (3000 samples on one
     spot next to a uniform background): the rows are summed in parts whose
-    partials are folded in order; coil-minor (chunks of 8 and 3 coils) and
-    lane-per-cell (1 coil per item) kernels, against the oracle, twice."""
+    partials are folded in order; coil-minor (chunks of 16, 8 and 3 coils)
+    and lane-per-cell (1 coil per item) kernels, against the oracle, twice."""
     n = 16
     print(f"grid matrix long rows coils={ncoil} chunk={chunk}", flush=True)
     rng = np.random.default_rng(57)
@@ -352,6 +353,27 @@ def test_grid_matrix_long_rows_in_parts(ncoil, chunk):
     torch.cuda.synchronize()
     assert torch.equal(a.view(torch.float32), b.view(torch.float32))
     ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0))
+    assert rel(a.cpu().numpy(), ref) < RTOL
+
+
+@pytest.mark.parametrize("chunk, split", [(6, 32), (16, 32), (16, 0)])
+def test_grid_matrix_3d_records_and_parts(chunk, split):
+    """3D matrix gridding over 11 coils: 8-coil records (one coil per lane in
+    3D) and 16-coil records (two coils per lane), with the dense kooshball
+    centre summed in parts when rows exceed 32 entries, against the oracle,
+    twice (bitwise)."""
+    n = 16
+    print(f"grid matrix 3D chunk={chunk} split={split}", flush=True)
+    coord = testing.kooshball(300, 32, n)
+    k = cnormal((11,) + coord.shape[:-1], 59)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 1.5, device=DEV, grid_mode=3, grid_chunk=chunk,
+                                    grid_split=split)
+    kt = torch.from_numpy(k).to(DEV)
+    a = tilenufft.fft.apply_nufft_adj(kt, plan)
+    b = tilenufft.fft.apply_nufft_adj(kt, plan)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+    ref = orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 1.5))
     assert rel(a.cpu().numpy(), ref) < RTOL
 
 
