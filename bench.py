@@ -220,6 +220,42 @@ def kooshball_bench():
     print(json.dumps(out), flush=True)
 
 
+def cgsense_bench(steps=5):
+    """Unrolled CG-SENSE with autograd: 256^2 image, 8 coils, 201 golden-angle
+    radial spokes x 512 samples, KB width 4, osf 1.25 (320^2 grid), 10 CGStep
+    iterations from zero, MSE loss against a seeded target, backward to the
+    k-space. One step = forward + backward, run eagerly; its duration comes
+    from HIP events on the launch stream. The encoding operator (plans) is
+    built once, outside the timed steps."""
+    import tilenufft
+    from tilenufft import testing
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, nc, nspokes, nsamp, niter = 256, 8, 201, 512, 10
+    coord = testing.golden_angle_radial(nspokes, nsamp, n)
+    smaps = torch.from_numpy(testing.gaussian_coil_maps(nc, (n, n), seed=0)).to(dev)
+    gen = torch.Generator().manual_seed(0)
+    y = testing.complex_normal((nc, nspokes, nsamp), generator=gen).to(dev)
+    target = (testing.complex_normal((n, n), generator=gen) * 0.1).to(dev)
+    t0 = time.perf_counter()
+    model = tilenufft.recon.UnrolledCGSENSE(torch.from_numpy(coord), (n, n), smaps, niter=niter, device=dev)
+    torch.cuda.synchronize(dev)
+    plan_s = time.perf_counter() - t0
+
+    def step():
+        yy = y.detach().requires_grad_(True)
+        loss = (model(yy) - target).abs().pow(2).mean()
+        loss.backward()
+
+    log("bench: CG-SENSE 10 iterations, forward + backward")
+    t = time_kernel(step, steps, dev)
+    out = {"workload": "unrolled CG-SENSE, 256^2, 8 coils, 201 radial spokes x 512, osf 1.25, W 4, "
+                       "10 iterations, forward + backward (autograd) per step",
+           "ms_per_step": t, "ms_per_iteration_fwd_bwd": t / niter, "plan_s": plan_s, "steps": steps}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,9 +275,15 @@ def main():
     ap.add_argument("--kooshball", action="store_true",
                     help="instead: time 3D kooshball gridding / interpolation (128^3, 8 coils, 20000 x 256, "
                          "W 4, osf 2.0 and 1.5) with register runs and the gridding matrix; one JSON line")
+    ap.add_argument("--cgsense", action="store_true",
+                    help="instead: time 10 unrolled CG-SENSE iterations forward + backward (256^2, 8 coils, "
+                         "201 radial spokes x 512, osf 1.25); one JSON line")
     args = ap.parse_args()
     if args.kooshball:
         kooshball_bench()
+        return
+    if args.cgsense:
+        cgsense_bench()
         return
     if args.tune_cache:
         os.environ["TILENUFFT_CACHE"] = args.tune_cache
