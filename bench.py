@@ -220,13 +220,14 @@ def kooshball_bench():
     print(json.dumps(out), flush=True)
 
 
-def cgsense_bench(steps=5):
+def cgsense_bench(steps=10, warmup=3):
     """Unrolled CG-SENSE with autograd: 256^2 image, 8 coils, 201 golden-angle
     radial spokes x 512 samples, KB width 4, osf 1.25 (320^2 grid), 10 CGStep
     iterations from zero, MSE loss against a seeded target, backward to the
-    k-space. One step = forward + backward, run eagerly; its duration comes
-    from HIP events on the launch stream. The encoding operator (plans) is
-    built once, outside the timed steps."""
+    k-space. One step = forward + backward, timed eagerly and then with the
+    module captured in HIP graphs (capture_unrolled); durations come from
+    HIP events on the launch stream after three untimed steps. The encoding
+    operator (plans) is built once, outside the timed steps."""
     import tilenufft
     from tilenufft import testing
 
@@ -248,11 +249,31 @@ def cgsense_bench(steps=5):
         loss = (model(yy) - target).abs().pow(2).mean()
         loss.backward()
 
-    log("bench: CG-SENSE 10 iterations, forward + backward")
-    t = time_kernel(step, steps, dev)
+    def result():
+        yy = y.detach().requires_grad_(True)
+        x = model(yy)
+        (x - target).abs().pow(2).mean().backward()
+        return x.detach().clone(), yy.grad.detach().clone()
+
+    log("bench: CG-SENSE 10 iterations, forward + backward (eager)")
+    for _ in range(warmup):  # first steps grow the allocator's pools and lay out the autograd buffers
+        step()
+    torch.cuda.synchronize(dev)
+    t_eager = time_kernel(step, steps, dev)
+    x_ref, g_ref = result()
+    log("bench: CG-SENSE captured in HIP graphs (forward and backward)")
+    tilenufft.recon.capture_unrolled(model, y.detach().clone().requires_grad_(True))
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    t_graph = time_kernel(step, steps, dev)
+    x_g, g_g = result()
+    rel = lambda a, b: float((a - b).abs().norm() / b.abs().norm())
     out = {"workload": "unrolled CG-SENSE, 256^2, 8 coils, 201 radial spokes x 512, osf 1.25, W 4, "
                        "10 iterations, forward + backward (autograd) per step",
-           "ms_per_step": t, "ms_per_iteration_fwd_bwd": t / niter, "plan_s": plan_s, "steps": steps}
+           "ms_per_step_eager": t_eager, "ms_per_step_graphed": t_graph,
+           "ms_per_iteration_fwd_bwd_graphed": t_graph / niter, "plan_s": plan_s, "steps": steps,
+           "warmup": warmup, "graphed_vs_eager_rel_l2": {"image": rel(x_g, x_ref), "grad": rel(g_g, g_ref)}}
     print(json.dumps(out), flush=True)
 
 

@@ -10,7 +10,7 @@ gradients flow to the k-space (and to anything upstream of it) through the
 hand-written forward / adjoint kernels.
 """
 
-__all__ = ["encoding_op", "UnrolledCGSENSE", "cg_sense"]
+__all__ = ["encoding_op", "UnrolledCGSENSE", "cg_sense", "capture_unrolled"]
 
 import torch
 import torch.nn as nn
@@ -78,3 +78,22 @@ class UnrolledCGSENSE(nn.Module):
 
     def forward(self, kspace):
         return cg_sense(kspace, self.E, self.EHE, self.niter, self.lamda, self.dcf)
+
+
+def capture_unrolled(model, sample_kspace, warmup=3):
+    """Capture ``model``'s forward and backward (an UnrolledCGSENSE, or any
+    module over the same operators) into HIP graphs with
+    torch.cuda.make_graphed_callables and return it: the module's forward
+    is replaced in place by the graphed one, so later calls replay the
+    captured kernels instead of launching them one by one from Python.
+
+    ``sample_kspace`` fixes what later calls must match: shape, dtype,
+    device and requires_grad. The native launches take no host
+    synchronisation and record no memsets (plans and launch tables are built
+    before capture), so the captured work is the same kernels on the same
+    buffers. Capture the model on the stream it will run on; do not rebuild
+    its plans afterwards."""
+    if not isinstance(sample_kspace, torch.Tensor) or sample_kspace.device.type != "cuda":
+        raise ValueError("sample_kspace must be a HIP device tensor")
+    sample = sample_kspace.detach().clone().requires_grad_(sample_kspace.requires_grad)
+    return torch.cuda.make_graphed_callables(model, (sample,), num_warmup_iters=warmup)

@@ -130,3 +130,39 @@ def test_unrolled_cgsense_grad_vs_deepmr_wiring():
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
     assert rel(grads["deepmr"], g_ref) < 1e-3
     assert rel(grads["exact"] * nos, g_ref) < 1e-3
+
+
+def test_unrolled_cgsense_graph_replays_equal_eager():
+    """capture_unrolled: the HIP-graph forward and backward of the unrolled
+    CG-SENSE give the eager module's image and k-space gradient on every
+    replay. The replays take different k-space, and eager NUFFT work and
+    host synchronisations run between them."""
+    print("unrolled CG-SENSE graph capture: replays vs eager", flush=True)
+    n, coord, smaps, y, target, _ = _cgsense_problem()
+    tt = torch.from_numpy(target.astype(np.complex64)).to(DEV)
+
+    def make():
+        return tilenufft.recon.UnrolledCGSENSE(torch.from_numpy(coord), (n, n),
+                                               torch.from_numpy(smaps).to(DEV), niter=10, device=DEV)
+
+    def run(model, yv):
+        yy = yv.detach().clone().requires_grad_(True)
+        x = model(yy)
+        (x - tt).abs().pow(2).mean().backward()
+        return x.detach().clone(), yy.grad.detach().clone()
+
+    ys = [torch.from_numpy((y * s + cn(y.shape, 20 + k) * 0.05).astype(np.complex64)).to(DEV)
+          for k, s in enumerate((1.0, 0.5, -2.0))]
+    eager = make()
+    refs = [run(eager, yv) for yv in ys]
+    graphed = tilenufft.recon.capture_unrolled(make(), ys[0].clone().requires_grad_(True))
+    side_plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV)
+    side_img = torch.from_numpy(cn((2, n, n), 30).astype(np.complex64)).to(DEV)
+    for rep in range(2):
+        for k, (yv, (x_ref, g_ref)) in enumerate(zip(ys, refs)):
+            x, g = run(graphed, yv)
+            tilenufft.fft.apply_nufft_adj(tilenufft.fft.apply_nufft(side_img, side_plan), side_plan)
+            torch.cuda.synchronize()
+            ex = float((x - x_ref).abs().norm() / x_ref.abs().norm())
+            eg = float((g - g_ref).abs().norm() / g_ref.abs().norm())
+            assert ex < 1e-6 and eg < 1e-6, (rep, k, ex, eg)
