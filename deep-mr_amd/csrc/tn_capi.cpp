@@ -231,8 +231,8 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             const int64_t f = (j.pbin_off - pl->d_pbin_off) / std::max<int64_t>(pl->nitiles_total, 1);
             const int32_t *off = pl->h_pbin_off.data() + f * pl->nitiles_total;
             for (int64_t tl = 0; tl < nt; ++tl) {
-                for (int32_t e0 = off[tl]; e0 < off[tl + 1]; e0 += kInterpSegment) {
-                    const int32_t e1 = std::min<int32_t>(off[tl + 1], e0 + kInterpSegment);
+                for (int32_t e0 = off[tl]; e0 < off[tl + 1]; e0 += interp_segment(j.ndim)) {
+                    const int32_t e1 = std::min<int32_t>(off[tl + 1], e0 + interp_segment(j.ndim));
                     interp.push_back(Item{(int)ji, (int)tl, 0, j.nbatch, e0, e1, 0, 0});
                 }
             }

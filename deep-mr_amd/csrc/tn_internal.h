@@ -18,9 +18,11 @@ constexpr int kMaxWidth = 16;      // largest kernel width supported per axis
 constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
 constexpr size_t kGridLdsBudget = 80 * 1024;  // LDS per gridding workgroup (two per CU)
 constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation workgroup
-constexpr int kStagedEpt = 2;  // samples per lane in the staged interpolation
+// samples per lane in the staged interpolation: 1 in 2D, 2 in 3D (each
+// measured fastest on its workload; both appear in one table only alone)
+constexpr int staged_ept(int ndim) { return ndim == 3 ? 2 : 1; }
 constexpr int kLoadBatch = 4;  // global loads per lane issued before their LDS writes in staging loops
-constexpr int kInterpSegment = 256 * kStagedEpt;  // samples per staged work item
+constexpr int interp_segment(int ndim) { return 256 * staged_ept(ndim); }  // samples per staged work item
 constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
 constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch
 constexpr int kGridPad = 4;      // accumulator row padding (floats)

@@ -255,7 +255,7 @@ __device__ __forceinline__ float2 staged_taps(const float2 *r, const float (&w)[
     return make_float2(re, im);
 }
 
-// Staged interpolation: a workgroup takes one segment (<= kBlock * kStagedEpt
+// Staged interpolation: a workgroup takes one segment (<= kBlock * staged_ept(NDIM)
 // samples) of one interpolation tile. Each lane evaluates the taps of its
 // samples once and keeps them in registers; then, coil chunk by coil chunk,
 // the workgroup loads the (tile + W-1 halo) grid region of the chunk into
@@ -284,6 +284,7 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
     const int cells = Rx * Ry * Rz;
     const int nx = J.W[0], ny = NDIM > 1 ? J.W[1] : 1, nz = NDIM > 2 ? J.W[2] : 1;
 
+    constexpr int kStagedEpt = staged_ept(NDIM);
     // taps of this lane's samples, kept across coil chunks
     float w[kStagedEpt][3][WC];
     int loc[kStagedEpt];  // region offset of the footprint origin, -1 = none
