@@ -225,7 +225,8 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         if (mode == kInterpStaged) {
             // one item per (tile, segment) covers every coil; the region is
             // loaded `ic` coils at a time
-            const int ic = std::min(chunk_for(pl->p.interp_chunk, j.nbatch, region_coils), region_coils);
+            const int ic_auto = j.ndim == 3 ? std::min(region_coils, kInterpChunk3d) : region_coils;
+            const int ic = std::min(chunk_for(pl->p.interp_chunk, j.nbatch, ic_auto), region_coils);
             max_ic = std::max(max_ic, ic);
             const int64_t nt = (int64_t)j.INT_[0] * j.INT_[1] * j.INT_[2];
             const int64_t f = (j.pbin_off - pl->d_pbin_off) / std::max<int64_t>(pl->nitiles_total, 1);

@@ -22,6 +22,9 @@ constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation 
 // measured fastest on its workload; both appear in one table only alone)
 constexpr int staged_ept(int ndim) { return ndim == 3 ? 2 : 1; }
 constexpr int kLoadBatch = 4;  // global loads per lane issued before their LDS writes in staging loops
+// default coils per region load of the staged interpolation in 3D (more
+// workgroups per CU than the LDS budget allows; measured fastest)
+constexpr int kInterpChunk3d = 2;
 constexpr int interp_segment(int ndim) { return 256 * staged_ept(ndim); }  // samples per staged work item
 constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
 constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch
