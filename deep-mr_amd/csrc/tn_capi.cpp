@@ -225,7 +225,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         if (mode == kInterpStaged) {
             // one item per (tile, segment) covers every coil; the region is
             // loaded `ic` coils at a time
-            const int ic_auto = j.ndim == 3 ? std::min(region_coils, kInterpChunk3d) : region_coils;
+            const int ic_auto = std::min(region_coils, j.ndim == 3 ? kInterpChunk3d : kInterpChunk2d);
             const int ic = std::min(chunk_for(pl->p.interp_chunk, j.nbatch, ic_auto), region_coils);
             max_ic = std::max(max_ic, ic);
             const int64_t nt = (int64_t)j.INT_[0] * j.INT_[1] * j.INT_[2];
@@ -385,9 +385,9 @@ void choose_tiles(tn_plan *pl) {
     } else {
         def[0] = 16; def[1] = 16; def[2] = 16;
     }
-    int idef[3];
+    int idef[3];  // default staged-interpolation tiles (z, y, x): measured fastest with the default coil chunks
     if (p.ndim == 2) {
-        idef[0] = 1; idef[1] = 32; idef[2] = 32;
+        idef[0] = 1; idef[1] = 16; idef[2] = 16;
     } else {
         idef[0] = 8; idef[1] = 8; idef[2] = 16;
     }
