@@ -1055,10 +1055,10 @@ __global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__res
 // at most kMatrixStage, contiguous in the CSR order from it.slot) and row
 // offsets are first staged in LDS with independent coalesced loads, so that
 // the only dependent global loads left are the sample records; kBlock / CW
-// cells per pass, each row four entries at a time. Part 1 strides one long
+// cells per pass, each row U entries at a time. Part 1 strides one long
 // row's entries over the cell groups of the workgroup and reduces the group
 // sums with a fixed butterfly per wave, then across the waves in wave order.
-template <int LCW, int CPL>
+template <int LCW, int CPL, int U>
 __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,
                                                                const float2 *__restrict__ st,
@@ -1068,7 +1068,6 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     constexpr int CW = 1 << LCW;      // lanes per cell
     constexpr int RC = CW * CPL;      // coils per record
     constexpr int CPP = kBlock / CW;  // cells per pass
-    constexpr int U = 4;              // entries in flight per lane
     // cells per item (the host builds items of at most this many cells for
     // this record width); the item's sums are staged per coil plane in LDS
     // and stored as contiguous runs (row pitch CPI + 1 against bank conflicts)
@@ -1457,19 +1456,26 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         TrFn tr = nullptr;
         MtFn mt = nullptr;
         const bool pair = t.st_cpl == 2;  // two coils per lane (the table's cells per item assume it)
+        // entries in flight per lane: 4 in 2D, 8 in 3D (each measured fastest)
+        const bool d3 = t.kind_grid / 16 == 3;
         switch (t.st_cw) {
-            case 2: tr = samp_transpose_kernel<1>; mt = grid_matrix_t_kernel<1, 1>; break;
+            case 2:
+                tr = samp_transpose_kernel<1>;
+                mt = d3 ? grid_matrix_t_kernel<1, 1, 8> : grid_matrix_t_kernel<1, 1, 4>;
+                break;
             case 4:
                 tr = samp_transpose_kernel<2>;
-                mt = pair ? grid_matrix_t_kernel<1, 2> : grid_matrix_t_kernel<2, 1>;
+                mt = pair ? (d3 ? grid_matrix_t_kernel<1, 2, 8> : grid_matrix_t_kernel<1, 2, 4>)
+                          : (d3 ? grid_matrix_t_kernel<2, 1, 8> : grid_matrix_t_kernel<2, 1, 4>);
                 break;
             case 8:
                 tr = samp_transpose_kernel<3>;
-                mt = pair ? grid_matrix_t_kernel<2, 2> : grid_matrix_t_kernel<3, 1>;
+                mt = pair ? (d3 ? grid_matrix_t_kernel<2, 2, 8> : grid_matrix_t_kernel<2, 2, 4>)
+                          : (d3 ? grid_matrix_t_kernel<3, 1, 8> : grid_matrix_t_kernel<3, 1, 4>);
                 break;
             case 16:
-                tr = samp_transpose_kernel<4>;
-                mt = grid_matrix_t_kernel<3, 2>;  // 8 lanes x 2 coils
+                tr = samp_transpose_kernel<4>;  // 8 lanes x 2 coils
+                mt = d3 ? grid_matrix_t_kernel<3, 2, 8> : grid_matrix_t_kernel<3, 2, 4>;
                 break;
             default: return fail(TN_E_ARG, "matrix gridding: %d coils per record", t.st_cw);
         }
