@@ -53,6 +53,16 @@ def make_spiral_workload(device, nframes=8, ncoils=16, n=256, nsamp=2048, osf=2.
                 n=n, ncoils=ncoils, osf=osf, width=width, nsamp=nsamp)
 
 
+def variant_key(plan, kind):
+    """Launch variant of a workload's dominant kernel, as the traffic file keys it."""
+    if kind == "grid":
+        return (f"grid|mode={plan.grid_mode}|tile={list(plan.interpolator.tile())}|chunk={plan.grid_chunk}"
+                f"|split={plan.grid_split}")
+    if kind == "interp":
+        return f"interp|mode={plan.interp_mode}|itile={plan.itile}|chunk={plan.interp_chunk}"
+    return kind
+
+
 def time_kernel(fn, reps, device):
     """Average duration of fn() from HIP events recorded on the launch stream.
     Python's cyclic garbage collector is run before and held off during the
@@ -121,6 +131,61 @@ def cpu_baseline(wl, batch, frame=0, reps=5, warm=1, budget_s=30.0):
         orc.set_fft_backend("numpy")
 
 
+def config1_bench(dev, reps=10, warm=2):
+    """Plain 2D single-coil NUFFT forward + adjoint: 128x128 image (complex
+    N(0,1)), golden-angle radial 128 spokes x 256 samples in [-64, 64), osf
+    2.0, KB width 4. Times the oracle's -ffast-math C/OpenMP restatement of
+    deepmr's CPU loops with torch's CPU FFT on every host thread and on one
+    thread (deepmr's gridding of one coil and frame runs on one thread:
+    its parallel loop is over frame x coil), and the GPU path (plan reused)
+    with HIP events. Medians after warm-up; plan time separate."""
+    import tilenufft
+    from oracle import nufft_oracle as orc
+    from tilenufft import testing
+
+    n = 128
+    coord = testing.golden_angle_radial(128, 256, n)
+    gen = torch.Generator().manual_seed(0)
+    x = testing.complex_normal((1, n, n), generator=gen).numpy()
+    nsamp = 128 * 256
+    out = {"workload": "2D single-coil radial 128 spokes x 256, 128x128, osf 2.0, KB width 4, fwd+adj",
+           "samples": nsamp}
+    nproc = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lib = orc.load_loops(fast=True)
+    orc.use_fast_loops(True)
+    orc.set_fft_backend("torch")
+    try:
+        t0 = time.perf_counter()
+        plan = orc.plan_nufft(coord, n, 4, 2.0)
+        out["cpu_plan_s"] = time.perf_counter() - t0
+        for threads in (nproc, 1):
+            lib.orc_set_threads(threads)
+            torch.set_num_threads(threads)
+            ts = []
+            for r in range(warm + reps):
+                t0 = time.perf_counter()
+                y = orc.apply_nufft(x, plan)
+                orc.apply_nufft_adj(y, plan)
+                if r >= warm:
+                    ts.append(time.perf_counter() - t0)
+            med = float(np.median(ts))
+            key = "cpu_1thread" if threads == 1 else "cpu"
+            out[key] = {"ms": med * 1e3, "samples_per_s": nsamp / med, "cores": threads}
+    finally:
+        lib.orc_set_threads(nproc)
+        torch.set_num_threads(nproc)
+        orc.use_fast_loops(False)
+        orc.set_fft_backend("numpy")
+    gplan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=dev)
+    xt = torch.from_numpy(x).to(dev)
+    yk = tilenufft.fft.apply_nufft(xt, gplan)
+    t = time_kernel(lambda: tilenufft.fft.apply_nufft_adj(tilenufft.fft.apply_nufft(xt, gplan), gplan), reps, dev)
+    out["gpu"] = {"ms": t, "samples_per_s": nsamp / (t * 1e-3)}
+    y_ref = orc.apply_nufft(x, orc.plan_nufft(coord, n, 4, 2.0))
+    out["gpu_vs_oracle_rel_l2"] = float(np.linalg.norm(yk.cpu().numpy() - y_ref) / np.linalg.norm(y_ref))
+    return out
+
+
 def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
     """Mixed-size CG-SENSE stream (Poisson arrivals; N in {128,192,256,320},
     4..32 coils, radial or spiral, 10k..200k samples per coil, 10 CG
@@ -173,51 +238,48 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
     )
 
 
-def kooshball_bench():
+def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
     """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
-    256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3); gridding
-    with register runs (tile 16^3) and with the gridding matrix (long rows above
-    256 entries), staged interpolation (tile 8x8x16). Durations from HIP events
-    on the launch stream; algorithmic bytes as for the 2D workload."""
+    256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3): the
+    automatic plan's gridding (tile matrix) and staged interpolation, and with
+    ``compare_runs`` the register-run gridding too. Durations from HIP events
+    on the launch stream; algorithmic bytes as for the 2D workload. Returns a
+    dict."""
     import tilenufft
     from tilenufft import testing
     from tilenufft.batch import BatchedNUFFT
     from tilenufft.fft.nufft import retune
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     coord = testing.kooshball(20000, 256, 128)
     out = {"workload": "3D kooshball 128^3, 8 coils, 20000 spokes x 256 samples, KB width 4", "cases": []}
     gen = torch.Generator().manual_seed(0)
-    for osf in (2.0, 1.5):
+    for osf in osfs:
         log(f"bench: kooshball osf {osf}")
-        plan = tilenufft.fft.plan_nufft(coord, 128, 4, osf, device=dev, tile=(16, 16, 16), itile=(8, 8, 16),
-                                        interp_mode=2, grid_mode=2)
-        case = {"osf": osf, "grid": list(plan.os_shape)}
-        for name, gm, split in (("runs", 2, 0), ("matrix", 3, 256)):
-            if gm != 2:
-                t0 = time.perf_counter()
-                retune(plan, grid_mode=gm, grid_split=split)
-                torch.cuda.synchronize(dev)
-                case["matrix_plan_s"] = time.perf_counter() - t0
-                case["matrix_plan_bytes"] = plan.interpolator.device_bytes()
+        t0 = time.perf_counter()
+        plan = tilenufft.fft.plan_nufft(coord, 128, 4, osf, device=dev)
+        torch.cuda.synchronize(dev)
+        case = {"osf": osf, "grid": list(plan.os_shape), "plan_s": time.perf_counter() - t0,
+                "plan_bytes": plan.interpolator.device_bytes()}
+        b = BatchedNUFFT([plan], 8)
+        b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(dev))
+        b_interp, b_grid = b.algorithmic_bytes()
+        case["grid_alg_bytes"], case["interp_alg_bytes"] = b_grid, b_interp
+        t = time_kernel(lambda: b.gridding(b.grid), reps, dev)
+        case["grid_ms"] = t
+        case["grid_frac"] = b_grid / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
+        ti = time_kernel(lambda: b.interp(b.grid), reps, dev)
+        case["interp_ms"] = ti
+        case["interp_frac"] = b_interp / (ti * 1e-3) / 1e9 / HBM_PEAK_GBS
+        b = None
+        if compare_runs:
+            retune(plan, grid_mode=2, tile=(16, 16, 16))
             b = BatchedNUFFT([plan], 8)
-            b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(dev))
-            t = time_kernel(lambda: b.gridding(b.grid), 3, dev)
-            b_interp, b_grid = b.algorithmic_bytes()
-            case[f"grid_{name}_ms"] = t
-            case[f"grid_{name}_frac"] = b_grid / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
-            if gm == 2:
-                ti = time_kernel(lambda: b.interp(b.grid), 3, dev)
-                case["interp_ms"] = ti
-                case["interp_frac"] = b_interp / (ti * 1e-3) / 1e9 / HBM_PEAK_GBS
-                case["grid_alg_bytes"], case["interp_alg_bytes"] = b_grid, b_interp
+            case["grid_runs_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)
             b = None
-            torch.cuda.empty_cache()
         out["cases"].append(case)
         plan = None
         torch.cuda.empty_cache()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def cgsense_bench(steps=10, warmup=3):
@@ -287,6 +349,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--no-config1", action="store_true", help="skip the plain single-coil 2D leg")
+    ap.add_argument("--no-kooshball", action="store_true", help="skip the 3D kooshball kernel leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only replay the step and time the kernels (short runs under a profiler)")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--tune-cache", default=None, help="directory of the autotuner's cache file")
@@ -301,7 +367,9 @@ def main():
                          "201 radial spokes x 512, osf 1.25); one JSON line")
     args = ap.parse_args()
     if args.kooshball:
-        kooshball_bench()
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps(kooshball_bench(dev, compare_runs=True)), flush=True)
         return
     if args.cgsense:
         cgsense_bench()
@@ -382,18 +450,22 @@ def main():
     times = dict(grid=t_grid, interp=t_interp, **t_fft)
     dominant = max(times, key=times.get)
     ach = {"grid": gbs_grid, "interp": gbs_interp}.get(dominant, b_fft / (times[dominant] * 1e-3) / 1e9)
+    # HBM bytes per launch of the timed variant, from rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (scaled by a copy kernel's counters);
+    # null when the autotuner picked a variant that was not profiled
+    variant = variant_key(plans[0], dominant)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_spiral2d.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dominant)
+            traffic = json.load(f).get("variants", {}).get(variant)
 
     total_samples = batch.nsamples
     value = total_samples / (ms_step * 1e-3)
 
     cpu = None
     parity = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.kernels_only:
         log("bench: CPU baseline (oracle restatement of deepmr's CPU loops) on frame 0")
         cpu, parity = cpu_baseline(wl, batch)
 
@@ -406,8 +478,16 @@ def main():
         torch.cuda.synchronize(dev)
         del src, dst
 
+    config1 = None
+    if not args.no_config1 and not args.kernels_only:
+        log("bench: plain single-coil 2D radial (CPU oracle on all threads / one thread, GPU)")
+        config1 = config1_bench(dev)
+    kooshball = None
+    if not args.no_kooshball and not args.kernels_only:
+        kooshball = kooshball_bench(dev)
+
     serving = None
-    if not args.no_serve:
+    if not args.no_serve and not args.kernels_only:
         log(f"bench: serving stream ({args.serve_requests} requests) served vs plain")
         serving = serving_bench(dev, args.serve_requests, args.serve_rate)
 
@@ -443,6 +523,7 @@ def main():
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
             "traffic": traffic,
+            "variant": variant,
         },
         "kernels": {
             "interp_ms": t_interp,
@@ -460,6 +541,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "parity_rel_l2_frame0_vs_oracle": parity,
+        "config1_plain_2d": config1,
+        "kooshball_3d": kooshball,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -497,6 +497,7 @@ int build_fft(tn_plan *plan) {
             tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
         }
         TN_HIP(hipMalloc(&plan->d_fft_tw[a], sizeof(float2) * L));
+        plan->fft_tw_bytes += (int64_t)sizeof(float2) * L;
         TN_HIP(hipMemcpy(plan->d_fft_tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
     }
     return TN_OK;

@@ -66,6 +66,10 @@ _SIGNATURES = {
     "tn_batch_destroy": (c_int32, [c_void_p]),
     "tn_batch_interp": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "tn_batch_grid": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tn_sense_expand": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "tn_sense_combine": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    "tn_cg_work_floats": (c_int64, [c_int64, c_int64]),
+    "tn_cg_step": (c_int32, [c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     "tn_last_error": (ctypes.c_char_p, []),
     "tn_version": (ctypes.c_char_p, []),
 }

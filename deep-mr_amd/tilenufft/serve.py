@@ -5,46 +5,54 @@ of CG iterations) arrive over time. The server advances every active request
 by one CG-SENSE iteration per step, and all active requests share the
 launches of that step:
 
-  coil expansion -> per-geometry apod+pad -> per-geometry batched rocFFT ->
-  ONE interpolation launch over every active request (ragged work queue) ->
-  ONE gridding launch over every active request -> per-geometry batched
-  inverse rocFFT -> per-geometry crop+apod -> coil combination -> CG update
+  SENSE expansion of each request's CG direction (one launch per image size)
+  -> fused apodize / pad / pruned FFT per geometry group -> ONE interpolation
+  launch over every active request (ragged work queue) -> ONE gridding launch
+  -> pruned inverse FFT / crop / apodize per group -> SENSE combination and
+  CG update per group (three launches)
 
 Requests are admitted at step boundaries while a slot is free and the HBM
-budget covers their resident state (plan / sorted trajectory, coil maps,
-k-space, CG vectors and their share of the step arenas); finished requests
-leave and free their budget, so new arrivals join while others are mid-solve.
-Plans are kept resident and reused for repeated trajectories (``traj_key``).
-A request is planned as soon as it is submitted, by a worker thread on its
-own HIP stream, so planning (host arithmetic, binning and sorting on the
-device) overlaps the steps of the requests already running; its k-space and
-coil maps are copied to the device without blocking at admission.
+budget covers what they hold on the device (plan and launch tables, coil
+maps, k-space, CG vectors and their share of the step arenas, counted from
+the library's own figures); finished requests leave and free their budget,
+so new arrivals join while others are mid-solve. Plans are kept resident and
+reused for repeated trajectories (``traj_key``). A request is planned as soon
+as it is submitted, by a worker thread on its own HIP stream, so planning
+overlaps the steps of the requests already running.
 
 The CG recursion is deepmr's CGStep (src/deepmr/optim/cg.py:194-212) on
 E^H E x = E^H y with E = NUFFT * SENSE (src/deepmr/recon/alg/linop.py:108-148),
-without density compensation.
+without density compensation. The coil combination and the CG update are the
+native kernels of ``tilenufft.sense``, shared with ``run_plain``: their
+reductions have a fixed order that depends only on the image size, the
+interpolation and gridding of a request do not depend on which other
+requests share a launch, and the FFT stages transform every coil image on
+its own, so a request's result is the same bits alone, in any mix, and
+through the plain one-request-at-a-time path.
 
-The per-request interpolation and gridding results do not depend on which
-other requests share a launch (each work item is one request's sample block
-or tile); only the batched FFT may pick another algorithm, so a result alone
-and in a mix agree to float32 round-off.
+Nothing in a step waits for the device: finished requests record an event
+(their completion time is read from it), and plans and launch tables of
+finished requests are released only when the device is idle, because
+freeing device memory synchronizes.
 """
 
-__all__ = ["ReconRequest", "ReconServer", "run_plain", "estimate_request_bytes", "latency_stats",
-           "make_stream"]
+__all__ = ["ReconRequest", "ReconServer", "run_plain", "estimate_request_bytes", "request_bytes",
+           "latency_stats", "make_stream"]
 
 import collections
 import concurrent.futures
+import ctypes
 import math
 import threading
 import time
-from dataclasses import dataclass, field
+import weakref
+from dataclasses import dataclass
 
 import numpy as np
 import torch
 
-from . import _core, fft as _fft
-from .batch import BatchedNUFFT
+from . import _core, _lib, fft as _fft
+from .sense import CGState, SenseGroup, START, ITERATE
 
 
 @dataclass
@@ -83,15 +91,34 @@ class ReconRequest:
         self.cancelled = True
 
 
-def estimate_request_bytes(n, ncoils, npts, oversamp=1.25, dup=1.35):
-    """Device bytes a request holds while active."""
+def _plan_model_bytes(npts, g, width=4):
+    """Device bytes of a 2D plan and its launch tables before it exists:
+    coordinates (8 B), interpolation entries (16 B) and gridding tile bins
+    (16 B, ~1.5 copies) per sample, the gridding matrix (2 B per tap and
+    sample), its row offsets (2 B per cell and batch), the tile offsets and
+    the launch tables."""
+    taps = width * width
+    return int(npts * (8 + 16 + 24 + 2 * taps) + 4 * g * g + (1 << 20))
+
+
+def estimate_request_bytes(n, ncoils, npts, oversamp=1.25, plan_bytes=None):
+    """Device bytes a request holds while active: its plan (``plan_bytes``
+    from the library when the plan exists, else a model), CG state (x, r, p,
+    ap), coil maps and k-space, and its rows of the step arenas (coil images,
+    oversampled grids, k-space)."""
     g = math.ceil(np.float32(oversamp) * n)
-    plan = 8 * npts + int(16 * dup * npts) + 4 * 512
-    state = 8 * (ncoils * n * n  # coil maps
-                 + ncoils * npts  # measured k-space
-                 + 5 * n * n)  # x, r, p, AHAp, b
-    arenas = 8 * (ncoils * n * n + 2 * ncoils * g * g + ncoils * npts)
+    plan = _plan_model_bytes(npts, g) if plan_bytes is None else int(plan_bytes)
+    state = 8 * (4 * n * n + ncoils * n * n + ncoils * npts) + 8
+    arenas = 8 * (ncoils * n * n + ncoils * g * g + ncoils * npts)
     return plan + state + arenas
+
+
+def request_bytes(req, plan):
+    """``estimate_request_bytes`` with the plan's own device bytes (sorted
+    trajectory, bins, gridding matrix) plus its launch tables in a batch."""
+    tables = 8 * req.ncoils * req.npts + (1 << 20)
+    return estimate_request_bytes(req.shape, req.ncoils, req.npts, plan.oversamp[-1],
+                                  plan_bytes=plan.interpolator.device_bytes() + tables)
 
 
 class _Active:
@@ -100,16 +127,17 @@ class _Active:
     def __init__(self, req, plan, device):
         self.req = req
         self.plan = plan
-        # pinned host tensors copy without blocking the host
-        self.smaps = req.smaps.to(device, non_blocking=True).to(torch.complex64).contiguous()
-        self.y = req.kspace.to(device, non_blocking=True).to(torch.complex64).reshape(req.ncoils, -1).contiguous()
-        n = req.shape
-        self.x = torch.zeros((n, n), dtype=torch.complex64, device=device)
-        self.r = None
-        self.p = None
-        self.rs_old = None
-        self.it = -1  # -1: E^H y pending, then 0..niter-1
+        self.state = CGState(req.smaps.to(device, non_blocking=True), device)
+        self.y = req.kspace.to(device, non_blocking=True).to(torch.complex64).reshape(-1).contiguous()
         self.bytes = 0
+
+    @property
+    def it(self):
+        return self.state.it
+
+
+def _geometry(plan):
+    return (tuple(plan.shape), tuple(plan.os_shape), tuple(plan.width), tuple(np.float32(plan.beta)))
 
 
 class ReconServer:
@@ -132,11 +160,13 @@ class ReconServer:
         self._planner = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         self._plan_stream = torch.cuda.Stream(device=self.device)  # default priority
         self._pending = {}  # id(req) -> future of (plan, ready event)
-        self._layout_key = None
         self._layout = None
+        self._retired = []  # plans / launch tables of finished requests, freed when idle
+        self._events = []   # (request, completion event) awaiting their time stamp
         self.steps = 0
         self.stats = dict(admitted=0, rejected=0, deferred_for_memory=0, deferred_for_slots=0)
         self.t0 = time.perf_counter()
+        self._ev0 = None
 
     # ---- admission -----------------------------------------------------------
     def now(self):
@@ -147,17 +177,23 @@ class ReconServer:
         self._pending[id(req)] = self._planner.submit(self._prepare, req)
 
     def _prepare(self, req):
-        """Validate and plan a request on the planning stream (worker thread)."""
+        """Validate, plan and size a request on the planning stream (worker thread)."""
         with torch.cuda.device(self.device), torch.cuda.stream(self._plan_stream):
             self._validate(req)
+            need = estimate_request_bytes(req.shape, req.ncoils, req.npts, self.oversamp)
+            if need > self.budget:  # hopeless even before planning
+                return None, None, need
             plan = self._plan_for(req)
+            need = request_bytes(req, plan)
             ev = torch.cuda.Event()
             ev.record(self._plan_stream)
-        return plan, ev
+        return plan, ev, need
 
     def close(self):
-        """Stop the planning worker (pending plans are finished first)."""
+        """Stop the planning worker (pending plans are finished first) and
+        release retired device memory."""
         self._planner.shutdown(wait=True)
+        self._release()
 
     def _plan_for(self, req):
         key = req.traj_key
@@ -172,18 +208,28 @@ class ReconServer:
             with self._plan_lock:
                 self._plans[key] = plan
                 while len(self._plans) > self._plan_cache:
-                    self._plans.popitem(last=False)
+                    self._retired.append(self._plans.popitem(last=False)[1])
         return plan
 
     def _validate(self, req):
         if req.coord.ndim < 2 or req.coord.shape[-1] != 2:
             raise ValueError("coord must be (..., 2)")
+        if req.npts < 1:
+            raise ValueError("the trajectory holds no samples")
         if tuple(req.smaps.shape) != (req.ncoils, req.shape, req.shape):
             raise ValueError(f"smaps shape {tuple(req.smaps.shape)} != ({req.ncoils}, {req.shape}, {req.shape})")
         if req.kspace.numel() != req.ncoils * req.npts:
             raise ValueError("k-space does not match the trajectory")
         if not np.isfinite(req.coord).all():
             raise ValueError("coordinates contain NaN or Inf")
+        if req.niter < 0:
+            raise ValueError("niter must be >= 0")
+
+    def _reject(self, req, msg):
+        self.stats["rejected"] += 1
+        req.error = msg
+        req.t_done = self.now()
+        self.done.append(req)
 
     def _admit(self):
         admitted = False
@@ -198,114 +244,86 @@ class ReconServer:
                 req.t_done = self.now()
                 self.done.append(req)
                 continue
-            need = estimate_request_bytes(req.shape, req.ncoils, req.npts, self.oversamp)
+            fut = self._pending.get(id(req))
+            if fut is None:  # not submitted through submit()
+                fut = self._pending[id(req)] = self._planner.submit(self._prepare, req)
+            try:
+                plan, ready, need = fut.result()
+            except Exception as exc:  # a malformed request fails alone
+                self.queue.popleft()
+                self._pending.pop(id(req), None)
+                self._reject(req, f"{type(exc).__name__}: {exc}")
+                continue
             if need > self.budget:
                 self.queue.popleft()
                 self._pending.pop(id(req), None)
-                req.error = f"request needs {need} bytes, more than the HBM budget {self.budget}"
-                self.stats["rejected"] += 1
-                req.t_done = self.now()
-                self.done.append(req)
+                self._reject(req, f"request needs {need} bytes, more than the HBM budget {self.budget}")
                 continue
             if self.reserved + need > self.budget:
                 self.stats["deferred_for_memory"] += 1
                 break  # wait for memory to free up (FIFO)
             self.queue.popleft()
+            self._pending.pop(id(req), None)
+            torch.cuda.current_stream(self.device).wait_event(ready)
             try:
-                fut = self._pending.pop(id(req), None)
-                if fut is None:  # not submitted through submit()
-                    fut = self._planner.submit(self._prepare, req)
-                plan, ready = fut.result()
-                torch.cuda.current_stream(self.device).wait_event(ready)
                 st = _Active(req, plan, self.device)
-            except Exception as exc:  # a malformed request fails alone
-                self.stats["rejected"] += 1
-                req.error = f"{type(exc).__name__}: {exc}"
-                req.t_done = self.now()
-                self.done.append(req)
+            except Exception as exc:
+                self._reject(req, f"{type(exc).__name__}: {exc}")
                 continue
             st.bytes = need
             self.reserved += need
             self.stats["admitted"] += 1
             req.t_admit = self.now()
             self.active.append(st)
+            self._invalidate()
             admitted = True
         return admitted
 
     # ---- one step ------------------------------------------------------------
     def _build_layout(self):
-        """Arenas, stacked CG state and batched launch tables for the active set.
+        """Arenas, SENSE groups and batched launch tables for the active set.
 
-        Requests with the same matrix size form a group: their coil images,
-        padded grids and CG vectors are stacked so that each step costs a
-        fixed number of launches per group, and all groups share one
-        interpolation and one gridding launch.
-        """
-        self._store_state()
+        Requests with the same plan geometry (image size, oversampled grid,
+        kernel) form a group: their coil images are contiguous rows of the
+        group's image arena, so each step costs a fixed number of launches
+        per group, and all groups share one interpolation and one gridding
+        launch over the grid and k-space arenas."""
         act = sorted(self.active, key=lambda s: (s.req.shape, s.req.rid))
         dev = self.device
         groups = collections.OrderedDict()
         for s in act:
-            groups.setdefault(s.req.shape, []).append(s)
-        grid_elems = sum(s.req.ncoils * math.ceil(np.float32(self.oversamp) * s.req.shape) ** 2 for s in act)
+            groups.setdefault(_geometry(s.plan), []).append(s)
+        grid_elems = sum(s.req.ncoils * int(np.prod(s.plan.os_shape)) for s in act)
         ksp_elems = sum(s.req.ncoils * s.req.npts for s in act)
-        lay = dict(order=act, groups=[])
-        lay["gridA"] = torch.empty(grid_elems, dtype=torch.complex64, device=dev)
-        lay["gridB"] = torch.empty(grid_elems, dtype=torch.complex64, device=dev)
+        lay = dict(groups=[], order=act)
+        lay["grid"] = torch.empty(max(grid_elems, 1), dtype=torch.complex64, device=dev)
         lay["kspace"] = torch.empty(max(ksp_elems, 1), dtype=torch.complex64, device=dev)
         goff = koff = 0
-        offs = {}
-        for n, members in groups.items():
-            g = math.ceil(np.float32(self.oversamp) * n)
+        gofs, kofs = [], []
+        for key, members in groups.items():
+            plan0 = members[0].plan
+            n0, n1 = plan0.shape
+            g0, g1 = plan0.os_shape
             rows = sum(s.req.ncoils for s in members)
-            row2req = torch.cat([torch.full((s.req.ncoils,), i, dtype=torch.long)
-                                 for i, s in enumerate(members)]).to(dev)
-            init = torch.tensor([s.it < 0 for s in members], device=dev)
-            grp = dict(n=n, g=g, members=members, rows=rows, row2req=row2req, init=init,
-                       any_init=bool(init.any()), any_cg=bool((~init).any()),
-                       gslice=(goff, goff + rows * g * g),
-                       smaps=torch.cat([s.smaps for s in members]),
-                       images=torch.empty((rows, n, n), dtype=torch.complex64, device=dev))
-            grp["X"] = torch.stack([s.x for s in members])
-            grp["R"] = torch.stack([s.r if s.r is not None else s.x for s in members])
-            grp["P"] = torch.stack([s.p if s.p is not None else s.x for s in members])
-            grp["rs"] = torch.stack([s.rs_old if s.rs_old is not None else
-                                     torch.ones((), device=dev) for s in members]).reshape(-1)
+            img_off, r = [], 0
             for s in members:
-                offs[s.req.rid] = (goff, koff)
-                goff += s.req.ncoils * g * g
+                img_off.append(r * n0 * n1)
+                r += s.req.ncoils
+            grp = dict(members=members, rows=rows, native=plan0.interpolator, os_shape=(g0, g1),
+                       gslice=(goff, goff + rows * g0 * g1),
+                       images=torch.empty((rows, n0, n1), dtype=torch.complex64, device=dev),
+                       sense=SenseGroup([s.state for s in members], img_off, dev),
+                       kslices=[])
+            for s in members:
+                gofs.append(goff)
+                kofs.append(koff)
+                grp["kslices"].append((s, koff))
+                goff += s.req.ncoils * g0 * g1
                 koff += s.req.ncoils * s.req.npts
             lay["groups"].append(grp)
-        lay["offs"] = offs
-        lay["grid_batch"] = _NativeBatch([s.plan for s in act], [s.req.ncoils for s in act],
-                                         [offs[s.req.rid][0] for s in act],
-                                         [offs[s.req.rid][1] for s in act], dev)
-        cg = [s for s in act if s.it >= 0]
-        lay["interp_batch"] = (_NativeBatch([s.plan for s in cg], [s.req.ncoils for s in cg],
-                                            [offs[s.req.rid][0] for s in cg],
-                                            [offs[s.req.rid][1] for s in cg], dev) if cg else None)
+        order = [s for grp in lay["groups"] for s in grp["members"]]
+        lay["batch"] = _NativeBatch([s.plan for s in order], [s.req.ncoils for s in order], gofs, kofs, dev)
         return lay
-
-    def _store_state(self):
-        """Write the stacked CG state of the current layout back to the requests."""
-        lay = self._layout
-        if lay is None:
-            return
-        for grp in lay["groups"]:
-            for i, s in enumerate(grp["members"]):
-                if s.it < 0 or s not in self.active:
-                    continue
-                s.x, s.r, s.p = grp["X"][i].clone(), grp["R"][i].clone(), grp["P"][i].clone()
-                s.rs_old = grp["rs"][i].clone()
-        self._layout = None
-        self._layout_key = None
-
-    def _layout_for_step(self):
-        key = tuple((s.req.rid, s.it >= 0) for s in sorted(self.active, key=lambda s: (s.req.shape, s.req.rid)))
-        if key != self._layout_key:
-            self._layout = self._build_layout()
-            self._layout_key = key
-        return self._layout
 
     def step(self):
         """Admit what fits, then advance every active request by one iteration."""
@@ -314,101 +332,91 @@ class ReconServer:
             if s.req.cancelled:
                 self._finish(s, error="cancelled")
         if not self.active:
+            self._release()
             return False
-        lay = self._layout_for_step()
+        if self._layout is None:
+            if self._ev0 is None:
+                self._ev0 = torch.cuda.Event(enable_timing=True)
+                self._ev0.record(torch.cuda.current_stream(self.device))
+                self._t_ev0 = self.now()
+            self._layout = self._build_layout()
+        lay = self._layout
         with torch.cuda.device(self.device):
-            # forward NUFFT of the search directions (CG requests): coil
-            # expansion, pad + FFT per group, one interpolation launch
-            if lay["interp_batch"] is not None:
-                for grp in lay["groups"]:
-                    a, b = grp["gslice"]
-                    g = grp["g"]
-                    if grp["any_cg"]:
-                        torch.mul(grp["smaps"], grp["P"].index_select(0, grp["row2req"]), out=grp["images"])
-                    nat = grp["members"][0].plan.interpolator
-                    if nat.has_fft:
-                        nat.fft_forward(grp["images"], lay["gridB"][a:b].view(-1, g, g), grp["rows"])
-                    else:
-                        ga = lay["gridA"][a:b].view(-1, g, g)
-                        nat.apod_pad(grp["images"], ga, grp["rows"])
-                        torch.fft.fftn(ga, dim=(-2, -1), out=lay["gridB"][a:b].view(-1, g, g))
-                lay["interp_batch"].interp(lay["gridB"], lay["kspace"])
-            # measured k-space of requests starting this step
+            any_iter = False
             for grp in lay["groups"]:
-                if grp["any_init"]:
-                    for s in grp["members"]:
-                        if s.it < 0:
-                            _, ko = lay["offs"][s.req.rid]
-                            lay["kspace"][ko:ko + s.y.numel()].copy_(s.y.reshape(-1))
-            # adjoint of every request: one gridding launch, IFFT + crop per group
-            lay["grid_batch"].grid(lay["kspace"], lay["gridA"])
+                modes = [START if s.it < 0 else ITERATE for s in grp["members"]]
+                grp["sense"].set_modes(modes)
+                grp["any_iter"] = ITERATE in modes
+                any_iter |= grp["any_iter"]
+            if any_iter:
+                # forward NUFFT of the CG directions: SENSE expansion, pad +
+                # FFT per group, one interpolation launch
+                for grp in lay["groups"]:
+                    if not grp["any_iter"]:
+                        continue
+                    grp["sense"].expand(grp["images"])
+                    a, b = grp["gslice"]
+                    _fft_forward(grp["native"], grp["images"], lay["grid"][a:b].view(-1, *grp["os_shape"]),
+                                 grp["rows"])
+                lay["batch"].interp(lay["grid"], lay["kspace"])
+            # measured k-space of requests starting this step (their
+            # interpolation output, if any, is overwritten here)
+            for grp in lay["groups"]:
+                for s, ko in grp["kslices"]:
+                    if s.it < 0:
+                        lay["kspace"][ko:ko + s.y.numel()].copy_(s.y)
+            # adjoint of every request: one gridding launch, IFFT + crop per
+            # group, then coil combination and the CG update
+            lay["batch"].grid(lay["kspace"], lay["grid"])
             for grp in lay["groups"]:
                 a, b = grp["gslice"]
-                g = grp["g"]
-                nat = grp["members"][0].plan.interpolator
-                if nat.has_fft:  # gridA is scratch from here on
-                    nat.fft_adjoint(lay["gridA"][a:b].view(-1, g, g), grp["images"], grp["rows"])
-                else:
-                    gb = lay["gridB"][a:b].view(-1, g, g)
-                    torch.fft.ifftn(lay["gridA"][a:b].view(-1, g, g), dim=(-2, -1), out=gb)
-                    nat.crop_apod(gb, grp["images"], grp["rows"])
-                self._cg_update(grp)
+                _fft_adjoint(grp["native"], lay["grid"][a:b].view(-1, *grp["os_shape"]), grp["images"],
+                             grp["rows"])
+                grp["sense"].combine(grp["images"])
+                grp["sense"].cg_step()
             finished = []
-            for grp in lay["groups"]:
-                for s in grp["members"]:
-                    s.it += 1
-                    if s.it >= s.req.niter:
-                        finished.append(s)
-            if finished:
-                torch.cuda.synchronize(self.device)
-                self._store_state()
-                for s in finished:
-                    self._finish(s)
+            for s in self.active:
+                s.state.it += 1
+                if s.state.it >= s.req.niter:
+                    finished.append(s)
+            for s in finished:
+                self._finish(s)
         self.steps += 1
         return True
 
-    @staticmethod
-    def _cg_update(grp):
-        """Coil combination and one CGStep (src/deepmr/optim/cg.py:203-212) for
-        every request of the group; requests starting this step set
-        r = p = E^H y instead."""
-        nreq = len(grp["members"])
-        n = grp["n"]
-        ap = torch.zeros((nreq, n, n), dtype=torch.complex64, device=grp["images"].device)
-        ap.index_add_(0, grp["row2req"], grp["smaps"].conj() * grp["images"])
-        X, R, P, rs = grp["X"], grp["R"], grp["P"], grp["rs"]
-        if grp["any_cg"]:
-            alpha = rs / (P.conj() * ap).sum(dim=(-2, -1))
-            a3 = alpha[:, None, None]
-            x_new = X + P * a3
-            r_new = R + ap * (-a3)
-            rs_new = torch.real((r_new.conj() * r_new).sum(dim=(-2, -1)))
-            p_new = r_new + P * (rs_new / rs)[:, None, None]
-        if grp["any_init"]:
-            rs_init = torch.real((ap.conj() * ap).sum(dim=(-2, -1)))
-            if grp["any_cg"]:
-                m = grp["init"]
-                m3 = m[:, None, None]
-                x_new = torch.where(m3, torch.zeros_like(X), x_new)
-                r_new = torch.where(m3, ap, r_new)
-                p_new = torch.where(m3, ap, p_new)
-                rs_new = torch.where(m, rs_init, rs_new)
-            else:
-                x_new, r_new, p_new, rs_new = torch.zeros_like(X), ap, ap.clone(), rs_init
-        X.copy_(x_new)
-        R.copy_(r_new)
-        P.copy_(p_new)
-        rs.copy_(rs_new)
-
     def _finish(self, s, error=None):
-        if self._layout is not None:
-            self._store_state()
         self.active.remove(s)
         self.reserved -= s.bytes
-        s.req.result = None if error else s.x
-        s.req.error = error
-        s.req.t_done = self.now()
-        self.done.append(s.req)
+        req = s.req
+        req.error = error
+        if error is None:
+            req.result = s.state.x.clone()
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(torch.cuda.current_stream(self.device))
+            self._events.append((req, ev))
+        else:
+            req.t_done = self.now()
+        if req.traj_key is None:
+            self._retired.append(s.plan)
+        s.state = None
+        self.done.append(req)
+        self._invalidate()
+
+    def _invalidate(self):
+        """Drop the layout of the active set; its launch tables are freed
+        later (freeing device memory synchronizes)."""
+        if self._layout is not None:
+            self._retired.append(self._layout)
+            self._layout = None
+
+    def _release(self):
+        """Time-stamp finished requests and free retired plans and launch
+        tables (the device is idle or about to be synchronized anyway)."""
+        torch.cuda.synchronize(self.device)
+        for req, ev in self._events:
+            req.t_done = self._t_ev0 + self._ev0.elapsed_time(ev) * 1e-3 if self._ev0 is not None else self.now()
+        self._events.clear()
+        self._retired.clear()
 
     # ---- driving a stream ----------------------------------------------------
     def run(self, requests, realtime=True, on_step=None):
@@ -416,6 +424,7 @@ class ReconServer:
         with ``result`` / ``error`` and timing fields filled."""
         pending = sorted(requests, key=lambda r: r.arrival)
         self.t0 = time.perf_counter()
+        self._ev0 = None
         i = 0
         total = len(pending)
         while len(self.done) < total:
@@ -432,19 +441,29 @@ class ReconServer:
                     time.sleep(min(wait, 0.05))
             elif not progressed and not self.queue and i >= total:
                 break
-        torch.cuda.synchronize(self.device)
+        self._release()
         return requests
+
+
+def _fft_forward(native, images, grid, nimg):
+    if native.has_fft:
+        native.fft_forward(images, grid, nimg)
+    else:
+        native.apod_pad(images, grid, nimg)
+        grid.copy_(torch.fft.fftn(grid, dim=(-2, -1)))
+
+
+def _fft_adjoint(native, grid, images, nimg):
+    if native.has_fft:
+        native.fft_adjoint(grid, images, nimg)
+    else:
+        native.crop_apod(torch.fft.ifftn(grid, dim=(-2, -1)), images, nimg)
 
 
 class _NativeBatch:
     """tn_batch over arbitrary plans with explicit arena offsets."""
 
     def __init__(self, plans, ncoils, grid_off, samp_off, device):
-        import ctypes
-        import weakref
-
-        from . import _lib
-
         n = len(plans)
         lib = _lib.load()
         handles = (ctypes.c_void_p * n)(*[p.interpolator.handle.value for p in plans])
@@ -462,16 +481,45 @@ class _NativeBatch:
         self._fin = weakref.finalize(self, _destroy_batch, out.value, device)
 
     def interp(self, grid, kspace):
-        from . import _lib
-
         _lib.check(_lib.load().tn_batch_interp(self.handle, _core.ptr(grid), _core.ptr(kspace), None,
                                                _core.stream_ptr(self.device)), "tn_batch_interp")
 
     def grid(self, kspace, grid):
-        from . import _lib
-
         _lib.check(_lib.load().tn_batch_grid(self.handle, _core.ptr(kspace), _core.ptr(grid), None,
                                              _core.stream_ptr(self.device)), "tn_batch_grid")
+
+
+def cg_sense_plain(req, plan, device, replan=False, width=4, oversamp=1.25):
+    """One request's CG-SENSE through the plain NUFFT (apply_nufft /
+    apply_nufft_adj with the request's plan, or the per-call planning
+    nufft / nufft_adj when ``replan``) and the same native SENSE / CG
+    kernels as the server. Returns the image (on the device)."""
+    dev = device
+    state = CGState(req.smaps.to(dev), dev)
+    group = SenseGroup([state], [0], dev)
+    y = req.kspace.to(dev, torch.complex64).reshape(req.ncoils, *req.coord.shape[:-1])
+
+    def fwd(v):
+        if replan:
+            return _fft.nufft(v, req.coord, req.shape, device=dev, width=width, oversamp=oversamp)
+        return _fft.apply_nufft(v, plan)
+
+    def adj(k):
+        if replan:
+            return _fft.nufft_adj(k, req.coord, req.shape, device=dev, width=width, oversamp=oversamp)
+        return _fft.apply_nufft_adj(k, plan)
+
+    img = adj(y).contiguous()
+    group.set_modes([START])
+    group.combine(img)
+    group.cg_step()
+    for _ in range(req.niter):
+        group.set_modes([ITERATE])
+        group.expand(img)
+        img = adj(fwd(img)).contiguous()
+        group.combine(img)
+        group.cg_step()
+    return state.x.clone()
 
 
 def run_plain(requests, device=None, width=4, oversamp=1.25, realtime=True, replan=False):
@@ -479,10 +527,18 @@ def run_plain(requests, device=None, width=4, oversamp=1.25, realtime=True, repl
 
     Each request runs its CG-SENSE with apply_nufft / apply_nufft_adj calls
     (one request on the device at a time, plan reused across its iterations
-    unless ``replan``, where every call plans like deepmr's nufft/nufft_adj).
+    unless ``replan``, where every call plans like deepmr's nufft/nufft_adj),
+    with the server's SENSE / CG kernels. Completion times come from events
+    on the stream, as in the server.
     """
     dev = _core.compute_device(device)
     t0 = time.perf_counter()
+    stream = torch.cuda.current_stream(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    t_ev0 = time.perf_counter() - t0
+    events = []
+    plans = []
     for req in sorted(requests, key=lambda r: r.arrival):
         if realtime:
             wait = req.arrival - (time.perf_counter() - t0)
@@ -491,42 +547,19 @@ def run_plain(requests, device=None, width=4, oversamp=1.25, realtime=True, repl
         req.t_admit = time.perf_counter() - t0
         try:
             with torch.cuda.device(dev):
-                smaps = req.smaps.to(dev, torch.complex64)
-                y = req.kspace.to(dev, torch.complex64).reshape(req.ncoils, *req.coord.shape[:-1])
                 plan = None if replan else _fft.plan_nufft(req.coord, req.shape, width, oversamp, device=dev)
-
-                def fwd(v):
-                    if replan:
-                        return _fft.nufft(v, req.coord, req.shape, device=dev, width=width, oversamp=oversamp)
-                    return _fft.apply_nufft(v, plan)
-
-                def adj(k):
-                    if replan:
-                        return _fft.nufft_adj(k, req.coord, req.shape, device=dev, width=width,
-                                              oversamp=oversamp)
-                    return _fft.apply_nufft_adj(k, plan)
-
-                def EHE(v):
-                    return (smaps.conj() * adj(fwd(smaps * v))).sum(0)
-
-                b = (smaps.conj() * adj(y)).sum(0)
-                x = torch.zeros_like(b)
-                r = b
-                p = b.clone()
-                rs_old = torch.real(torch.vdot(r.reshape(-1), r.reshape(-1)))
-                for _ in range(req.niter):
-                    ap = EHE(p)
-                    alpha = rs_old / torch.vdot(p.reshape(-1), ap.reshape(-1))
-                    x = x + p * alpha
-                    r = r + ap * (-alpha)
-                    rs_new = torch.real(torch.vdot(r.reshape(-1), r.reshape(-1)))
-                    p = r + p * (rs_new / rs_old)
-                    rs_old = rs_new
-                torch.cuda.synchronize(dev)
-            req.result = x
+                req.result = cg_sense_plain(req, plan, dev, replan, width, oversamp)
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(stream)
+                events.append((req, ev))
+                plans.append(plan)  # freed after the stream has drained (freeing synchronizes)
         except Exception as exc:
             req.error = f"{type(exc).__name__}: {exc}"
-        req.t_done = time.perf_counter() - t0
+            req.t_done = time.perf_counter() - t0
+    torch.cuda.synchronize(dev)
+    for req, ev in events:
+        req.t_done = t_ev0 + ev0.elapsed_time(ev) * 1e-3
+    plans.clear()
     return requests
 
 
@@ -578,8 +611,8 @@ def make_stream(nreq, rate_hz, seed=0, sizes=(128, 192, 256, 320), coils=(4, 32)
             coord = testing.vd_spiral(nv, ns, n, rotation=float(trng.uniform(0, 2 * np.pi)))
         pin = torch.device(device).type == "cpu" and torch.cuda.is_available()
         if (c, n) not in maps_cache:
-            m = torch.from_numpy(testing.gaussian_coil_maps(c, (n, n), seed=c))
-            maps_cache[(c, n)] = m.pin_memory() if pin else m.to(device)
+            mp = torch.from_numpy(testing.gaussian_coil_maps(c, (n, n), seed=c))
+            maps_cache[(c, n)] = mp.pin_memory() if pin else mp.to(device)
         k = testing.complex_normal((c, *coord.shape[:-1]), generator=gen)
         k = k.pin_memory() if pin else k.to(device)  # pinned host data copies asynchronously
         out.append(ReconRequest(rid=i, coord=coord, kspace=k, smaps=maps_cache[(c, n)],

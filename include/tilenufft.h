@@ -71,11 +71,14 @@ typedef struct tn_plan_params {
                              tap and sample (entries spread over footprint
                              origins), 2 per-lane runs of origin-sorted
                              entries summed in registers, one z plane of the
-                             footprint per pass (W_y * W_x <= 36), 3 per-cell
-                             sums over a sparse gridding matrix built with
-                             the plan (CSR by cell, 8 bytes per sample and
-                             tap) reading coil-minor sample records;
-                             grid_chunk = coils per record, <= 16 (0: 16)  */
+                             footprint per pass (W_y * W_x <= 36), 3 tile
+                             matrix: per tile, the samples of its bin in
+                             batches of up to 256, their records staged in
+                             LDS once, every cell summing its row of a
+                             plan-time matrix of 2-byte entries (batch slot |
+                             tap code); grid_chunk = coils per record (1, 2,
+                             4, 8, 16; 0: 16), grid_split = samples per
+                             workgroup before a bin is summed in parts    */
     float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
     const float *apod[3]; /* host apodization vectors of length shape[d], or
                              NULL for ones (src/deepmr/fft/nufft.py:423-447)   */
@@ -86,8 +89,11 @@ typedef struct tn_plan_params {
  * gridding tiles its kernel footprint touches (stable, sorted by sample
  * index inside each tile) and uploads the apodization vectors.
  * Replaces plan_interpolator / _prepare_interpolator
- * (src/deepmr/fft/_interp/plan.py:14-143, :251-269); the per-axis tap
- * tables are not stored: taps are evaluated in registers at apply time. */
+ * (src/deepmr/fft/_interp/plan.py:14-143, :251-269). The per-axis tap
+ * values are not stored (deepmr's tables hold 8 * W * d bytes per sample):
+ * taps are evaluated in registers at apply time. Gridding mode 3 stores its
+ * tile matrix instead: 2 bytes per (sample, tap), 4 bytes per (tile, sample)
+ * bin slot and 2 bytes per (batch, tile cell) row start. */
 int tn_plan_create(const tn_plan_params *params, const float *coord_dev,
                    tn_stream stream, tn_plan **out);
 int tn_plan_destroy(tn_plan *plan);
@@ -201,6 +207,46 @@ int tn_batch_interp(tn_batch *batch, const void *grid_base, void *samp_base,
                     const float *weight_base, tn_stream stream);
 int tn_batch_grid(tn_batch *batch, const void *samp_base, void *grid_base,
                   const float *weight_base, tn_stream stream);
+
+/* ---- CG-SENSE of a group of reconstructions sharing the image size --------
+ * One record per reconstruction, in device memory. Every kernel handles one
+ * reconstruction per workgroup row and reduces over its image in a fixed
+ * order that depends only on npix, so a result does not depend on which
+ * other reconstructions share the call. */
+typedef struct tn_sense_job {
+    const void *smaps;  /* (ncoils, npix) complex64 coil maps               */
+    void *x, *r, *p;    /* (npix) complex64 CG iterate, residual, direction */
+    void *ap;           /* (npix) complex64 coil-combined normal product     */
+    float *scal;        /* 2 floats of CG state (rs, next rs)                */
+    int64_t img_off;    /* first coil image in the coil-image arena (complex
+                           elements); coil c at img_off + c * npix           */
+    int32_t ncoils;
+    int32_t mode;       /* 0: start, 1: CG iteration, 2: idle (untouched)    */
+} tn_sense_job;
+
+/* SENSE expansion of the CG direction, img[c] = smaps[c] * p, for every
+ * record in mode 1. Replaces SenseOp.forward
+ * (src/deepmr/linops/coil.py:11-98). */
+int tn_sense_expand(const tn_sense_job *jobs, int64_t njobs, int64_t npix, void *img,
+                    tn_stream stream);
+
+/* SENSE combination ap = sum_c conj(smaps[c]) * img[c] (coils in order) for
+ * every record in mode 0 or 1. Replaces SenseAdjointOp.forward
+ * (src/deepmr/linops/coil.py:101-205). */
+int tn_sense_combine(const tn_sense_job *jobs, int64_t njobs, int64_t npix, const void *img,
+                     tn_stream stream);
+
+/* Floats of `work` that tn_cg_step needs for njobs records of npix pixels. */
+int64_t tn_cg_work_floats(int64_t njobs, int64_t npix);
+
+/* One conjugate-gradient update per record, given ap = A^H A p:
+ *   mode 1: alpha = rs / <p, ap>, x += alpha p, r -= alpha ap,
+ *           rs' = <r, r>, p = r + (rs' / rs) p, rs = rs'
+ *   mode 0: x = 0, r = p = ap, rs = <ap, ap>   (start: ap = A^H y)
+ * Replaces CGStep.forward (src/deepmr/optim/cg.py:203-212). Three launches,
+ * no allocation. */
+int tn_cg_step(const tn_sense_job *jobs, int64_t njobs, int64_t npix, float *work,
+               tn_stream stream);
 
 /* Thread-local description of the last error (empty string if none). */
 const char *tn_last_error(void);

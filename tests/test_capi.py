@@ -38,12 +38,15 @@ def test_library_exports_every_declared_symbol():
     assert "gfx950" in tilenufft.version()
 
 
-def test_plan_params_layout_matches_c(tmp_path):
-    """Every field of the ctypes mirror sits at the C offset (and sizeof matches)."""
-    P = _lib.PlanParams
+@pytest.mark.parametrize("ctype", ["tn_plan_params", "tn_sense_job"])
+def test_struct_layouts_match_c(tmp_path, ctype):
+    """Every field of a ctypes mirror sits at the C offset (and sizeof matches)."""
+    from tilenufft import sense
+
+    P = {"tn_plan_params": _lib.PlanParams, "tn_sense_job": sense.SenseJob}[ctype]
     names = [f[0] for f in P._fields_]
     fmt = " ".join(["%zu"] * (len(names) + 1))
-    args = ", ".join(["sizeof(tn_plan_params)"] + [f"offsetof(tn_plan_params, {n})" for n in names])
+    args = ", ".join([f"sizeof({ctype})"] + [f"offsetof({ctype}, {n})" for n in names])
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "tilenufft.h"\n'
