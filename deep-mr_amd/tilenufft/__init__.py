@@ -7,10 +7,13 @@ Modules mirror deepmr's layout for the NUFFT path:
   tilenufft.optim     CGStep, cg_solve
   tilenufft.calib     find_acs (non-Cartesian calibration through nufft_adj)
   tilenufft.dcf       voronoi, analytical_dcf, angular_compensation
+  tilenufft.sampling  radial, radial_proj, spiral (trajectory generators)
+  tilenufft.sense     CG-SENSE state and the fixed-order SENSE / CG kernels
+  tilenufft.serve     continuous-batching CG-SENSE server
 The compute path is the native library libtilenufft.so (HIP, gfx950).
 """
 
-from . import _interp, _signal, autotune, batch, calib, dcf, fft, linops, optim, recon, serve, testing  # noqa: F401
+from . import _interp, _signal, autotune, batch, calib, dcf, fft, linops, optim, recon, sampling, sense, serve, testing  # noqa: F401
 from ._lib import version  # noqa: F401
 from ._signal import resize  # noqa: F401
 
