@@ -57,14 +57,9 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.bin_off = plan->d_bin_off + (int64_t)f * plan->ntiles_total;
         j.pentries = plan->d_pentries;
         j.pbin_off = plan->d_pbin_off + (int64_t)f * plan->nitiles_total;
-        j.bins = plan->d_bins;
-        j.batch_off = plan->d_batch_off ? plan->d_batch_off + (int64_t)f * plan->ntiles_total : nullptr;
-        j.mbase = plan->d_mbase;
-        j.mrow = plan->d_mrow;
-        j.ment = plan->d_ment;
-        j.tslots = plan->tslots;
-        j.sbits = plan->sbits;
-        j.tbits = plan->tbits;
+        j.mrow = plan->d_mrow ? plan->d_mrow + (int64_t)f * gvol : nullptr;
+        j.mnz = plan->d_mnz;
+        j.mlong = p.grid_split == 0 ? kMatrixLongRow : p.grid_split < 0 ? kMatrixStage : std::min(p.grid_split, kMatrixStage);
         // deepmr layout (batch, nframes, ...): frame f of coil b
         j.grid_off = grid_off + (int64_t)f * gvol;
         j.grid_bstride = (int64_t)p.nframes * gvol;
@@ -91,6 +86,8 @@ void free_tables(LaunchTables &t) {
     if (t.d_fold) (void)hipFree(t.d_fold);
     if (t.d_slot_base) (void)hipFree(t.d_slot_base);
     if (t.d_scratch) (void)hipFree(t.d_scratch);
+    if (t.d_tr) (void)hipFree(t.d_tr);
+    if (t.d_st) (void)hipFree(t.d_st);
     t = LaunchTables{};
 }
 
@@ -128,11 +125,21 @@ static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &
     return TN_OK;
 }
 
+// Matrix-gridding runs of frame f, cached in the plan per cells-per-item
+// limit (matrix_runs_of).
+static const std::vector<MatrixRun> &matrix_runs(const tn_plan *pl, int64_t f, int64_t ncell, int cpi, int mlong) {
+    auto &per_frame = pl->mruns[cpi];
+    if (per_frame.empty()) per_frame.resize(pl->p.nframes);
+    std::vector<MatrixRun> &out = per_frame[f];
+    if (out.empty() && ncell > 0) matrix_runs_of(pl->h_mrow.data() + f * ncell, ncell, cpi, mlong, out);
+    return out;
+}
+
 // Work lists. Interpolation items are (job, sample block, coil range);
 // gridding items are (job, tile, coil range). `plan_of[j]` is the plan job j
 // came from (for its chunk preferences).
 static int build_tables_for(const std::vector<Job> &jobs, const std::vector<const tn_plan *> &plan_of,
-                            LaunchTables &t, bool force_gather = false) {
+                            LaunchTables &t, bool force_gather = false, bool no_copy = false) {
     t.kind_interp = interp_kind(jobs);
     t.kind_grid = t.kind_interp;
     int64_t max_plane = 1, max_region = 1;
@@ -164,26 +171,26 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     t.interp_mode = mode;
 
     int max_gc = 1, max_ic = 1;
-    // tile matrix: one record width (coils per item) for the table, the
-    // widest any job asks for, narrowed until the tile's cells fit the lane
-    // groups' registers (kTileMaxCpg cells per group)
-    int tile_rc = 1;
-    int64_t max_cells = 1;
+    int st_cw = 0;
+    bool st_none = false;
+    // matrix gridding: record width of the coil-minor copy (the widest chunk
+    // of the table) and the cells per item the kernel assumes for it
+    int mat_cpi = kBlock, mat_cpl = 1;
     if (gmode == kGridMatrix) {
+        int cwmax = 0;
+        bool inplace = false;
         for (size_t ji = 0; ji < jobs.size(); ++ji) {
-            const int mc = std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch, kTileCoils), kTileCoils);
-            int rc = 1;
-            while (rc < mc) rc <<= 1;
-            tile_rc = std::max(tile_rc, rc);
-            max_cells = std::max<int64_t>(max_cells, (int64_t)jobs[ji].T[0] * jobs[ji].T[1] * jobs[ji].T[2]);
+            const int mc = std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch, kMatrixChunk), kMatrixCoils);
+            if (mc < 2) inplace = true;
+            cwmax = std::max(cwmax, mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16);
         }
-        auto lanes = [](int rc) { return rc >= 2 ? rc / 2 : 1; };
-        while (tile_rc > 1 && max_cells * lanes(tile_rc) > (int64_t)kTileMaxCpg * kBlock) tile_rc >>= 1;
-        if (max_cells * lanes(tile_rc) > (int64_t)kTileMaxCpg * kBlock)
-            return fail(TN_E_ARG, "tile matrix: tile of %lld cells exceeds %d", (long long)max_cells,
-                        kTileMaxCpg * kBlock);
+        // coils per lane: two for 16-coil records, and for 4-8 coil records
+        // in 2D (half the gather instructions); one for 4-8 coil records in
+        // 3D, where twice the cells per wave measured slower (divergence)
+        mat_cpl = (cwmax == 16 || (cwmax >= 4 && t.kind_grid / 16 == 2)) ? 2 : 1;
+        if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / (cwmax / mat_cpl) * kMatrixPasses);
     }
-    std::vector<Item> interp, grid, fold;
+    std::vector<Item> interp, grid, fold, tr;
     std::vector<int64_t> slot_base;
     int64_t scratch = 0;
     for (size_t ji = 0; ji < jobs.size(); ++ji) {
@@ -217,17 +224,38 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                     interp.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + ic), 0, 0, 0, 0});
         }
         if (gmode == kGridMatrix) {
-            // Items are (tile, coil record, batch range): a tile's bin of nb
-            // batches is one item when nb <= kb, else ceil(nb / kb) parts of
-            // kb batches whose partial tiles are folded in part order.
-            const int kb = pl->p.grid_split > 0 ? std::max(1, pl->p.grid_split / pl->tslots)
-                           : pl->p.grid_split < 0 ? INT32_MAX : kTileBatches;
-            max_gc = std::max(max_gc, tile_rc);
-            const int64_t ntiles = (int64_t)j.NT[0] * j.NT[1] * j.NT[2];
-            const int64_t f = (j.batch_off - pl->d_batch_off) / std::max<int64_t>(pl->ntiles_total, 1);
-            const int32_t *boff = pl->h_batch_off.data() + f * pl->ntiles_total;
-            const int64_t cells = (int64_t)j.T[0] * j.T[1] * j.T[2];
-            tile_matrix_items((int)ji, boff, ntiles, j.nbatch, tile_rc, kb, cells, grid, fold, slot_base, scratch);
+            // Items are runs of consecutive cells of one frame (unit = first
+            // cell, e1 = cell count | entry count << 16, slot = first entry,
+            // e0 = coil chunk): at most mat_cpi cells (what the coil-minor
+            // kernel of the table's record width assumes; kBlock for the
+            // lane-per-cell kernel) and at most kMatrixStage entries, so that
+            // an item's entries (contiguous in the CSR order) fit the kernel's
+            // LDS stage. Rows longer than mlong end the run and are summed by
+            // workgroups of their own in parts of kMatrixPart entries (part 1,
+            // unit = cell, e1 = part); rows of several parts leave partial
+            // records folded in part order. Long rows are queued first.
+            const int mc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, kMatrixChunk), kMatrixCoils);
+            max_gc = std::max(max_gc, mc);
+            const int cw = mc <= 2 ? 2 : mc <= 4 ? 4 : mc <= 8 ? 8 : 16;
+            const int cpi = mat_cpi;
+            const int64_t ncell = (int64_t)j.G[0] * j.G[1] * j.G[2];
+            const int64_t f = (j.mrow - pl->d_mrow) / std::max<int64_t>(ncell, 1);
+            const std::vector<MatrixRun> &runs = matrix_runs(pl, f, ncell, cpi, j.mlong);
+            matrix_items((int)ji, runs, j.nbatch, mc, grid, fold, scratch);
+            // coil-minor sample copy: chunks of mc coils padded to cw lanes
+            // (not for tables whose gridding reads the samples in place:
+            // low-rank gridding, or tables only used to interpolate)
+            if (no_copy || force_gather) {
+                st_none = true;
+            } else if (mc >= 2) {
+                st_cw = std::max(st_cw, cw);
+                const int64_t pblk = (j.npts + kBlock - 1) / kBlock;
+                for (int c0 = 0; c0 < j.nbatch; c0 += mc)
+                    for (int64_t b = 0; b < pblk; ++b)
+                        tr.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + mc), c0 / mc, 0, 0, -1});
+            } else {
+                st_none = true;  // a one-coil chunk: the lane-per-cell kernel reads in place
+            }
             continue;
         }
         max_gc = std::max(max_gc, gc);
@@ -260,29 +288,31 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     }
     t.scratch_floats = scratch;
     t.grid_coils = max_gc;
-    if (gmode == kGridMatrix) {
-        // LDS: the staged records [slots][rc + pad] (the output stage reuses
-        // them, at least one coil plane [cells + 1]) and the per-axis taps
-        // [ndim][2^tbits][slots] of each slot
-        int slots = 1, wp = 1;
-        for (const tn_plan *pl : plan_of) {
-            slots = std::max(slots, pl->tslots);
-            wp = std::max(wp, 1 << pl->tbits);
+    std::vector<Job> jobs_st(jobs);
+    if (gmode == kGridMatrix && st_cw > 0 && !st_none) {  // every job gets its copy at st_cw coils per record
+        int64_t off = 0;
+        for (size_t ji = 0; ji < jobs.size(); ++ji) {
+            const int mc = std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch, kMatrixChunk), kMatrixCoils);
+            jobs_st[ji].st_off = off;
+            off += (int64_t)((jobs[ji].nbatch + mc - 1) / mc) * jobs[ji].npts * st_cw;
         }
-        const int cpl = tile_rc >= 2 ? 2 : 1;
-        const int64_t rec = 2 * (int64_t)slots * (tile_rc + cpl);
-        const int64_t plane = 2 * (max_cells + 1);
-        const int64_t region = (std::max(rec, plane) + 3) & ~(int64_t)3;
-        t.tile_coils = tile_rc;
-        t.tile_region_floats = (int)region;
-        t.tile_out_coils = (int)std::max<int64_t>(1, std::min<int64_t>(tile_rc, region / plane));
-        t.grid_lds = sizeof(float) * (size_t)(region + (int64_t)gndim * wp * slots);
-    } else {
-        t.grid_lds = grid_lds_bytes(max_gc, max_plane, wc, gmode, gndim);
+        t.st_cw = st_cw;
+        t.st_cpl = mat_cpl;
+        t.st_elems = off;
+        t.n_tr = (int64_t)tr.size();
+        TN_HIP(hipMalloc(&t.d_tr, sizeof(Item) * std::max<size_t>(tr.size(), 1)));
+        if (!tr.empty()) TN_HIP(hipMemcpy(t.d_tr, tr.data(), sizeof(Item) * tr.size(), hipMemcpyHostToDevice));
+        TN_HIP(hipMalloc(&t.d_st, sizeof(float2) * std::max<int64_t>(off, 1)));
     }
+    const int64_t copy_bytes = t.d_st ? (int64_t)(sizeof(Item) * std::max<size_t>(tr.size(), 1) +
+                                                  sizeof(float2) * std::max<int64_t>(t.st_elems, 1))
+                                      : 0;
+    t.grid_lds = gmode == kGridMatrix ? 0 : grid_lds_bytes(max_gc, max_plane, wc, gmode, gndim);
     t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
     t.interp_coils = max_ic;
-    return upload_tables(jobs, interp, grid, fold, slot_base, t);
+    const int rc = upload_tables(jobs_st, interp, grid, fold, slot_base, t);
+    t.device_bytes += copy_bytes;
+    return rc;
 }
 
 }  // namespace tn
@@ -320,12 +350,11 @@ int validate(const tn_plan_params *p) {
 
 void choose_tiles(tn_plan *pl) {
     const tn_plan_params &p = pl->p;
-    int def[3];  // gridding tiles (z, y, x): the tile matrix keeps a tile's cells in registers
+    int def[3];
     if (p.ndim == 2) {
-        def[0] = 1;
-        def[1] = def[2] = pl->grid_mode == kGridMatrix ? 16 : 32;
+        def[0] = 1; def[1] = 32; def[2] = 32;
     } else {
-        def[0] = def[1] = def[2] = pl->grid_mode == kGridMatrix ? 8 : 16;
+        def[0] = 16; def[1] = 16; def[2] = 16;
     }
     int idef[3];  // default staged-interpolation tiles (z, y, x): measured fastest with the default coil chunks
     if (p.ndim == 2) {
@@ -356,16 +385,14 @@ void destroy(tn_plan *pl) {
     (void)hipSetDevice(pl->device);
     for (auto &kv : pl->cache) free_tables(kv.second);
     for (auto &kv : pl->cache_gather) free_tables(kv.second);
+    for (auto &kv : pl->cache_lowrank) free_tables(kv.second);
     if (pl->d_coord) (void)hipFree(pl->d_coord);
     if (pl->d_entries) (void)hipFree(pl->d_entries);
     if (pl->d_bin_off) (void)hipFree(pl->d_bin_off);
     if (pl->d_pentries) (void)hipFree(pl->d_pentries);
     if (pl->d_pbin_off) (void)hipFree(pl->d_pbin_off);
-    if (pl->d_bins) (void)hipFree(pl->d_bins);
-    if (pl->d_batch_off) (void)hipFree(pl->d_batch_off);
-    if (pl->d_mbase) (void)hipFree(pl->d_mbase);
     if (pl->d_mrow) (void)hipFree(pl->d_mrow);
-    if (pl->d_ment) (void)hipFree(pl->d_ment);
+    if (pl->d_mnz) (void)hipFree(pl->d_mnz);
     for (int a = 0; a < 2; ++a)
         if (pl->d_fft_tw[a]) (void)hipFree(pl->d_fft_tw[a]);
     for (int d = 0; d < 3; ++d)
@@ -374,8 +401,10 @@ void destroy(tn_plan *pl) {
     delete pl;
 }
 
-tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc, bool gather = false) {
-    auto &cache = gather ? plan->cache_gather : plan->cache;
+enum TableUse { kTablesPlain = 0, kTablesGather = 1, kTablesLowRank = 2 };
+
+tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc, int use = kTablesPlain) {
+    auto &cache = use == kTablesGather ? plan->cache_gather : use == kTablesLowRank ? plan->cache_lowrank : plan->cache;
     auto it = cache.find(nbatch);
     if (it != cache.end()) {
         *rc = TN_OK;
@@ -385,7 +414,7 @@ tn::LaunchTables *plan_tables(tn_plan *plan, int64_t nbatch, int *rc, bool gathe
     make_jobs(plan, nbatch, 0, 0, 0, jobs);
     std::vector<const tn_plan *> owners(jobs.size(), plan);
     LaunchTables t;
-    *rc = build_tables_for(jobs, owners, t, gather);
+    *rc = build_tables_for(jobs, owners, t, use == kTablesGather, use == kTablesLowRank);
     if (*rc != TN_OK) {
         free_tables(t);
         return nullptr;
@@ -445,6 +474,7 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
         delete pl;
         return fail(TN_E_HIP, "hipGetDevice failed");
     }
+    choose_tiles(pl);
     {
         const int taps = params->width[1] * params->width[2];  // one z plane of the footprint
         const bool runs_ok = taps <= kRunsMaxTaps;
@@ -452,12 +482,14 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
             delete pl;
             return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per z plane (got %d)", kRunsMaxTaps, taps);
         }
-        // automatic: the tile matrix when its entries (2 bytes per sample and
-        // tap) fit kMatrixAutoBytes, else register runs when the taps of a z
-        // plane fit the registers, else per-sample LDS atomics
+        // automatic: the sparse gridding matrix when it fits kMatrixAutoBytes
+        // (8 bytes per sample and tap), else register runs when the taps of a
+        // z plane fit the registers, else per-sample LDS atomics
         const int64_t nnz = (int64_t)params->nframes * params->npts * params->width[0] * params->width[1] *
                             params->width[2];
-        const bool matrix_ok = nnz < INT32_MAX && (uint64_t)nnz * sizeof(uint16_t) <= kMatrixAutoBytes;
+        const int64_t nrows = (int64_t)params->nframes * params->grid[0] * params->grid[1] * params->grid[2];
+        const bool matrix_ok = nnz < INT32_MAX && (uint64_t)nnz * sizeof(int2) <= kMatrixAutoBytes &&
+                               nrows < (int64_t)UINT32_MAX && params->npts <= ((int64_t)1 << 28);
         pl->grid_mode = params->grid_mode == kGridSpread   ? kGridSpread
                         : params->grid_mode == kGridMatrix ? kGridMatrix
                         : params->grid_mode == kGridRuns   ? kGridRuns
@@ -465,7 +497,6 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
                         : runs_ok                          ? kGridRuns
                                                            : kGridSpread;
     }
-    choose_tiles(pl);
     const int64_t ncoord = (int64_t)params->nframes * params->npts * params->ndim;
     auto cleanup = [&](int code) {
         destroy(pl);
@@ -504,24 +535,20 @@ int64_t tn_plan_device_bytes(const tn_plan *plan) {
     if (!plan) return 0;
     const tn_plan_params &p = plan->p;
     const int64_t total = (int64_t)p.nframes * p.npts;
-    int64_t b = sizeof(float) * std::max<int64_t>(total * p.ndim, 1);           // coordinates
-    b += sizeof(int32_t) * (plan->ntiles_total * p.nframes + 1);                 // gridding bin offsets
-    b += sizeof(float4) * total + sizeof(int32_t) * (plan->nitiles_total * p.nframes + 1);  // interpolation bins
-    if (plan->grid_mode == kGridMatrix) {
-        const int64_t cells = (int64_t)plan->tile[0] * plan->tile[1] * plan->tile[2];
-        b += sizeof(int32_t) * std::max<int64_t>(plan->nentries, 1);             // bin slots
-        b += sizeof(int32_t) * (plan->ntiles_total * p.nframes + 1);             // batch offsets
-        b += sizeof(int32_t) * (plan->nbatches + 1);                             // entry bases
-        b += sizeof(uint16_t) * std::max<int64_t>(plan->nbatches * (cells + 1), 1);  // row starts
-        b += sizeof(uint16_t) * std::max<int64_t>(plan->mnnz, 1);                // entries
-    } else {
-        b += sizeof(float4) * std::max<int64_t>(plan->nentries, 1);
-    }
+    int64_t b = sizeof(float) * std::max<int64_t>(total * p.ndim, 1);                     // coordinates
+    b += sizeof(float4) * std::max<int64_t>(plan->nentries, 1);                            // gridding bins
+    b += sizeof(int32_t) * (plan->ntiles_total * p.nframes + 1);
+    b += sizeof(float4) * std::max<int64_t>(total, 1);                                     // interpolation bins
+    b += sizeof(int32_t) * (plan->nitiles_total * p.nframes + 1);
+    if (plan->d_mrow)                                                                      // gridding matrix
+        b += sizeof(int32_t) * ((int64_t)p.grid[0] * p.grid[1] * p.grid[2] * p.nframes + 1) +
+             sizeof(int2) * std::max<int64_t>(plan->mnnz, 1);
     for (int d = 0; d < 3; ++d)
         if (plan->d_apod[d]) b += sizeof(float) * p.shape[d];
     b += plan->fft_tw_bytes;
     for (const auto &kv : plan->cache) b += kv.second.device_bytes;
     for (const auto &kv : plan->cache_gather) b += kv.second.device_bytes;
+    for (const auto &kv : plan->cache_lowrank) b += kv.second.device_bytes;
     return b;
 }
 
@@ -595,7 +622,7 @@ int tn_interp_lowrank(tn_plan *plan, const void *grid, void *samp, int64_t nbatc
     if (rc) return rc;
     rc = check_device(plan->device);
     if (rc) return rc;
-    LaunchTables *t = plan_tables(plan, nbatch, &rc, true);
+    LaunchTables *t = plan_tables(plan, nbatch, &rc, kTablesGather);
     if (!t) return rc;
     return launch_interp_lowrank(*t, (const float2 *)grid, (float2 *)samp, weight, (hipStream_t)stream, lr);
 }
@@ -612,7 +639,7 @@ int tn_grid_lowrank(tn_plan *plan, const void *samp, void *grid, int64_t nbatch,
     rc = check_device(plan->device);
     if (rc) return rc;
     // accumulator coils are (coil, coeff) pairs
-    LaunchTables *t = plan_tables(plan, nbatch * ncoeff, &rc);
+    LaunchTables *t = plan_tables(plan, nbatch * ncoeff, &rc, kTablesLowRank);
     if (!t) return rc;
     return launch_grid(*t, (const float2 *)samp, (float2 *)grid, weight, (hipStream_t)stream, lr);
 }

@@ -36,18 +36,15 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 
 // Gridding variants: one LDS atomic per tap and sample over entries spread
 // across footprint origins, or per-lane register runs over origin-sorted
-// entries (for at most kRunsMaxTaps taps per sample), or tile matrix: per
-// tile, the samples whose footprint touches it (its bin, in sample order) are
-// taken in batches of at most kTileSlots; each batch's sample records are
-// staged in LDS once and every cell of the tile sums its row of a plan-time
-// sparse matrix of 2-byte entries (batch slot | tap code), the weight being
-// the product of the slot's per-axis taps, also staged in LDS.
+// entries (for at most kRunsMaxTaps taps per sample).
+// Matrix: per-cell sums over the plan's sparse gridding matrix (CSR by grid
+// cell), one lane per cell.
 enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2, kGridMatrix = 3 };
-constexpr int kTileSlots = 256;       // bin slots per batch (one per thread when staged)
-constexpr int kTileCoils = 16;        // widest record (coils per item)
-constexpr int kTileMaxCpg = 8;        // tile cells per lane group (accumulators in registers)
-constexpr int kTileBatches = 4;       // default batches per item; longer bins are summed in parts
-constexpr uint64_t kMatrixAutoBytes = 4ull << 30;  // largest tile matrix the automatic mode builds
+constexpr int kMatrixLongRow = 128;  // default: rows with more entries get a whole workgroup
+constexpr int kMatrixChunk = 16;     // default coils per item (record width of the coil-minor copy)
+constexpr uint64_t kMatrixAutoBytes = 4ull << 30;  // largest gridding matrix the automatic mode builds
+constexpr int kMatrixPasses = 4;     // cell passes per item of the coil-minor matrix gridding
+constexpr int kLongGroups = 32;      // lane groups summing one long row of the coil-minor matrix gridding
 constexpr int kXcds = 8;          // accelerator complex dies (each with its own L2)
 constexpr int kRunsMaxTaps = 36;  // register budget of the run sums of one z plane (W <= 6)
 constexpr int kRunsBatch = 256;                 // entries per staged batch (= kBlock)
@@ -84,7 +81,7 @@ struct alignas(16) Job {
     float inv_scale;
     int32_t ndim;
     int32_t nbatch;     // coils (batch) of this job
-    int32_t tslots;     // tile matrix: bin slots per batch
+    int32_t mlong;      // matrix gridding: rows with more entries are summed by a whole workgroup
     int64_t npts;
     const float *coord;       // (npts, ndim) grid units
     const float4 *entries;    // (x, y, z, bits(sample)) sorted by tile
@@ -92,18 +89,14 @@ struct alignas(16) Job {
     const float4 *pentries;   // each sample once, sorted by the interpolation
                               // tile holding its footprint origin
     const int32_t *pbin_off;  // (interp tiles + 1) offsets into pentries
-    const int32_t *bins;      // tile matrix: sample index of every bin slot, sorted by (tile, sample)
-    const int32_t *batch_off; // tile matrix: (ntiles + 1) first batch of each tile of this frame
-    const int32_t *mbase;     // tile matrix: first entry of every batch (global batch index)
-    const uint16_t *mrow;     // tile matrix: (cells + 1) row starts per batch, relative to mbase
-    const uint16_t *ment;     // tile matrix: slot | tap code << sbits per entry
-    int32_t sbits;            // tile matrix: bits of the slot in an entry
-    int32_t tbits;            // tile matrix: bits per axis of the tap code
+    const int32_t *mrow;      // matrix gridding: (cells + 1) row offsets of this frame
+    const int2 *mnz;          // matrix gridding: (sample, weight bits) per entry
     int64_t grid_off;         // complex elements from the grid base
     int64_t grid_bstride;     // complex elements between coils
     int64_t samp_off;
     int64_t samp_bstride;
     int64_t w_off;            // offset into the weight base (-1: no weight)
+    int64_t st_off;           // matrix gridding: this job's coil-minor sample copy in the table's buffer
 };
 
 // Low-rank subspace arguments of a launch (ncoeff == 0: plain transform).
@@ -136,9 +129,14 @@ struct LaunchTables {
     size_t interp_lds = 0;     // dynamic LDS bytes for the staged interpolation
     int interp_coils = 1;      // coils per LDS region load (staged interpolation)
     int grid_mode = kGridSpread;  // gridding variant (matches the plans' entry order)
-    int tile_coils = 1;        // tile matrix: record width (coils per item: 1, 2, 4, 8, 16)
-    int tile_region_floats = 0;  // tile matrix: LDS floats of the record / output region
-    int tile_out_coils = 1;      // tile matrix: coil planes per output pass through LDS
+    // matrix gridding over a coil-minor copy of the samples: chunks of
+    // st_cw coils, [chunk][sample][st_cw] per job, built by the transpose items
+    Item *d_tr = nullptr;
+    int64_t n_tr = 0;
+    float2 *d_st = nullptr;
+    int64_t st_elems = 0;
+    int st_cw = 0;  // 0: the lane-per-cell matrix kernel reads the samples in place
+    int st_cpl = 1; // coils per lane of the matrix kernel over the copy (1 or 2)
     int64_t device_bytes = 0;  // bytes of the device allocations above
 };
 
@@ -157,7 +155,11 @@ struct tn_plan {
     int32_t *d_pbin_off = nullptr;
     std::vector<int32_t> h_pbin_off;  // host copies (work-list construction)
     std::vector<int32_t> h_bin_off;
-    std::vector<int32_t> h_batch_off; // host copy of d_batch_off (work-list construction)
+    std::vector<int32_t> h_mrow;      // host copy of d_mrow (work-list construction)
+    // matrix-gridding runs per frame, keyed by the cells-per-item limit
+    // (built once per plan; launch tables of any coil count reuse them; the
+    // long-row threshold is the plan's grid_split, fixed for its lifetime)
+    mutable std::map<int, std::vector<std::vector<tn::MatrixRun>>> mruns;
     float *d_coord = nullptr;
     float4 *d_entries = nullptr;
     int32_t *d_bin_off = nullptr;
@@ -165,20 +167,14 @@ struct tn_plan {
     int64_t nentries = 0;
     int64_t footprint = -1;
     int grid_mode = tn::kGridSpread;  // effective gridding variant (entry order)
-    // tile matrix (grid_mode 3)
-    int32_t *d_bins = nullptr;      // sample of every bin slot, (nentries)
-    int32_t *d_batch_off = nullptr; // (nframes * ntiles + 1) first batch of every tile
-    int32_t *d_mbase = nullptr;     // (nbatches + 1) first entry of every batch
-    uint16_t *d_mrow = nullptr;     // (nbatches * (tile cells + 1)) row starts relative to mbase
-    uint16_t *d_ment = nullptr;     // (mnnz) slot | tap code << sbits
-    int64_t nbatches = 0;
-    int64_t mnnz = 0;
-    int32_t tslots = tn::kTileSlots;  // bin slots per batch
-    int32_t sbits = 8, tbits = 2;     // entry slot bits, tap-code bits per axis
+    int32_t *d_mrow = nullptr;  // matrix gridding: row offsets, (nframes * cells + 1)
+    int2 *d_mnz = nullptr;      // matrix gridding: (sample, weight bits) sorted by (frame, cell)
     float2 *d_fft_tw[2] = {nullptr, nullptr};  // pruned FFT twiddles exp(-2 pi i m / G), (x, y); null: no pruned FFT
     int64_t fft_tw_bytes = 0;
+    int64_t mnnz = 0;
     std::map<int64_t, tn::LaunchTables> cache;
-    std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank)
+    std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank interpolation)
+    std::map<int64_t, tn::LaunchTables> cache_lowrank; // tables of the low-rank gridding (no coil-minor copy)
 };
 
 struct tn_batch {
@@ -206,11 +202,13 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid,
                 float *scratch = nullptr);
 int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *samp,
                           const float *weight, hipStream_t stream, LowRank lr);
+int launch_fold(const LaunchTables &t, float2 *grid, hipStream_t stream);
 int launch_apod_pad(const tn_plan *plan, const float2 *image, float2 *grid,
                     int64_t nimg, hipStream_t stream);
 int launch_crop_apod(const tn_plan *plan, const float2 *grid, float2 *image,
                      int64_t nimg, hipStream_t stream);
 int build_bins(tn_plan *plan, hipStream_t stream);
+int build_matrix(tn_plan *plan, hipStream_t stream);
 // Pruned, fused oversampled-grid FFT stages (2D plans with supported lengths).
 bool fft_supported(const tn_plan *plan);
 int build_fft(tn_plan *plan);

@@ -874,167 +874,339 @@ __global__ __launch_bounds__(kBlock) void grid_runs_kernel(const Job *__restrict
                      scratch, slot_base);
 }
 
-// Gridding by the tile matrix. One workgroup owns one tile of one frame for
-// a record of RC = CW * CPL coils and walks a range of its bin's batches
-// (each at most J.tslots samples, in sample order):
-//   A. one lane per batch slot loads the sample's coordinates, evaluates its
-//      per-axis taps once into LDS, and loads the scaled values of the
-//      record's coils into an LDS record (independent loads, coalesced along
-//      the samples of each coil): every sample value is read from global
-//      memory once per tile it touches;
-//   B. lane group g (CW lanes, CPL coils each) owns cells g, g + CPP, ... of
-//      the tile (accumulators in registers across the batches) and walks the
-//      batch's row of each cell: entry = slot | tap code, weight = product of
-//      the slot's taps of those indices, value = the slot's record from LDS.
-// Each cell is summed by one lane group in the plan's fixed (batch, slot)
-// order, so the result is deterministic and independent of the coil record
-// width. The tile is written once (zeros included) through LDS in
-// contiguous rows, or, for a bin summed in parts, stored as partial `part`
-// of fold slot `slot` for fold_kernel. No atomics.
-template <int NDIM, int LCW, int CPL>
-__global__ __launch_bounds__(kBlock) void grid_tile_kernel(const Job *__restrict__ jobs,
-                                                           const Item *__restrict__ items, int64_t nitems,
-                                                           const float2 *__restrict__ samp,
-                                                           float2 *__restrict__ grid,
-                                                           const float *__restrict__ weight,
-                                                           float *__restrict__ scratch,
-                                                           const int64_t *__restrict__ slot_base, LowRank lr,
-                                                           int region_floats, int out_coils) {
+// Gridding by the plan's sparse gridding matrix: per frame, every (cell,
+// sample, weight) triple of the kernel footprints, sorted by output cell
+// (CSR), samples ascending inside a cell. Two kinds of work item, both
+// summing weight * scaled sample for up to CM coils in registers and writing
+// each cell of every coil once (zeros included) with streaming stores:
+//   part 0: a run of (it.e1 & 0xffff) <= kBlock consecutive cells from
+//           it.unit, one lane per cell, walking its row with the next entry
+//           prefetched; rows longer than J.mlong are skipped;
+//   part 1: entries [it.e1 * kMatrixPart, + kMatrixPart) of one such long
+//           row (dense trajectory centres hold hundreds to 10^5 entries per
+//           cell), strided over kLongGroups lanes, each summing its
+//           entries in order, the lane sums then added in lane order; a row
+//           of several parts (it.slot >= 0) leaves partial record
+//           it.slot + it.e1 for matrix_fold_kernel, else the cell is
+//           written.
+// No atomics, and a fixed summation order for every cell. Long rows are
+// queued first; items are dealt to the XCDs in contiguous ranges (the
+// hardware deals workgroups out round-robin), so the neighbouring rows of
+// one frame, which read mostly the same samples, share one XCD's L2.
+// Low rank (LR): accumulator coils are (coil, coeff) pairs; the sample of
+// frame f enters coefficient k times basis[k, f].
+template <int CM, bool LR>
+__global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restrict__ jobs,
+                                                             const Item *__restrict__ items, int64_t nitems,
+                                                             const float2 *__restrict__ samp,
+                                                             float2 *__restrict__ grid,
+                                                             const float *__restrict__ weight, LowRank lr,
+                                                             float2 *__restrict__ parts) {
+    __shared__ float red[kLongGroups][2 * CM];
+    const int64_t idx = xcd_item(nitems);
+    if (idx < 0) return;
+    const Item it = items[idx];
+    const Job &J = jobs[it.job];
+    const int ncoil = it.c1 - it.c0;
+    const bool dcf = weight != nullptr && J.w_off >= 0;
+    // per-coil sample rows (uniform); coils past the item's last repeat it
+    // (loads stay in bounds and unconditional, their sums are not stored)
+    const char *cb[CM];
+    const float2 *qb[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+        const int vc = it.c0 + min(c, ncoil - 1);
+        const int b = LR ? vc / lr.ncoeff : vc;
+        cb[c] = (const char *)(samp + J.samp_off + (int64_t)b * J.samp_bstride);
+        qb[c] = LR ? lr.coef + (int64_t)(vc - b * lr.ncoeff) * lr.nframes : nullptr;
+    }
+    float ar[CM], ai[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) ar[c] = ai[c] = 0.0f;
+    auto add_entry = [&](i32x2 m) {
+        const uint32_t boff = (uint32_t)m.x * (uint32_t)sizeof(float2);  // npts < 2^28
+        float2 v[CM];
+#pragma unroll
+        for (int c = 0; c < CM; ++c) v[c] = *(const float2 *)(cb[c] + boff);
+        if (!LR) {
+            // the sample scaled first, then weighted: the same operations as
+            // the coil-minor copy and its kernel, so both give equal sums
+            float sc = J.inv_scale;
+            if (dcf) sc *= weight[J.w_off + m.x];
+            const float w = __int_as_float(m.y);
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                ar[c] += w * (v[c].x * sc);
+                ai[c] += w * (v[c].y * sc);
+            }
+        } else {
+            float w = __int_as_float(m.y) * J.inv_scale;
+            if (dcf) w *= weight[J.w_off + m.x];
+            const int f = m.x / lr.frame_npts;
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+                const float2 q = qb[c][f];
+                ar[c] += w * (q.x * v[c].x - q.y * v[c].y);
+                ai[c] += w * (q.x * v[c].y + q.y * v[c].x);
+            }
+        }
+    };
+    const i32x2 *mnz = (const i32x2 *)J.mnz;
+    if (it.part == 0) {  // cells it.unit .. it.unit + (it.e1 & 0xffff) - 1 (<= kBlock)
+        const int64_t cell = (int64_t)it.unit + threadIdx.x;
+        if ((int)threadIdx.x >= (it.e1 & 0xffff)) return;
+        const int r0 = J.mrow[cell], r1 = J.mrow[cell + 1];
+        if (r1 - r0 > J.mlong) return;  // written by its long-row item
+        if (r0 < r1) {
+            i32x2 m = __builtin_nontemporal_load(mnz + r0);
+            for (int e = r0; e < r1; ++e) {
+                const i32x2 mn = __builtin_nontemporal_load(mnz + min(e + 1, r1 - 1));
+                add_entry(m);
+                m = mn;
+            }
+        }
+        float2 *g = grid + J.grid_off + cell;
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+            if (c < ncoil) {
+                const f32x2 v = {ar[c], ai[c]};
+                __builtin_nontemporal_store(v, (f32x2 *)(g + (int64_t)(it.c0 + c) * J.grid_bstride));
+            }
+        }
+        return;
+    }
+    // a long row: lane k < kLongGroups sums the entries r0 + k, r0 + k +
+    // kLongGroups, ... in order, then the lane sums are added in lane order
+    // (the partition and order of the coil-minor kernel's long rows)
+    const int64_t cell = it.unit;
+    const int r0 = J.mrow[cell] + it.e1 * kMatrixPart, r1 = min(J.mrow[cell + 1], r0 + kMatrixPart);
+    if ((int)threadIdx.x < kLongGroups) {
+        int e = r0 + (int)threadIdx.x;
+        for (; e + 3 * kLongGroups < r1; e += 4 * kLongGroups) {  // four entries in flight
+            i32x2 m[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) m[u] = __builtin_nontemporal_load(mnz + e + u * kLongGroups);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) add_entry(m[u]);
+        }
+        for (; e < r1; e += kLongGroups) add_entry(__builtin_nontemporal_load(mnz + e));
+#pragma unroll
+        for (int c = 0; c < CM; ++c) {
+            red[threadIdx.x][2 * c] = ar[c];
+            red[threadIdx.x][2 * c + 1] = ai[c];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncoil) {
+        const int c = threadIdx.x;
+        float re = red[0][2 * c], im = red[0][2 * c + 1];
+        for (int k = 1; k < kLongGroups; ++k) {
+            re += red[k][2 * c];
+            im += red[k][2 * c + 1];
+        }
+        if (it.slot < 0)
+            grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + cell] = make_float2(re, im);
+        else
+            parts[((int64_t)it.slot + it.e1) * kMatrixCoils + c] = make_float2(re, im);
+    }
+}
+
+// Coil-minor copy of the samples for the matrix gridding: chunk q of a job
+// (coils c0 .. c1 - 1 of the item) becomes st[q][p][CW] = samp[c0 + j][p]
+// * inv_scale * weight[p], zeros for j >= c1 - c0. One item = kBlock
+// samples of one chunk, transposed through LDS: coalesced reads along the
+// samples of each coil, contiguous writes.
+template <int LCW>
+__global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__restrict__ jobs,
+                                                                const Item *__restrict__ items,
+                                                                const float2 *__restrict__ samp,
+                                                                const float *__restrict__ weight,
+                                                                float2 *__restrict__ st) {
+    constexpr int CW = 1 << LCW;
+    __shared__ float2 tile[kBlock * (CW + 1)];
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)it.unit * kBlock;
+    const int64_t p = p0 + t;
+    const int nc = it.c1 - it.c0;
+    float sc = J.inv_scale;
+    if (p < J.npts && weight != nullptr && J.w_off >= 0) sc *= weight[J.w_off + p];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (j < nc && p < J.npts) {
+            const float2 a = samp[J.samp_off + (int64_t)(it.c0 + j) * J.samp_bstride + p];
+            v = make_float2(a.x * sc, a.y * sc);
+        }
+        tile[t * (CW + 1) + j] = v;
+    }
+    __syncthreads();
+    float2 *dst = st + J.st_off + ((int64_t)it.e0 * J.npts + p0) * CW;
+    const int np = (int)min<int64_t>(kBlock, J.npts - p0);
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const int flat = k * kBlock + t;
+        const int pp = flat >> LCW, j = flat & (CW - 1);
+        if (pp < np) dst[flat] = tile[pp * (CW + 1) + j];
+    }
+}
+
+// Matrix gridding over the coil-minor sample copy: CW lanes per cell, each
+// lane summing CPL adjacent coils, so that the cell's lanes read one
+// contiguous RC = CW * CPL coil record per entry (one cache line for 16
+// coils instead of one line per coil; with CPL = 2 each lane reads 16 bytes).
+// Items as for grid_matrix_kernel (it.e0 = chunk index). Part 0, a run of
+// (it.e1 & 0xffff) cells from it.unit whose entries (it.e1 >> 16 of them,
+// at most kMatrixStage, contiguous in the CSR order from it.slot) and row
+// offsets are first staged in LDS with independent coalesced loads, so that
+// the only dependent global loads left are the sample records; kBlock / CW
+// cells per pass, each row U entries at a time. Part 1 strides one long
+// row's entries over kLongGroups cell groups and adds the group sums in
+// group order.
+template <int LCW, int CPL, int U>
+__global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
+                                                               const Item *__restrict__ items, int64_t nitems,
+                                                               const float2 *__restrict__ st,
+                                                               float2 *__restrict__ grid,
+                                                               float2 *__restrict__ parts) {
+    static_assert(CPL == 1 || CPL == 2, "one or two coils per lane");
     constexpr int CW = 1 << LCW;      // lanes per cell
     constexpr int RC = CW * CPL;      // coils per record
-    constexpr int RCP = RC + CPL;     // record pitch in float2 (padded; 16-byte aligned for CPL = 2)
     constexpr int CPP = kBlock / CW;  // cells per pass
-    constexpr int U = 4;              // entries in flight per lane
-    extern __shared__ float smem[];
-    const int64_t item = xcd_item(nitems);
-    if (item < 0) return;
-    const Item it = items[item];
+    // cells per item (the host builds items of at most this many cells for
+    // this record width); the item's sums are staged per coil plane in LDS
+    // and stored as contiguous runs (row pitch CPI + 1 against bank conflicts)
+    constexpr int CPI = CPP * kMatrixPasses < kBlock ? CPP * kMatrixPasses : kBlock;
+    __shared__ float2 sout[RC * (CPI + 1)];
+    __shared__ i32x2 sent[kMatrixStage];
+    __shared__ int srow[kBlock + 1];
+    const int64_t idx = xcd_item(nitems);
+    if (idx < 0) return;
+    const Item it = items[idx];
     const Job &J = jobs[it.job];
-    const int tile = it.unit;
-    const int Tx = J.T[0], Ty = J.T[1], Tz = J.T[2];
-    const int tx = tile % J.NT[0];
-    const int ty = (tile / J.NT[0]) % J.NT[1];
-    const int tz = tile / (J.NT[0] * J.NT[1]);
-    const int ox = tx * Tx, oy = ty * Ty, oz = tz * Tz;
-    const int ex = min(Tx, J.G[0] - ox), ey = min(Ty, J.G[1] - oy), ez = min(Tz, J.G[2] - oz);
-    const int CT = Tx * Ty * Tz;
-    const int S = J.tslots;
-    const int sb = J.sbits, tb = J.tbits;
-    const int smask = (1 << sb) - 1, tmask = (1 << tb) - 1;
-    const int WP = 1 << tb;
-    float2 *rec = (float2 *)smem;          // [S][RCP]; the output stage reuses it
-    float *sw = smem + region_floats;      // [NDIM][WP][S] per-axis taps of each slot
-    const int ncoil = it.c1 - it.c0;
-    const int g = threadIdx.x >> LCW, j = threadIdx.x & (CW - 1);
-    const int ncg = (CT + CPP - 1) / CPP;  // cells per group (<= kTileMaxCpg, checked on the host)
-
-    float ar[kTileMaxCpg][CPL], ai[kTileMaxCpg][CPL];
+    const int j = threadIdx.x & (CW - 1), g = threadIdx.x >> LCW;
+    // this lane's coils j * CPL .. j * CPL + CPL - 1 of the record
+    const float2 *sq = st + J.st_off + (int64_t)it.e0 * J.npts * RC + j * CPL;
+    const i32x2 *mnz = (const i32x2 *)J.mnz;
+    struct Acc {
+        float re[CPL], im[CPL];
+    };
+    // entries e0, e0 + stride, ... below r1 of one row (from `ent`), U at a time
+    auto row_sum = [&](const i32x2 *ent, bool global, int e0, int r1, int stride, Acc &a) {
+        for (int e = e0; e < r1; e += U * stride) {
+            i32x2 m[U];
+            float2 v[U][CPL];
 #pragma unroll
-    for (int i = 0; i < kTileMaxCpg; ++i)
-#pragma unroll
-        for (int q = 0; q < CPL; ++q) ar[i][q] = ai[i][q] = 0.0f;
-
-    const int32_t tbin0 = J.bin_off[tile];
-    const int nslot_tile = J.bin_off[tile + 1] - tbin0;
-    const int32_t bat0 = J.batch_off[tile];
-    const bool dcf = weight != nullptr && J.w_off >= 0;
-    for (int kb = it.e0; kb < it.e1; ++kb) {
-        const int s0 = kb * S;
-        const int ns = min(S, nslot_tile - s0);
-        __syncthreads();  // the previous batch is consumed
-        if ((int)threadIdx.x < ns) {  // A. stage slot threadIdx.x
-            const int s = threadIdx.x;
-            const int p = J.bins[tbin0 + s0 + s];
-            const float *cp = J.coord + (int64_t)p * NDIM;
-#pragma unroll
-            for (int d = 0; d < NDIM; ++d) {
-                int base;
-                float w[8];
-                axis_taps<8>(cp[d], J.W[d], J.G[d], J.shift[d], J.beta[d], base, w);
-                for (int k = 0; k < WP; ++k) sw[(d * WP + k) * S + s] = k < 8 ? w[k] : 0.0f;
+            for (int u = 0; u < U; ++u) {
+                const int ei = min(e + u * stride, r1 - 1);
+                m[u] = global ? __builtin_nontemporal_load(ent + ei) : ent[ei];
             }
-            float sc = J.inv_scale;
-            if (dcf) sc *= weight[J.w_off + p];
-            float2 v[RC];
 #pragma unroll
-            for (int c = 0; c < RC; ++c)
-                v[c] = c < ncoil ? grid_sample(J, samp, it.c0 + c, p, sc, lr) : make_float2(0.0f, 0.0f);
+            for (int u = 0; u < U; ++u) {
+                if constexpr (CPL == 2) {
+                    const float4 r = *(const float4 *)(sq + (int64_t)m[u].x * RC);
+                    v[u][0] = make_float2(r.x, r.y);
+                    v[u][1] = make_float2(r.z, r.w);
+                } else {
+                    v[u][0] = sq[(int64_t)m[u].x * RC];
+                }
+            }
 #pragma unroll
-            for (int c = 0; c < RC; ++c) rec[s * RCP + c] = v[c];
-        }
-        __syncthreads();
-        // B. rows of this group's cells
-        const int b = bat0 + kb;
-        const uint16_t *rows = J.mrow + (int64_t)b * (CT + 1);
-        const uint16_t *ent = J.ment + J.mbase[b];
+            for (int u = 0; u < U; ++u) {
+                const float w = e + u * stride < r1 ? __int_as_float(m[u].y) : 0.0f;
 #pragma unroll
-        for (int i = 0; i < kTileMaxCpg; ++i) {
-            const int c = g + i * CPP;
-            if (i >= ncg || c >= CT) continue;
-            const int r0 = rows[c], r1 = rows[c + 1];
-            for (int e = r0; e < r1; e += U) {
-                uint32_t code[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) code[u] = ent[min(e + u, r1 - 1)];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int slot = (int)(code[u] & smask);
-                    const uint32_t t = code[u] >> sb;
-                    float w = sw[(int)(t & tmask) * S + slot] * sw[(WP + (int)((t >> tb) & tmask)) * S + slot];
-                    if (NDIM > 2) w *= sw[(2 * WP + (int)(t >> (2 * tb))) * S + slot];
-                    w = e + u < r1 ? w : 0.0f;
-                    const float2 *r = rec + slot * RCP + j * CPL;
-                    if constexpr (CPL == 2) {
-                        const float4 v = *(const float4 *)r;
-                        ar[i][0] += w * v.x;
-                        ai[i][0] += w * v.y;
-                        ar[i][1] += w * v.z;
-                        ai[i][1] += w * v.w;
-                    } else {
-                        const float2 v = *r;
-                        ar[i][0] += w * v.x;
-                        ai[i][0] += w * v.y;
-                    }
+                for (int k = 0; k < CPL; ++k) {
+                    a.re[k] += w * v[u][k].x;
+                    a.im[k] += w * v[u][k].y;
                 }
             }
         }
-    }
-    // C. store out_coils coil planes at a time through LDS ([coil][CT + 1])
-    float2 *out = (float2 *)smem;
-    for (int cc = 0; cc < ncoil; cc += out_coils) {
-        __syncthreads();
+    };
+    if (it.part == 0) {
+        const int64_t cell0 = it.unit;
+        const int ncells = min(it.e1 & 0xffff, CPI);
+        const int rb = it.slot;
+        const int nent = min(it.e1 >> 16, kMatrixStage);  // <= kMatrixStage by construction
+        for (int i = threadIdx.x; i <= ncells; i += kBlock) srow[i] = min(max(J.mrow[cell0 + i] - rb, 0), nent);
+        for (int i0 = threadIdx.x; i0 < nent; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
+            i32x2 m[kLoadBatch];
 #pragma unroll
-        for (int i = 0; i < kTileMaxCpg; ++i) {
-            const int c = g + i * CPP;
-            if (i >= ncg || c >= CT) continue;
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (i0 + u * kBlock < nent) m[u] = __builtin_nontemporal_load(mnz + rb + i0 + u * kBlock);
 #pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                const int co = j * CPL + q - cc;
-                if (co >= 0 && co < out_coils) out[co * (CT + 1) + c] = make_float2(ar[i][q], ai[i][q]);
-            }
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (i0 + u * kBlock < nent) sent[i0 + u * kBlock] = m[u];
         }
         __syncthreads();
-        const int nco = min(out_coils, ncoil - cc);
-        if (it.slot >= 0) {  // partial `part` of a bin summed in parts: [coil][cell]
-            float2 *dst = (float2 *)(scratch + slot_base[it.slot]) + (int64_t)it.part * ncoil * CT + (int64_t)cc * CT;
-            int co = 0, c = threadIdx.x;
-            while (c >= CT) { c -= CT; ++co; }
-            for (int i = threadIdx.x; i < nco * CT; i += kBlock) {
-                dst[i] = out[co * (CT + 1) + c];
-                c += kBlock;
-                while (c >= CT) { c -= CT; ++co; }
-            }
-            continue;
+        for (int c = g; c < ncells; c += CPP) {
+            Acc a;
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) a.re[k] = a.im[k] = 0.0f;
+            row_sum(sent, false, srow[c], srow[c + 1], 1, a);
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) sout[(j * CPL + k) * (CPI + 1) + c] = make_float2(a.re[k], a.im[k]);
         }
-        Idx4 q(threadIdx.x, kBlock, Tx, Ty, Tz);
-        for (int i = threadIdx.x; i < nco * CT; i += kBlock, q.next()) {
-            if (q.x < ex && q.y < ey && q.z < ez) {
-                grid[J.grid_off + (int64_t)(it.c0 + cc + q.b) * J.grid_bstride +
-                     ((int64_t)(oz + q.z) * J.G[1] + (oy + q.y)) * J.G[0] + (ox + q.x)] =
-                    out[(int)q.b * (CT + 1) + (q.z * Ty + q.y) * Tx + q.x];
+        __syncthreads();
+        for (int i = threadIdx.x; i < RC * CPI; i += kBlock) {
+            const int jj = i / CPI, c = i - jj * CPI;
+            if (c < ncells && it.c0 + jj < it.c1) {
+                const float2 a = sout[jj * (CPI + 1) + c];
+                const f32x2 v = {a.x, a.y};
+                __builtin_nontemporal_store(
+                    v, (f32x2 *)(grid + J.grid_off + (int64_t)(it.c0 + jj) * J.grid_bstride + cell0 + c));
             }
         }
+        return;
     }
+    // a long row: kLongGroups lane groups, group k summing the entries
+    // r0 + k, r0 + k + kLongGroups, ... in order, then the group sums added in
+    // group order; the same partition and order for every record width, so a
+    // cell's value does not depend on which other coils share the table
+    static_assert(CPP >= kLongGroups, "a long row needs kLongGroups lane groups");
+    static_assert(RC * (CPI + 1) >= kLongGroups * RC, "group sums reuse the output stage");
+    const int64_t cell = it.unit;
+    const int r0 = J.mrow[cell] + it.e1 * kMatrixPart, r1 = min(J.mrow[cell + 1], r0 + kMatrixPart);
+    Acc a;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) a.re[k] = a.im[k] = 0.0f;
+    if (g < kLongGroups) {
+        row_sum(mnz, true, r0 + g, r1, kLongGroups, a);
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) sout[g * RC + j * CPL + k] = make_float2(a.re[k], a.im[k]);
+    }
+    __syncthreads();
+    const int c = threadIdx.x;  // coil of the record
+    if (c < RC && it.c0 + c < it.c1) {
+        float2 s = sout[c];
+        for (int k = 1; k < kLongGroups; ++k) {
+            s.x += sout[k * RC + c].x;
+            s.y += sout[k * RC + c].y;
+        }
+        if (it.slot < 0)
+            grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + cell] = s;
+        else
+            parts[((int64_t)it.slot + it.e1) * kMatrixCoils + c] = s;
+    }
+}
+
+// Cells of long rows summed in parts: cell = part 0 + part 1 + ... in part
+// order (item: unit = cell, e1 = number of parts, slot = first record).
+__global__ __launch_bounds__(kWave) void matrix_fold_kernel(const Job *__restrict__ jobs,
+                                                            const Item *__restrict__ items,
+                                                            const float2 *__restrict__ parts,
+                                                            float2 *__restrict__ grid) {
+    const Item it = items[blockIdx.x];
+    const Job &J = jobs[it.job];
+    const int c = threadIdx.x;
+    if (c >= it.c1 - it.c0) return;
+    float2 a = parts[(int64_t)it.slot * kMatrixCoils + c];
+    for (int k = 1; k < it.e1; ++k) {
+        const float2 p = parts[((int64_t)it.slot + k) * kMatrixCoils + c];
+        a.x += p.x;
+        a.y += p.y;
+    }
+    grid[J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + it.unit] = a;
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
@@ -1256,42 +1428,78 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const
     return TN_OK;
 }
 
-using TileFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, float *,
-                       const int64_t *, LowRank, int, int);
+using MatrixFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, const float *, LowRank,
+                          float2 *);
 
-template <int NDIM>
-TileFn pick_tile(int coils) {  // record width: 1, 2, 4, 8 or 16 coils
-    switch (coils) {
-        case 1: return grid_tile_kernel<NDIM, 0, 1>;
-        case 2: return grid_tile_kernel<NDIM, 0, 2>;
-        case 4: return grid_tile_kernel<NDIM, 1, 2>;
-        case 8: return grid_tile_kernel<NDIM, 2, 2>;
-        case 16: return grid_tile_kernel<NDIM, 3, 2>;
-        default: return nullptr;
-    }
-}
-
-int launch_fold(const LaunchTables &t, float2 *grid, float *scratch, hipStream_t stream) {
+int launch_matrix_fold(const LaunchTables &t, float2 *grid, float *scratch, hipStream_t stream) {
     if (t.n_fold == 0) return TN_OK;
-    hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs, t.d_fold,
-                       scratch, t.d_slot_base, grid);
+    hipLaunchKernelGGL(matrix_fold_kernel, dim3((unsigned)t.n_fold), dim3(kWave), 0, stream, t.d_jobs, t.d_fold,
+                       (const float2 *)scratch, grid);
     TN_HIP(hipGetLastError());
     return TN_OK;
+}
+
+template <bool LR>
+MatrixFn pick_matrix(int coils) {
+    if (coils <= 1) return grid_matrix_kernel<1, LR>;
+    if (coils <= 2) return grid_matrix_kernel<2, LR>;
+    if (coils <= 4) return grid_matrix_kernel<4, LR>;
+    if (coils <= 8) return grid_matrix_kernel<8, LR>;
+    return grid_matrix_kernel<kMatrixCoils, LR>;
 }
 
 int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const float *weight,
                 hipStream_t stream, LowRank lr, float *scratch) {
     if (t.n_grid == 0) return TN_OK;
     if (scratch == nullptr) scratch = t.d_scratch;
-    if (t.grid_mode == kGridMatrix) {
-        const int ndim = t.kind_grid / 16;
-        TileFn fn = ndim == 2 ? pick_tile<2>(t.tile_coils) : ndim == 3 ? pick_tile<3>(t.tile_coils) : nullptr;
-        if (!fn) return fail(TN_E_ARG, "tile matrix: unsupported record (ndim=%d, coils=%d)", ndim, t.tile_coils);
-        hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), t.grid_lds, stream, t.d_jobs, t.d_grid,
-                           t.n_grid, samp, grid, weight, scratch, t.d_slot_base, lr, t.tile_region_floats,
-                           t.tile_out_coils);
+    if (t.grid_mode == kGridMatrix && lr.ncoeff == 0 && t.st_cw > 0) {
+        using TrFn = void (*)(const Job *, const Item *, const float2 *, const float *, float2 *);
+        using MtFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, float2 *);
+        TrFn tr = nullptr;
+        MtFn mt = nullptr;
+        const bool pair = t.st_cpl == 2;  // two coils per lane (the table's cells per item assume it)
+        // entries in flight per lane: 4 in 2D, 8 in 3D (each measured fastest)
+        const bool d3 = t.kind_grid / 16 == 3;
+        switch (t.st_cw) {
+            case 2:
+                tr = samp_transpose_kernel<1>;
+                mt = d3 ? grid_matrix_t_kernel<1, 1, 8> : grid_matrix_t_kernel<1, 1, 4>;
+                break;
+            case 4:
+                tr = samp_transpose_kernel<2>;
+                mt = pair ? (d3 ? grid_matrix_t_kernel<1, 2, 8> : grid_matrix_t_kernel<1, 2, 4>)
+                          : (d3 ? grid_matrix_t_kernel<2, 1, 8> : grid_matrix_t_kernel<2, 1, 4>);
+                break;
+            case 8:
+                tr = samp_transpose_kernel<3>;
+                mt = pair ? (d3 ? grid_matrix_t_kernel<2, 2, 8> : grid_matrix_t_kernel<2, 2, 4>)
+                          : (d3 ? grid_matrix_t_kernel<3, 1, 8> : grid_matrix_t_kernel<3, 1, 4>);
+                break;
+            case 16:
+                tr = samp_transpose_kernel<4>;  // 8 lanes x 2 coils
+                mt = d3 ? grid_matrix_t_kernel<3, 2, 8> : grid_matrix_t_kernel<3, 2, 4>;
+                break;
+            default: return fail(TN_E_ARG, "matrix gridding: %d coils per record", t.st_cw);
+        }
+        if (t.st_cw == 16 ? !pair : (pair && t.st_cw < 4))
+            return fail(TN_E_ARG, "matrix gridding: %d coils per lane for %d-coil records", t.st_cpl, t.st_cw);
+        if (t.n_tr > 0) {
+            hipLaunchKernelGGL(tr, dim3((unsigned)t.n_tr), dim3(kBlock), 0, stream, t.d_jobs, t.d_tr, samp, weight,
+                               t.d_st);
+            TN_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(mt, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid,
+                           (const float2 *)t.d_st, grid, (float2 *)scratch);
         TN_HIP(hipGetLastError());
-        return launch_fold(t, grid, scratch, stream);
+        return launch_matrix_fold(t, grid, scratch, stream);
+    }
+    if (t.grid_mode == kGridMatrix) {
+        if (t.grid_coils > kMatrixCoils) return fail(TN_E_ARG, "matrix gridding: %d coils per item", t.grid_coils);
+        MatrixFn fn = lr.ncoeff > 0 ? pick_matrix<true>(t.grid_coils) : pick_matrix<false>(t.grid_coils);
+        hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid, samp,
+                           grid, weight, lr, (float2 *)scratch);
+        TN_HIP(hipGetLastError());
+        return launch_matrix_fold(t, grid, scratch, stream);
     }
     const int ndim = t.kind_grid / 16, wc = t.kind_grid % 16;
     GridFn fn = nullptr;
@@ -1303,7 +1511,12 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
     hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_grid)), dim3(kBlock), t.grid_lds, stream, t.d_jobs,
                        t.d_grid, t.n_grid, samp, grid, weight, scratch, t.d_slot_base, lr);
     TN_HIP(hipGetLastError());
-    return launch_fold(t, grid, scratch, stream);
+    if (t.n_fold > 0) {
+        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
+                           t.d_fold, scratch, t.d_slot_base, grid);
+        TN_HIP(hipGetLastError());
+    }
+    return TN_OK;
 }
 
 int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *samp, const float *weight,

@@ -48,6 +48,8 @@ __device__ __forceinline__ int axis_tiles(const AxisGeom &a, int base, int (&tl)
     return n;
 }
 
+__device__ __forceinline__ float beta_of(const BinGeom &g, int d) { return g.beta[d]; }
+
 __device__ __forceinline__ int axis_base(const AxisGeom &a, float g) {
     if (a.W == 1) return wrap_mod((int)g + a.shift, a.G);
     const float x0 = ceilf(g - 0.5f * (float)a.W);
@@ -123,6 +125,43 @@ __global__ void bin_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
         const int64_t k = e < n ? (int64_t)keys[e] : nbins;
         const int64_t kp = e > 0 ? (int64_t)keys[e - 1] : -1;
         for (int64_t b = kp + 1; b <= k; ++b) bin_off[b] = (int32_t)e;
+    }
+}
+
+// Gridding matrix entries of every sample, in (sample, tap) order, tap x
+// fastest: key = frame * cells + output cell, value = (sample within its
+// frame, weight bits) packed low/high in 64 bits. The weight is the product
+// of the per-axis Kaiser-Bessel taps (z * y first, then x), the same taps
+// the other gridding variants evaluate.
+__global__ void matrix_emit_kernel(BinGeom geo, const float *__restrict__ coord, int taps,
+                                   uint32_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+    const int64_t total = geo.npts * geo.nframes;
+    const int64_t ncell = (int64_t)geo.ax[0].G * geo.ax[1].G * geo.ax[2].G;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frame = i / geo.npts;
+        const int64_t p = i - frame * geo.npts;
+        int base[3] = {0, 0, 0};
+        float w[3][8];
+        for (int d = 0; d < 3; ++d) {
+            if (d < geo.ndim) {
+                axis_taps<8>(coord[i * geo.ndim + d], geo.ax[d].W, geo.ax[d].G, geo.ax[d].shift, beta_of(geo, d),
+                             base[d], w[d]);
+            } else {
+                w[d][0] = 1.0f;
+            }
+        }
+        const int Wx = geo.ax[0].W, Wy = geo.ax[1].W;
+        const int64_t e0 = i * taps;
+        for (int t = 0; t < taps; ++t) {
+            const int kx = t % Wx, ky = (t / Wx) % Wy, kz = t / (Wx * Wy);
+            const int64_t cell = ((int64_t)wrap_add(base[2], kz, geo.ax[2].G) * geo.ax[1].G +
+                                  wrap_add(base[1], ky, geo.ax[1].G)) * geo.ax[0].G +
+                                 wrap_add(base[0], kx, geo.ax[0].G);
+            const float wt = (w[2][kz] * w[1][ky]) * w[0][kx];
+            keys[e0 + t] = (uint32_t)(frame * ncell + cell);
+            vals[e0 + t] = ((uint64_t)(uint32_t)__float_as_int(wt) << 32) | (uint64_t)(uint32_t)p;
+        }
     }
 }
 
@@ -366,289 +405,6 @@ int order_by_origin(const BinGeom &geo, const uint32_t *bin_keys, int64_t n, flo
     return TN_OK;
 }
 
-// ---- tile matrix (grid_mode 3) ----------------------------------------------
-// Gridding of a tile sums, for every cell, weight * value over the samples of
-// the tile's bin whose footprint covers the cell. The bin (samples in
-// ascending order) is cut into batches of `tslots` slots; per batch and cell
-// the matrix lists the entries (slot in the batch | tap code), in slot order.
-// The tap code is the footprint tap per axis, tbits bits each (x lowest), so
-// the gridding kernel forms the weight from the slot's per-axis taps.
-
-struct TileGeom {
-    BinGeom g;
-    int tslots, sbits, tbits;
-    int cells;  // tile cells T0 * T1 * T2 (edge tiles included in full)
-};
-
-__device__ __forceinline__ void tile_of_key(const BinGeom &g, uint32_t key, int64_t &frame, int (&t)[3]) {
-    frame = key / g.ntiles;
-    int64_t tile = key - frame * g.ntiles;
-    t[0] = (int)(tile % g.NT[0]);
-    tile /= g.NT[0];
-    t[1] = (int)(tile % g.NT[1]);
-    t[2] = (int)(tile / g.NT[1]);
-}
-
-// Batches per bin: ceil(bin size / tslots).
-__global__ void batch_count_kernel(const int32_t *__restrict__ bin_off, int64_t nbins, int tslots,
-                                   int32_t *__restrict__ nb) {
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nbins; k += (int64_t)gridDim.x * blockDim.x)
-        nb[k] = (bin_off[k + 1] - bin_off[k] + tslots - 1) / tslots;
-}
-
-// Footprint taps of bin entry e that fall inside its tile (per axis: the
-// taps k whose wrapped index lies in the tile), as (count, per-axis lists).
-__device__ __forceinline__ int tile_taps(const TileGeom &tg, const float *coord, int64_t frame, int32_t pt,
-                                         const int (&t)[3], int (&kk)[3][kMaxWidth], int (&nk)[3]) {
-    const BinGeom &g = tg.g;
-    int n = 1;
-    for (int d = 0; d < 3; ++d) {
-        nk[d] = 1;
-        kk[d][0] = 0;
-        if (d >= g.ndim) continue;
-        const AxisGeom &a = g.ax[d];
-        const int base = axis_base(a, coord[(frame * g.npts + pt) * g.ndim + d]);
-        int m = 0;
-        for (int k = 0; k < a.W; ++k)
-            if (wrap_add(base, k, a.G) / a.T == t[d]) kk[d][m++] = k;
-        nk[d] = m;
-        n *= m;
-    }
-    return n;
-}
-
-__global__ void tile_tap_count_kernel(TileGeom tg, const float *__restrict__ coord, const uint32_t *__restrict__ keys,
-                                      const int32_t *__restrict__ bins, int64_t n, int32_t *__restrict__ cnt) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        int64_t frame;
-        int t[3];
-        tile_of_key(tg.g, keys[e], frame, t);
-        int kk[3][kMaxWidth], nk[3];
-        cnt[e] = tile_taps(tg, coord, frame, bins[e], t, kk, nk);
-    }
-}
-
-__global__ void tile_emit_kernel(TileGeom tg, const float *__restrict__ coord, const uint32_t *__restrict__ keys,
-                                 const int32_t *__restrict__ bins, const int32_t *__restrict__ bin_off,
-                                 const int32_t *__restrict__ batch_off, const int32_t *__restrict__ eoff, int64_t n,
-                                 uint32_t *__restrict__ okey, uint16_t *__restrict__ oval) {
-    const BinGeom &g = tg.g;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t key = keys[e];
-        int64_t frame;
-        int t[3];
-        tile_of_key(g, key, frame, t);
-        int kk[3][kMaxWidth], nk[3];
-        tile_taps(tg, coord, frame, bins[e], t, kk, nk);
-        const int slot = (int)(e - bin_off[key]);
-        const int64_t batch = (int64_t)batch_off[key] + slot / tg.tslots;
-        const uint32_t s = (uint32_t)(slot % tg.tslots);
-        int base[3] = {0, 0, 0};
-        for (int d = 0; d < g.ndim; ++d)
-            base[d] = axis_base(g.ax[d], coord[(frame * g.npts + bins[e]) * g.ndim + d]);
-        const int T0 = g.ax[0].T, T1 = g.ndim > 1 ? g.ax[1].T : 1;
-        int64_t pos = eoff[e];
-        for (int a = 0; a < nk[2]; ++a) {
-            const int kz = kk[2][a];
-            const int lz = g.ndim > 2 ? wrap_add(base[2], kz, g.ax[2].G) - t[2] * g.ax[2].T : 0;
-            for (int b = 0; b < nk[1]; ++b) {
-                const int ky = kk[1][b];
-                const int ly = g.ndim > 1 ? wrap_add(base[1], ky, g.ax[1].G) - t[1] * T1 : 0;
-                for (int c = 0; c < nk[0]; ++c) {
-                    const int kx = kk[0][c];
-                    const int lx = wrap_add(base[0], kx, g.ax[0].G) - t[0] * T0;
-                    const int cell = (lz * T1 + ly) * T0 + lx;
-                    okey[pos] = (uint32_t)(batch * tg.cells + cell);
-                    const uint32_t code = (uint32_t)kx | ((uint32_t)ky << tg.tbits) | ((uint32_t)kz << (2 * tg.tbits));
-                    oval[pos] = (uint16_t)(s | (code << tg.sbits));
-                    ++pos;
-                }
-            }
-        }
-    }
-}
-
-// Per batch: entry base and the cells + 1 row starts relative to it.
-__global__ void tile_rows_kernel(const int32_t *__restrict__ rowstart, int64_t nbatch, int cells,
-                                 int32_t *__restrict__ mbase, uint16_t *__restrict__ mrow) {
-    const int64_t total = nbatch * (cells + 1);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = i / (cells + 1);
-        const int c = (int)(i - b * (cells + 1));
-        const int32_t b0 = rowstart[b * cells];
-        mrow[i] = (uint16_t)(rowstart[b * cells + c] - b0);
-        if (c == 0) mbase[b] = b0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) mbase[nbatch] = rowstart[nbatch * cells];
-}
-
-template <class T>
-int device_scan(const int32_t *in, T *out, int64_t n, hipStream_t stream, int64_t *total) {
-    // exclusive sum into out[0..n], out[n] = total
-    DevBuf tmp;
-    size_t bytes = 0;
-    TN_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, stream));
-    TN_HIP(hipMalloc(&tmp.p, std::max<size_t>(bytes, 1)));
-    TN_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, (int)n, stream));
-    T last_out = 0;
-    int32_t last_in = 0;
-    if (n > 0) {
-        TN_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(T), hipMemcpyDeviceToHost, stream));
-        TN_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-    }
-    TN_HIP(hipStreamSynchronize(stream));
-    *total = (int64_t)last_out + last_in;
-    const T tot = (T)*total;
-    TN_HIP(hipMemcpyAsync(out + n, &tot, sizeof(T), hipMemcpyHostToDevice, stream));
-    TN_HIP(hipStreamSynchronize(stream));
-    return TN_OK;
-}
-
-}  // namespace
-
-// Slot / tap-code split of the 16-bit entries and the batch size of a plan.
-void tile_matrix_params(const tn_plan *plan, int *tslots, int *sbits, int *tbits) {
-    const tn_plan_params &p = plan->p;
-    int tb = 0, in_tile = 1;
-    for (int d = 3 - p.ndim; d < 3; ++d) {
-        while ((1 << tb) < p.width[d]) ++tb;
-        in_tile *= std::min(p.width[d], plan->tile[d]);
-    }
-    const int sb = 16 - p.ndim * tb;
-    int s = std::min(kTileSlots, 1 << std::max(sb, 0));
-    while (s > 1 && (int64_t)s * in_tile > 65535) s >>= 1;  // batch entries fit the 16-bit row starts
-    *tslots = s;
-    *sbits = sb;
-    *tbits = tb;
-}
-
-int build_tile_matrix(tn_plan *plan, hipStream_t stream) {
-    TileGeom tg{};
-    tg.g = make_geom(plan);
-    tile_matrix_params(plan, &tg.tslots, &tg.sbits, &tg.tbits);
-    plan->tslots = tg.tslots;
-    plan->sbits = tg.sbits;
-    plan->tbits = tg.tbits;
-    if (tg.sbits < 1) return fail(TN_E_ARG, "tile matrix: kernel width too large for 16-bit entries");
-    tg.cells = plan->tile[0] * plan->tile[1] * plan->tile[2];
-    const BinGeom &geo = tg.g;
-    const int64_t total = geo.npts * geo.nframes;
-    const int64_t nbins = geo.nframes * geo.ntiles;
-    // ---- bins: every (tile, sample) pair, sorted by (frame, tile, sample)
-    int64_t n = 0;
-    DevBuf keys_sorted;
-    if (total > 0) {
-        DevBuf counts, offsets, keys, vals, sort_tmp;
-        TN_HIP(hipMalloc(&counts.p, sizeof(int32_t) * total));
-        TN_HIP(hipMalloc(&offsets.p, sizeof(int32_t) * (total + 1)));
-        hipLaunchKernelGGL(bin_count_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo, plan->d_coord,
-                           (int32_t *)counts.p);
-        TN_HIP(hipGetLastError());
-        int rc = device_scan((const int32_t *)counts.p, (int32_t *)offsets.p, total, stream, &n);
-        if (rc) return rc;
-        if (n >= INT32_MAX) return fail(TN_E_RANGE, "too many tile entries (%lld)", (long long)n);
-        TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * std::max<int64_t>(n, 1)));
-        TN_HIP(hipMalloc(&vals.p, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-        TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * std::max<int64_t>(n, 1)));
-        TN_HIP(hipMalloc(&plan->d_bins, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-        hipLaunchKernelGGL(bin_emit_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo, plan->d_coord,
-                           (const int32_t *)offsets.p, (uint32_t *)keys.p, (int32_t *)vals.p);
-        TN_HIP(hipGetLastError());
-        int end_bit = 1;
-        while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)nbins) ++end_bit;
-        size_t bytes = 0;
-        TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t *)keys.p, (uint32_t *)keys_sorted.p,
-                                                  (int32_t *)vals.p, plan->d_bins, (int)n, 0, end_bit, stream));
-        TN_HIP(hipMalloc(&sort_tmp.p, std::max<size_t>(bytes, 1)));
-        TN_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, bytes, (uint32_t *)keys.p, (uint32_t *)keys_sorted.p,
-                                                  (int32_t *)vals.p, plan->d_bins, (int)n, 0, end_bit, stream));
-        hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(n + 1)), dim3(kBlock), 0, stream,
-                           (const uint32_t *)keys_sorted.p, n, nbins, plan->d_bin_off);
-        TN_HIP(hipGetLastError());
-    } else {
-        TN_HIP(hipMalloc(&plan->d_bins, sizeof(int32_t)));
-        TN_HIP(hipMemsetAsync(plan->d_bin_off, 0, sizeof(int32_t) * (nbins + 1), stream));
-    }
-    plan->nentries = n;
-    // ---- batches per bin
-    TN_HIP(hipMalloc(&plan->d_batch_off, sizeof(int32_t) * (nbins + 1)));
-    int64_t nbatch = 0;
-    {
-        DevBuf nb;
-        TN_HIP(hipMalloc(&nb.p, sizeof(int32_t) * std::max<int64_t>(nbins, 1)));
-        hipLaunchKernelGGL(batch_count_kernel, dim3(grid_blocks(nbins)), dim3(kBlock), 0, stream, plan->d_bin_off,
-                           nbins, tg.tslots, (int32_t *)nb.p);
-        TN_HIP(hipGetLastError());
-        int rc = device_scan((const int32_t *)nb.p, plan->d_batch_off, nbins, stream, &nbatch);
-        if (rc) return rc;
-    }
-    plan->nbatches = nbatch;
-    if (nbatch * tg.cells >= (int64_t)UINT32_MAX)
-        return fail(TN_E_RANGE, "tile matrix: too many rows (%lld batches)", (long long)nbatch);
-    // ---- entries: (batch, cell) keys, slot | tap code values
-    int64_t ne = 0;
-    DevBuf okey_sorted;
-    if (n > 0) {
-        DevBuf cnt, eoff, okey, oval, sort_tmp;
-        TN_HIP(hipMalloc(&cnt.p, sizeof(int32_t) * n));
-        TN_HIP(hipMalloc(&eoff.p, sizeof(int32_t) * (n + 1)));
-        hipLaunchKernelGGL(tile_tap_count_kernel, dim3(grid_blocks(n)), dim3(kBlock), 0, stream, tg, plan->d_coord,
-                           (const uint32_t *)keys_sorted.p, plan->d_bins, n, (int32_t *)cnt.p);
-        TN_HIP(hipGetLastError());
-        int rc = device_scan((const int32_t *)cnt.p, (int32_t *)eoff.p, n, stream, &ne);
-        if (rc) return rc;
-        if (ne >= INT32_MAX) return fail(TN_E_RANGE, "tile matrix: too many entries (%lld)", (long long)ne);
-        TN_HIP(hipMalloc(&okey.p, sizeof(uint32_t) * ne));
-        TN_HIP(hipMalloc(&oval.p, sizeof(uint16_t) * ne));
-        TN_HIP(hipMalloc(&okey_sorted.p, sizeof(uint32_t) * ne));
-        TN_HIP(hipMalloc(&plan->d_ment, sizeof(uint16_t) * ne));
-        hipLaunchKernelGGL(tile_emit_kernel, dim3(grid_blocks(n)), dim3(kBlock), 0, stream, tg, plan->d_coord,
-                           (const uint32_t *)keys_sorted.p, plan->d_bins, plan->d_bin_off, plan->d_batch_off,
-                           (const int32_t *)eoff.p, n, (uint32_t *)okey.p, (uint16_t *)oval.p);
-        TN_HIP(hipGetLastError());
-        const uint64_t nrows = (uint64_t)nbatch * tg.cells;
-        int end_bit = 1;
-        while (end_bit < 32 && ((uint64_t)1 << end_bit) < nrows) ++end_bit;
-        size_t bytes = 0;
-        TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t *)okey.p, (uint32_t *)okey_sorted.p,
-                                                  (uint16_t *)oval.p, plan->d_ment, (int)ne, 0, end_bit, stream));
-        TN_HIP(hipMalloc(&sort_tmp.p, std::max<size_t>(bytes, 1)));
-        TN_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, bytes, (uint32_t *)okey.p, (uint32_t *)okey_sorted.p,
-                                                  (uint16_t *)oval.p, plan->d_ment, (int)ne, 0, end_bit, stream));
-        TN_HIP(hipStreamSynchronize(stream));  // the sort's temporaries are freed on scope exit
-    } else {
-        TN_HIP(hipMalloc(&plan->d_ment, sizeof(uint16_t)));
-    }
-    plan->mnnz = ne;
-    // ---- per-batch entry bases and row starts
-    TN_HIP(hipMalloc(&plan->d_mbase, sizeof(int32_t) * (nbatch + 1)));
-    TN_HIP(hipMalloc(&plan->d_mrow, sizeof(uint16_t) * std::max<int64_t>(nbatch * (tg.cells + 1), 1)));
-    if (nbatch > 0) {
-        DevBuf rowstart;
-        const int64_t nrows = nbatch * tg.cells;
-        TN_HIP(hipMalloc(&rowstart.p, sizeof(int32_t) * (nrows + 1)));
-        hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(ne + 1)), dim3(kBlock), 0, stream,
-                           (const uint32_t *)okey_sorted.p, ne, nrows, (int32_t *)rowstart.p);
-        TN_HIP(hipGetLastError());
-        hipLaunchKernelGGL(tile_rows_kernel, dim3(grid_blocks(nbatch * (tg.cells + 1))), dim3(kBlock), 0, stream,
-                           (const int32_t *)rowstart.p, nbatch, tg.cells, plan->d_mbase, plan->d_mrow);
-        TN_HIP(hipGetLastError());
-        TN_HIP(hipStreamSynchronize(stream));
-    } else {
-        TN_HIP(hipMemsetAsync(plan->d_mbase, 0, sizeof(int32_t), stream));
-    }
-    plan->h_bin_off.assign(nbins + 1, 0);
-    plan->h_batch_off.assign(nbins + 1, 0);
-    TN_HIP(hipMemcpyAsync(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1), hipMemcpyDeviceToHost,
-                          stream));
-    TN_HIP(hipMemcpyAsync(plan->h_batch_off.data(), plan->d_batch_off, sizeof(int32_t) * (nbins + 1),
-                          hipMemcpyDeviceToHost, stream));
-    TN_HIP(hipStreamSynchronize(stream));
-    return TN_OK;
-}
-
-namespace {
-
 }  // namespace
 
 int build_bins(tn_plan *plan, hipStream_t stream) {
@@ -669,13 +425,17 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
         TN_HIP(hipMemsetAsync(plan->d_pbin_off, 0, sizeof(int32_t) * (npbins + 1), stream));
         TN_HIP(hipStreamSynchronize(stream));
         plan->nentries = 0;
-        if (plan->grid_mode == kGridMatrix) return build_tile_matrix(plan, stream);  // empty bins, zero batches
+        // an empty gridding matrix: all-zero row offsets, so every cell is
+        // written as zero
+        if (plan->grid_mode == kGridMatrix) return build_matrix(plan, stream);
         return TN_OK;
     }
 
     // ---- gridding bins: every tile a footprint touches --------------------
-    if (plan->grid_mode == kGridMatrix) {
-        int rc = build_tile_matrix(plan, stream);
+    if (plan->grid_mode == kGridMatrix) {  // gridding reads the matrix instead
+        TN_HIP(hipMemsetAsync(plan->d_bin_off, 0, sizeof(int32_t) * (nbins + 1), stream));
+        plan->nentries = 0;
+        int rc = build_matrix(plan, stream);
         if (rc) return rc;
     } else {
         DevBuf counts, offsets, scan_tmp;
@@ -728,6 +488,55 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
         TN_HIP(hipMemcpy(plan->h_pbin_off.data(), plan->d_pbin_off, sizeof(int32_t) * (npbins + 1),
                          hipMemcpyDeviceToHost));
     }
+    return TN_OK;
+}
+
+// Sparse gridding matrix of every frame, CSR by output cell: the entries of
+// every sample are sorted by (frame, cell) with a stable radix sort, so the
+// samples of one cell stay in ascending order (a fixed summation order).
+int build_matrix(tn_plan *plan, hipStream_t stream) {
+    const BinGeom geo = make_geom(plan);
+    const int64_t total = geo.npts * geo.nframes;
+    const int64_t ncell = (int64_t)plan->p.grid[0] * plan->p.grid[1] * plan->p.grid[2];
+    const int64_t nrows = ncell * geo.nframes;
+    const int taps = plan->p.width[0] * plan->p.width[1] * plan->p.width[2];
+    const int64_t n = total * taps;
+    if (nrows >= (int64_t)UINT32_MAX) return fail(TN_E_RANGE, "gridding matrix: too many rows (%lld)", (long long)nrows);
+    if (n >= INT32_MAX) return fail(TN_E_RANGE, "gridding matrix: too many entries (%lld)", (long long)n);
+    if (geo.npts > ((int64_t)1 << 28)) return fail(TN_E_RANGE, "gridding matrix: too many samples per frame");
+    TN_HIP(hipMalloc(&plan->d_mrow, sizeof(int32_t) * (nrows + 1)));
+    TN_HIP(hipMalloc(&plan->d_mnz, sizeof(int2) * std::max<int64_t>(n, 1)));
+    plan->mnnz = n;
+    if (n == 0) {
+        TN_HIP(hipMemsetAsync(plan->d_mrow, 0, sizeof(int32_t) * (nrows + 1), stream));
+        TN_HIP(hipStreamSynchronize(stream));
+        plan->h_mrow.assign(nrows + 1, 0);
+        return TN_OK;
+    }
+    DevBuf keys, vals, keys_sorted, tmp;
+    TN_HIP(hipMalloc(&keys.p, sizeof(uint32_t) * n));
+    TN_HIP(hipMalloc(&vals.p, sizeof(uint64_t) * n));
+    TN_HIP(hipMalloc(&keys_sorted.p, sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(matrix_emit_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo, plan->d_coord,
+                       taps, (uint32_t *)keys.p, (uint64_t *)vals.p);
+    TN_HIP(hipGetLastError());
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) < (uint64_t)nrows) ++end_bit;
+    size_t bytes = 0;
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t *)keys.p, (uint32_t *)keys_sorted.p,
+                                              (uint64_t *)vals.p, (uint64_t *)plan->d_mnz, (int)n, 0, end_bit,
+                                              stream));
+    TN_HIP(hipMalloc(&tmp.p, bytes));
+    TN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, (uint32_t *)keys.p, (uint32_t *)keys_sorted.p,
+                                              (uint64_t *)vals.p, (uint64_t *)plan->d_mnz, (int)n, 0, end_bit,
+                                              stream));
+    hipLaunchKernelGGL(bin_offsets_kernel, dim3(grid_blocks(n + 1)), dim3(kBlock), 0, stream,
+                       (const uint32_t *)keys_sorted.p, n, nrows, plan->d_mrow);
+    TN_HIP(hipGetLastError());
+    plan->h_mrow.resize(nrows + 1);
+    TN_HIP(hipMemcpyAsync(plan->h_mrow.data(), plan->d_mrow, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost,
+                          stream));
+    TN_HIP(hipStreamSynchronize(stream));  // temporaries are freed on scope exit
     return TN_OK;
 }
 

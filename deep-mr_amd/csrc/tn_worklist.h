@@ -1,6 +1,6 @@
 // Work items of the gridding and interpolation launches, and the host-side
-// construction of the tile-matrix gridding items. Plain C++ (no HIP types),
-// so the construction can be checked by a host-only program.
+// construction of the sparse-matrix gridding items. Plain C++ (no HIP
+// types), so the construction can be checked by a host-only program.
 #pragma once
 
 #include <algorithm>
@@ -9,42 +9,86 @@
 
 namespace tn {
 
+constexpr int kMatrixCoils = 16;   // widest record of the matrix gridding (coils per item)
+constexpr int kMatrixStage = 1024; // entries of a run of cells staged in LDS (also the longest short row)
+constexpr int kMatrixPart = 2048;  // entries per workgroup of a long row; longer rows are summed
+                                   // in parts (kMatrixCoils-wide partial records, folded in order)
+
 // Work item. Gathering interpolation: unit = sample block. Staged
 // interpolation: unit = tile, [e0, e1) = slice of its entries. Gridding:
-// unit = tile, [e0, e1) = slice of its entries (tile matrix: of its batches);
-// part / slot >= 0 when the tile is summed in parts (partial `part` of fold
-// slot `slot`), slot = -1 otherwise. Fold items: unit = tile, e0 = slot,
-// e1 = number of parts.
+// unit = tile, [e0, e1) = slice of its entries; part / slot >= 0 when the
+// tile is split (partial `part` of fold slot `slot`), slot = -1 otherwise.
+// Fold items: unit = tile, e0 = slot, e1 = number of parts. Matrix gridding
+// items: see matrix_items.
 struct alignas(16) Item {
     int32_t job, unit, c0, c1;
     int32_t e0, e1, part, slot;
 };
 
-// Tile-matrix gridding items of one job (frame) with `nbatch` coils:
-// for every tile t (batch_off[t + 1] - batch_off[t] batches of its bin) and
-// coil record [c0, c0 + rc): one item over all batches when there are at
-// most kb, else ceil(nb / kb) part items of kb consecutive batches, each
-// leaving a [coil][cell] partial of the tile (2 * coils * cells floats) in
-// the scratch region of its fold slot, and one fold item that sums the parts
-// in order. Appends to grid / fold / slot_base and advances scratch (floats).
-inline void tile_matrix_items(int job, const int32_t *batch_off, int64_t ntiles, int nbatch, int rc, int kb,
-                              int64_t cells, std::vector<Item> &grid, std::vector<Item> &fold,
-                              std::vector<int64_t> &slot_base, int64_t &scratch) {
-    for (int64_t tl = 0; tl < ntiles; ++tl) {
-        const int nb = batch_off[tl + 1] - batch_off[tl];
-        const int nparts = nb <= kb ? 1 : (nb + kb - 1) / kb;
-        for (int c0 = 0; c0 < nbatch; c0 += rc) {
-            const int c1 = std::min(nbatch, c0 + rc);
-            if (nparts == 1) {
-                grid.push_back(Item{job, (int32_t)tl, c0, c1, 0, nb, 0, -1});
+// Matrix-gridding work of one frame, independent of the coils: a run of
+// ncells consecutive cells from `cell` whose entries are [e0, e0 + ne), or,
+// with ncells < 0, a long row `cell` summed in -ncells parts.
+struct MatrixRun {
+    int32_t cell, ncells, e0, ne;
+};
+
+// Runs of one frame from its CSR row offsets row[0 .. ncell]: consecutive
+// cells with at most cpi cells and at most kMatrixStage entries per run, so
+// that a run's entries (contiguous in the CSR order) fit the kernel's LDS
+// stage; a row longer than mlong (<= kMatrixStage) ends the run and becomes
+// a long-row run of ceil(len / kMatrixPart) parts. Long rows are listed first.
+inline void matrix_runs_of(const int32_t *row, int64_t ncell, int cpi, int mlong, std::vector<MatrixRun> &out) {
+    std::vector<MatrixRun> shorts;
+    int64_t start = 0;
+    int ncells = 0, nent = 0;
+    auto close = [&]() {
+        if (ncells > 0) shorts.push_back(MatrixRun{(int32_t)start, ncells, row[start], row[start + ncells] - row[start]});
+        ncells = 0;
+        nent = 0;
+    };
+    for (int64_t c = 0; c < ncell; ++c) {
+        const int len = row[c + 1] - row[c];
+        if (len > mlong) {
+            close();
+            out.push_back(MatrixRun{(int32_t)c, -((len + kMatrixPart - 1) / kMatrixPart), row[c], len});
+            continue;
+        }
+        if (ncells == cpi || nent + len > kMatrixStage) close();
+        if (ncells == 0) start = c;
+        ++ncells;
+        nent += len;
+    }
+    close();
+    out.insert(out.end(), shorts.begin(), shorts.end());
+}
+
+// Gridding items of one job (frame, `nbatch` coils in records of mc coils)
+// from its runs. A run of cells: unit = first cell, e1 = cell count | entry
+// count << 16, slot = first entry, e0 = coil record, part = 0. A long row:
+// one item per part (unit = cell, e1 = part, part = 1, e0 = coil record);
+// with several parts, slot = first partial record (kMatrixCoils complex
+// values each, 2 * kMatrixCoils floats of scratch) and one fold item (e1 =
+// number of parts, part = 0, slot = first record) sums them in part order;
+// slot = -1 for a single part. Appends to grid / fold, advances scratch
+// (floats).
+inline void matrix_items(int job, const std::vector<MatrixRun> &runs, int nbatch, int mc, std::vector<Item> &grid,
+                         std::vector<Item> &fold, int64_t &scratch) {
+    for (const MatrixRun &r : runs) {
+        for (int c0 = 0; c0 < nbatch; c0 += mc) {
+            const int c1 = std::min(nbatch, c0 + mc);
+            if (r.ncells > 0) {
+                grid.push_back(Item{job, r.cell, c0, c1, c0 / mc, r.ncells | (r.ne << 16), 0, r.e0});
                 continue;
             }
-            const int slot = (int)slot_base.size();
-            slot_base.push_back(scratch);
-            scratch += (int64_t)nparts * 2 * (c1 - c0) * cells;
-            for (int q = 0; q < nparts; ++q)
-                grid.push_back(Item{job, (int32_t)tl, c0, c1, q * kb, std::min(nb, (q + 1) * kb), q, slot});
-            fold.push_back(Item{job, (int32_t)tl, c0, c1, slot, nparts, 0, 0});
+            const int nparts = -r.ncells;
+            if (nparts == 1) {
+                grid.push_back(Item{job, r.cell, c0, c1, c0 / mc, 0, 1, -1});
+                continue;
+            }
+            const int32_t rec = (int32_t)(scratch / (2 * kMatrixCoils));
+            scratch += (int64_t)nparts * 2 * kMatrixCoils;
+            for (int k = 0; k < nparts; ++k) grid.push_back(Item{job, r.cell, c0, c1, c0 / mc, k, 1, rec});
+            fold.push_back(Item{job, r.cell, c0, c1, c0 / mc, nparts, 0, rec});
         }
     }
 }

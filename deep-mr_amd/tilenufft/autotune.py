@@ -23,16 +23,13 @@ from .fft.nufft import retune
 def default_candidates(ndim):
     if ndim == 2:
         tiles = [(32, 16), (16, 32), (32, 32), (16, 16), (16, 64), (8, 32)]
-        matrix_tiles = [(16, 16), (8, 16), (16, 8), (8, 32), (16, 32)]
         itiles = [(16, 16), (32, 32), (16, 64), (32, 64)]
     else:
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
-        matrix_tiles = [(8, 8, 8), (4, 8, 8), (8, 8, 16), (4, 8, 16)]
         itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
-    # matrix_splits: samples per tile-matrix item (1, 2, 4 or 8 batches of 256)
-    return dict(tiles=tiles, matrix_tiles=matrix_tiles, grid_chunks=[0, 1, 2, 4, 8, 16],
-                interp_chunks=[1, 2, 4, 8, 16], itiles=itiles, staged_chunks=[2, 4, 8, 16],
-                grid_splits=[512, 1024, 2048, 4096], matrix_splits=[256, 512, 1024, 2048], grid_modes=[3, 2, 1])
+    return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8, 16], interp_chunks=[1, 2, 4, 8, 16],
+                itiles=itiles, staged_chunks=[2, 4, 8, 16], grid_splits=[512, 1024, 2048, 4096],
+                matrix_splits=[64, 128, 256], grid_modes=[3, 2, 1])
 
 
 def _cache_path():
@@ -46,7 +43,7 @@ def _key(plans, ncoils):
     bucket = int(2 ** round(np.log2(max(m, 1))))
     # the library is built for gfx950 only, so the geometry alone identifies a
     # tuning problem (device-name queries are not stable under profilers)
-    return f"gfx950|v2|d{p.ndim}|N{p.shape}|G{p.os_shape}|W{p.width}|C{max(ncoils)}|M{bucket}|P{len(plans)}"
+    return f"gfx950|d{p.ndim}|N{p.shape}|G{p.os_shape}|W{p.width}|C{max(ncoils)}|M{bucket}|P{len(plans)}"
 
 
 def _time(fn, reps, device):
@@ -97,7 +94,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         # 1) gridding variant x tile, automatic chunks
         best_tile, best_t, best_gm = None, float("inf"), 0
         for gm in cand.get("grid_modes", [0]):
-            for tile in (cand.get("matrix_tiles", cand["tiles"]) if gm == 3 else cand["tiles"]):
+            # the matrix gridding does not use the gridding tile
+            for tile in (cand["tiles"][:1] if gm == 3 else cand["tiles"]):
                 try:
                     _apply(plans, tile, 0, 0, gmode=gm)
                     b = BatchedNUFFT(plans, ncoils)
@@ -123,8 +121,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
                 best_gc, best_gt = gc, t
         # 2b) hot-tile split threshold
         best_split = 0
-        # (for the tile matrix: the samples of a tile's bin one workgroup
-        # sums before the rest go to further parts)
+        # (for the matrix gridding: the row length above which a whole
+        # workgroup sums one cell)
         for sp in cand.get("matrix_splits" if best_gm == 3 else "grid_splits", []):
             _apply(plans, best_tile, 0, best_gc, split=sp, gmode=best_gm)
             b = BatchedNUFFT(plans, ncoils)

@@ -92,13 +92,11 @@ class ReconRequest:
 
 
 def _plan_model_bytes(npts, g, width=4):
-    """Device bytes of a 2D plan and its launch tables before it exists:
-    coordinates (8 B), interpolation entries (16 B) and gridding tile bins
-    (16 B, ~1.5 copies) per sample, the gridding matrix (2 B per tap and
-    sample), its row offsets (2 B per cell and batch), the tile offsets and
-    the launch tables."""
+    """Device bytes of a 2D plan before it exists: coordinates (8 B) and
+    interpolation entries (16 B) per sample, the gridding matrix (8 B per
+    tap and sample), its row offsets (4 B per cell) and the tile offsets."""
     taps = width * width
-    return int(npts * (8 + 16 + 24 + 2 * taps) + 4 * g * g + (1 << 20))
+    return int(npts * (8 + 16 + 8 * taps) + 4 * g * g + (1 << 20))
 
 
 def estimate_request_bytes(n, ncoils, npts, oversamp=1.25, plan_bytes=None):
@@ -115,8 +113,11 @@ def estimate_request_bytes(n, ncoils, npts, oversamp=1.25, plan_bytes=None):
 
 def request_bytes(req, plan):
     """``estimate_request_bytes`` with the plan's own device bytes (sorted
-    trajectory, bins, gridding matrix) plus its launch tables in a batch."""
-    tables = 8 * req.ncoils * req.npts + (1 << 20)
+    trajectory, bins, gridding matrix) plus its share of a batch's launch
+    tables: the coil-minor sample copy of the matrix gridding (records of up
+    to 16 coils, each request padded to the batch's widest record) and the
+    item lists."""
+    tables = 8 * 16 * -(-req.ncoils // 16) * req.npts + (1 << 20)
     return estimate_request_bytes(req.shape, req.ncoils, req.npts, plan.oversamp[-1],
                                   plan_bytes=plan.interpolator.device_bytes() + tables)
 

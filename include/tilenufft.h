@@ -71,14 +71,13 @@ typedef struct tn_plan_params {
                              tap and sample (entries spread over footprint
                              origins), 2 per-lane runs of origin-sorted
                              entries summed in registers, one z plane of the
-                             footprint per pass (W_y * W_x <= 36), 3 tile
-                             matrix: per tile, the samples of its bin in
-                             batches of up to 256, their records staged in
-                             LDS once, every cell summing its row of a
-                             plan-time matrix of 2-byte entries (batch slot |
-                             tap code); grid_chunk = coils per record (1, 2,
-                             4, 8, 16; 0: 16), grid_split = samples per
-                             workgroup before a bin is summed in parts    */
+                             footprint per pass (W_y * W_x <= 36), 3 per-cell
+                             sums over a sparse gridding matrix built with
+                             the plan (CSR by cell, 8 bytes per sample and
+                             tap) reading coil-minor sample records;
+                             grid_chunk = coils per record, <= 16 (0: 16),
+                             grid_split = row length above which one cell
+                             is summed by a workgroup of its own          */
     float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
     const float *apod[3]; /* host apodization vectors of length shape[d], or
                              NULL for ones (src/deepmr/fft/nufft.py:423-447)   */
@@ -90,10 +89,11 @@ typedef struct tn_plan_params {
  * index inside each tile) and uploads the apodization vectors.
  * Replaces plan_interpolator / _prepare_interpolator
  * (src/deepmr/fft/_interp/plan.py:14-143, :251-269). The per-axis tap
- * values are not stored (deepmr's tables hold 8 * W * d bytes per sample):
- * taps are evaluated in registers at apply time. Gridding mode 3 stores its
- * tile matrix instead: 2 bytes per (sample, tap), 4 bytes per (tile, sample)
- * bin slot and 2 bytes per (batch, tile cell) row start. */
+ * values deepmr tabulates (8 * W * d bytes per sample) are not stored:
+ * interpolation evaluates the taps in registers at apply time. Gridding
+ * mode 3 stores its matrix instead: 8 bytes per (sample, tap), i.e.
+ * 8 * W^d bytes per sample, plus 4 bytes per grid cell and frame of row
+ * offsets. */
 int tn_plan_create(const tn_plan_params *params, const float *coord_dev,
                    tn_stream stream, tn_plan **out);
 int tn_plan_destroy(tn_plan *plan);

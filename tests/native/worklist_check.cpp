@@ -1,7 +1,10 @@
-// Host-only check of the tile-matrix gridding work list (tn_worklist.h):
-// every (tile, coil record) gets items covering its batches exactly once, in
-// part order, with fold slots whose scratch regions hold every part's
-// partial tile and do not overlap. Prints "ok <cases>" or the first failure.
+// Host-only check of the matrix-gridding work list (tn_worklist.h): for
+// random CSR row offsets, the runs cover every cell exactly once (long rows
+// first), short runs stay within the kernel's cell and LDS-stage limits and
+// carry the entry range their cells' rows span, and the items cover every
+// (run, coil record) with long rows split into in-order parts whose partial
+// records are disjoint and inside the scratch size. Prints "ok <cases>" or
+// the first failure.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -9,76 +12,107 @@
 
 #include "tn_worklist.h"
 
-static int fail(const char *what, int cs, long t) {
-    std::printf("FAIL case %d tile %ld: %s\n", cs, t, what);
+static int fail(const char *what, int cs, long c) {
+    std::printf("FAIL case %d cell %ld: %s\n", cs, c, what);
     return 1;
 }
 
 int main() {
     std::mt19937 rng(7);
+    const int cpis[] = {16, 32, 64, 128, 256};
     int cases = 0;
     for (int cs = 0; cs < 400; ++cs) {
-        const int64_t ntiles = 1 + rng() % 300;
+        const int64_t ncell = cs % 50 == 0 ? 0 : 1 + rng() % 5000;
+        const int cpi = cpis[rng() % 5];
+        const int mlong = 16 + rng() % (tn::kMatrixStage - 15);
         const int nbatch = 1 + rng() % 40;
-        const int rc = 1 << (rng() % 5);
-        const int kb = cs % 7 == 0 ? 1 << 30 : 1 + rng() % 6;
-        const int64_t cells = 16 + rng() % 1009;
-        std::vector<int32_t> boff(ntiles + 1, 0);
-        for (int64_t t = 0; t < ntiles; ++t) {
-            const int r = rng() % 10;
-            const int nb = r < 3 ? 0 : r < 8 ? 1 + rng() % 3 : rng() % 60;
-            boff[t + 1] = boff[t] + nb;
+        const int mc = 1 + rng() % tn::kMatrixCoils;
+        // row lengths: mostly empty or short, a few long (a dense centre)
+        std::vector<int32_t> row(ncell + 1, 0);
+        for (int64_t c = 0; c < ncell; ++c) {
+            const int r = rng() % 100;
+            const int len = r < 30 ? 0 : r < 95 ? rng() % 40 : r < 98 ? rng() % (2 * mlong) : rng() % 9000;
+            row[c + 1] = row[c] + len;
         }
-        std::vector<tn::Item> grid, fold;
-        std::vector<int64_t> slot_base;
-        int64_t scratch = 0;
-        const int job = cs % 5;
-        tn::tile_matrix_items(job, boff.data(), ntiles, nbatch, rc, kb, cells, grid, fold, slot_base, scratch);
-        // per (tile, chunk): the batches each item covers
-        const int nchunk = (nbatch + rc - 1) / rc;
-        std::vector<std::vector<const tn::Item *>> by(ntiles * nchunk);
-        for (const tn::Item &it : grid) {
-            if (it.job != job || it.unit < 0 || it.unit >= ntiles) return fail("item job / tile", cs, it.unit);
-            if (it.c0 < 0 || it.c0 % rc || it.c1 <= it.c0 || it.c1 > nbatch || it.c1 - it.c0 > rc)
-                return fail("coil range", cs, it.unit);
-            const int nb = boff[it.unit + 1] - boff[it.unit];
-            if (it.e0 < 0 || it.e1 < it.e0 || it.e1 > nb || it.e1 - it.e0 > kb) return fail("batch range", cs, it.unit);
-            by[(int64_t)it.unit * nchunk + it.c0 / rc].push_back(&it);
-        }
-        std::vector<int> parts_of_slot(slot_base.size(), 0);
-        for (int64_t t = 0; t < ntiles; ++t) {
-            const int nb = boff[t + 1] - boff[t];
-            for (int c = 0; c < nchunk; ++c) {
-                const auto &v = by[t * nchunk + c];
-                if (v.empty()) return fail("tile without an item", cs, t);
-                if (v.size() == 1 && v[0]->slot < 0) {
-                    if (v[0]->e0 != 0 || v[0]->e1 != nb || v[0]->part != 0) return fail("whole item", cs, t);
-                    continue;
-                }
-                const int slot = v[0]->slot;
-                if (slot < 0 || slot >= (int)slot_base.size()) return fail("fold slot", cs, t);
-                int next = 0;
-                for (size_t q = 0; q < v.size(); ++q) {
-                    if (v[q]->slot != slot || v[q]->part != (int)q) return fail("part order", cs, t);
-                    if (v[q]->e0 != next) return fail("batches not contiguous", cs, t);
-                    next = v[q]->e1;
-                }
-                if (next != nb) return fail("batches not covered", cs, t);
-                if ((int)v.size() < 2) return fail("split into one part", cs, t);
-                parts_of_slot[slot] = (int)v.size();
+        std::vector<tn::MatrixRun> runs;
+        tn::matrix_runs_of(row.data(), ncell, cpi, mlong, runs);
+        std::vector<int> seen(ncell, 0);
+        bool in_shorts = false;
+        for (const tn::MatrixRun &r : runs) {
+            if (r.ncells == 0) return fail("empty run", cs, r.cell);
+            if (r.ncells < 0) {
+                if (in_shorts) return fail("long row after a short run", cs, r.cell);
+                if (r.cell < 0 || r.cell >= ncell) return fail("long row cell", cs, r.cell);
+                const int len = row[r.cell + 1] - row[r.cell];
+                if (len <= mlong) return fail("long row not above the threshold", cs, r.cell);
+                if (r.e0 != row[r.cell] || r.ne != len) return fail("long row entries", cs, r.cell);
+                if (-r.ncells != (len + tn::kMatrixPart - 1) / tn::kMatrixPart) return fail("part count", cs, r.cell);
+                ++seen[r.cell];
+                continue;
+            }
+            in_shorts = true;
+            if (r.cell < 0 || r.cell + r.ncells > ncell) return fail("run past the grid", cs, r.cell);
+            if (r.ncells > cpi) return fail("run longer than cells per item", cs, r.cell);
+            if (r.ne > tn::kMatrixStage || r.ne < 0) return fail("run entries exceed the LDS stage", cs, r.cell);
+            if (r.e0 != row[r.cell] || r.ne != row[r.cell + r.ncells] - row[r.cell]) return fail("run entries", cs, r.cell);
+            for (int i = 0; i < r.ncells; ++i) {
+                if (row[r.cell + i + 1] - row[r.cell + i] > mlong) return fail("long row inside a run", cs, r.cell + i);
+                ++seen[r.cell + i];
             }
         }
-        // fold items: one per slot, with its part count; scratch regions
-        if (fold.size() != slot_base.size()) return fail("fold items != slots", cs, -1);
-        int64_t end = 0;
-        for (const tn::Item &f : fold) {
-            if (f.e0 < 0 || f.e0 >= (int)slot_base.size() || f.e1 != parts_of_slot[f.e0])
-                return fail("fold item", cs, f.unit);
-            const int64_t need = (int64_t)f.e1 * 2 * (f.c1 - f.c0) * cells;
-            if (slot_base[f.e0] != end) return fail("scratch regions not packed", cs, f.unit);
-            end += need;
+        for (int64_t c = 0; c < ncell; ++c)
+            if (seen[c] != 1) return fail("cell not covered exactly once", cs, c);
+
+        std::vector<tn::Item> grid, fold;
+        int64_t scratch = 0;
+        const int job = cs % 5;
+        tn::matrix_items(job, runs, nbatch, mc, grid, fold, scratch);
+        // per (run, chunk): the items, in order
+        const int nchunk = (nbatch + mc - 1) / mc;
+        size_t gi = 0, fi = 0;
+        std::vector<char> used(scratch / (2 * tn::kMatrixCoils), 0);
+        if (scratch % (2 * tn::kMatrixCoils)) return fail("scratch not whole records", cs, -1);
+        for (const tn::MatrixRun &r : runs) {
+            for (int q = 0; q < nchunk; ++q) {
+                const int c0 = q * mc, c1 = std::min(nbatch, c0 + mc);
+                auto same = [&](const tn::Item &it) {
+                    return it.job == job && it.unit == r.cell && it.c0 == c0 && it.c1 == c1 && it.e0 == q;
+                };
+                if (r.ncells > 0) {
+                    if (gi >= grid.size() || !same(grid[gi])) return fail("run item", cs, r.cell);
+                    const tn::Item &it = grid[gi++];
+                    if (it.part != 0 || (it.e1 & 0xffff) != r.ncells || (it.e1 >> 16) != r.ne || it.slot != r.e0)
+                        return fail("run item fields", cs, r.cell);
+                    continue;
+                }
+                const int nparts = -r.ncells;
+                int rec = -1;
+                for (int k = 0; k < nparts; ++k) {
+                    if (gi >= grid.size() || !same(grid[gi])) return fail("part item", cs, r.cell);
+                    const tn::Item &it = grid[gi++];
+                    if (it.part != 1 || it.e1 != k) return fail("part order", cs, r.cell);
+                    // the kernel sums entries [row + k * part, min(row end, + part))
+                    if (row[r.cell] + (int64_t)k * tn::kMatrixPart >= row[r.cell + 1]) return fail("empty part", cs, r.cell);
+                    if (k == 0) rec = it.slot;
+                    if (it.slot != rec) return fail("parts of one row in different records", cs, r.cell);
+                }
+                if (nparts == 1) {
+                    if (rec != -1) return fail("single part with a record", cs, r.cell);
+                    continue;
+                }
+                if (rec < 0 || (int64_t)(rec + nparts) * 2 * tn::kMatrixCoils > scratch)
+                    return fail("partial records past the scratch", cs, r.cell);
+                for (int k = 0; k < nparts; ++k) {
+                    if (used[rec + k]) return fail("partial records overlap", cs, r.cell);
+                    used[rec + k] = 1;
+                }
+                if (fi >= fold.size() || !same(fold[fi])) return fail("fold item", cs, r.cell);
+                const tn::Item &f = fold[fi++];
+                if (f.e1 != nparts || f.slot != rec || f.part != 0) return fail("fold item fields", cs, r.cell);
+            }
         }
-        if (end != scratch) return fail("scratch size", cs, -1);
+        if (gi != grid.size()) return fail("extra grid items", cs, -1);
+        if (fi != fold.size()) return fail("extra fold items", cs, -1);
         ++cases;
     }
     std::printf("ok %d\n", cases);

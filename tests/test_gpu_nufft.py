@@ -277,10 +277,9 @@ GRID_CASES = [
 @pytest.mark.parametrize("ndim, n, width, osf, ncoil, split", GRID_CASES)
 def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf, ncoil, split):
     """The gridding variants (per-sample LDS atomics over origin-spread
-    entries; register runs over origin-sorted entries; the tile matrix, cells
-    summing their rows over LDS-staged sample records) on dense-centre
-    trajectories, bins summed in parts, coil counts below and above the wave
-    count and the record widths."""
+    entries; register runs over origin-sorted entries; per-cell sums over the
+    sparse gridding matrix) on dense-centre trajectories, split tiles, coil
+    counts below and above the wave count and the coils per lane."""
     if mode == 2 and width * width > 36:
         pytest.skip("register-run gridding holds at most 36 taps per z plane")
     print(f"grid mode={mode} ndim={ndim} N={n} W={width} osf={osf} coils={ncoil} split={split}", flush=True)
@@ -303,10 +302,10 @@ def test_grid_modes_match_oracle_and_are_deterministic(mode, ndim, n, width, osf
 
 @pytest.mark.parametrize("chunk", [1, 3, 8, 16])
 def test_grid_matrix_coil_chunks(chunk):
-    """Tile-matrix gridding with 1..16 coils per item over 21 coils (records
-    of 1, 4, 8 and 16 coils, two coils per lane from 2; the last item's
-    record is padded with zero coils that are not stored) and a density
-    weight, against the oracle."""
+    """Matrix gridding with 1..16 coils per item over 21 coils (records of 4,
+    8 and 16 coils summed two coils per lane in 2D; partial last items hold
+    zero coils that are not stored) and a density weight, against the
+    oracle."""
     n = 32
     print(f"grid matrix chunk={chunk}", flush=True)
     coord = testing.vd_spiral(8, 200, n)
@@ -319,9 +318,9 @@ def test_grid_matrix_coil_chunks(chunk):
 
 
 def test_grid_matrix_multiframe_plan_and_empty_frame_rows():
-    """One plan of several frames (per-frame tiles and batches in one
-    matrix), with repeated samples on the grid edge (taps that wrap), against
-    the oracle."""
+    """One plan of several frames (per-frame row offsets into one matrix),
+    with repeated samples on the grid edge (taps that wrap), against the
+    oracle."""
     n = 24
     print("grid matrix multi-frame", flush=True)
     rng = np.random.default_rng(55)
@@ -336,10 +335,10 @@ def test_grid_matrix_multiframe_plan_and_empty_frame_rows():
 
 @pytest.mark.parametrize("ncoil, chunk", [(16, 0), (8, 0), (3, 0), (2, 1)])
 def test_grid_matrix_long_rows_in_parts(ncoil, chunk):
-    """A tile whose bin holds 3000 samples on one spot next to a uniform
-    background: 12 batches, summed in parts of four whose partial tiles are
-    folded in order; records of 16, 8, 4 (3 coils) and 1 coil, against the
-    oracle, twice (bitwise)."""
+    """Cells with more entries than one workgroup part (3000 samples on one
+    spot next to a uniform background): the rows are summed in parts whose
+    partials are folded in order; coil-minor (chunks of 16, 8 and 3 coils)
+    and lane-per-cell (1 coil per item) kernels, against the oracle, twice."""
     n = 16
     print(f"grid matrix long rows coils={ncoil} chunk={chunk}", flush=True)
     rng = np.random.default_rng(57)
@@ -357,12 +356,36 @@ def test_grid_matrix_long_rows_in_parts(ncoil, chunk):
     assert rel(a.cpu().numpy(), ref) < RTOL
 
 
+@pytest.mark.parametrize("split", [0, 64])
+def test_grid_matrix_sums_do_not_depend_on_record_width(split):
+    """A coil's gridded values are bitwise the same whether it is gridded in
+    a 16-coil record, a 4-coil record (3 coils) or in place (one coil per
+    item): short rows are summed in entry order and long rows (a 3000-sample
+    hot spot, rows above 64 entries with split 64) by a fixed partition
+    whose group sums are added in order, for every record width."""
+    n = 16
+    print(f"grid matrix record-width independence split={split}", flush=True)
+    rng = np.random.default_rng(59)
+    bg = rng.uniform(-n / 2, n / 2, (1, 400, 2))
+    hot = np.tile(np.array([[[0.3, -0.2]]]), (1, 3000, 1)) + rng.normal(0, 0.05, (1, 3000, 2))
+    coord = np.concatenate([bg, hot], 1).astype(np.float32)
+    kt = torch.from_numpy(cnormal((16,) + coord.shape[:-1], 60)).to(DEV)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV, grid_mode=3, grid_split=split)
+    plan1 = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV, grid_mode=3, grid_split=split, grid_chunk=1)
+    a16 = tilenufft.fft.apply_nufft_adj(kt, plan)[:3]
+    a3 = tilenufft.fft.apply_nufft_adj(kt[:3].contiguous(), plan)
+    a1 = tilenufft.fft.apply_nufft_adj(kt[:3].contiguous(), plan1)
+    torch.cuda.synchronize()
+    assert torch.equal(a16.view(torch.float32), a3.view(torch.float32))
+    assert torch.equal(a16.view(torch.float32), a1.view(torch.float32))
+
+
 @pytest.mark.parametrize("chunk, split", [(6, 32), (16, 32), (16, 0)])
 def test_grid_matrix_3d_records_and_parts(chunk, split):
-    """3D tile-matrix gridding over 11 coils: records of 8 coils (6 asked,
-    and 16 asked, narrowed to 8 by the 8x8x8 tile's register budget), with
-    the dense kooshball centre's bins summed one batch per part (split 32),
-    against the oracle, twice (bitwise)."""
+    """3D matrix gridding over 11 coils: 8-coil records (one coil per lane in
+    3D) and 16-coil records (two coils per lane), with the dense kooshball
+    centre summed in parts when rows exceed 32 entries, against the oracle,
+    twice (bitwise)."""
     n = 16
     print(f"grid matrix 3D chunk={chunk} split={split}", flush=True)
     coord = testing.kooshball(300, 32, n)
@@ -382,7 +405,29 @@ def test_grid_runs_refuses_wide_kernels():
     coord = testing.golden_angle_radial(12, 64, 24)
     with pytest.raises(RuntimeError, match="taps"):
         tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV, grid_mode=2)
-    plan = tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV)  # automatic: the tile matrix (49 taps)
+    plan = tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV)  # automatic: the gridding matrix (49 taps)
     k = torch.from_numpy(cnormal((1,) + coord.shape[:-1], 52)).to(DEV)
     ref = orc.apply_nufft_adj(k.cpu().numpy(), orc.plan_nufft(coord, 24, 7, 2.0))
     assert rel(tilenufft.fft.apply_nufft_adj(k, plan).cpu().numpy(), ref) < RTOL
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_empty_trajectory_plan(mode):
+    """A plan with no samples (every gridding variant): the adjoint writes an
+    all-zero grid and image, the forward an empty k-space, through the plan
+    and through a batched launch."""
+    print(f"empty trajectory grid_mode={mode}", flush=True)
+    coord = np.zeros((0, 8, 2), np.float32)  # (views, samples, 2) with no views
+    plan = tilenufft.fft.plan_nufft(coord, 24, 4, 2.0, device=DEV, grid_mode=mode)
+    k = torch.zeros((3, 0, 8), dtype=torch.complex64, device=DEV)
+    img = tilenufft.fft.apply_nufft_adj(k, plan)
+    assert img.shape == (3, 24, 24) and not torch.isnan(img.real).any() and float(img.abs().max()) == 0.0
+    y = tilenufft.fft.apply_nufft(torch.ones((3, 24, 24), dtype=torch.complex64, device=DEV), plan)
+    assert y.shape == (3, 0, 8)
+    from tilenufft.batch import BatchedNUFFT
+
+    b = BatchedNUFFT([plan, plan], 2)
+    b.grid.fill_(float("nan"))
+    b.gridding(b.grid)
+    torch.cuda.synchronize()
+    assert float(b.grid.abs().max()) == 0.0
