@@ -610,6 +610,27 @@ int tn_grid(tn_plan *plan, const void *samp, void *grid, int64_t nbatch, const f
     return launch_grid(*t, (const float2 *)samp, (float2 *)grid, weight, (hipStream_t)stream);
 }
 
+int tn_grid_static(tn_plan *plan, const void *samp, void *grid, int64_t nbatch, const float *weight,
+                   tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nbatch < 1 || nbatch >= INT32_MAX) return fail(TN_E_ARG, "nbatch must be >= 1");
+    if (!grid || (!samp && plan->p.npts > 0)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    auto it = plan->cache.find(nbatch);
+    if (it == plan->cache.end())
+        return fail(TN_E_ARG, "no launch tables for %lld coils: call tn_plan_prepare first", (long long)nbatch);
+    const LaunchTables &t = it->second;
+    if (t.scratch_floats > kStaticScratchFloats)
+        return fail(TN_E_RANGE, "gridding needs %lld workspace floats, the fixed workspace holds %lld",
+                    (long long)t.scratch_floats, (long long)kStaticScratchFloats);
+    float *ws = nullptr;
+    rc = static_scratch(plan->device, &ws);
+    if (rc) return rc;
+    return launch_grid(t, (const float2 *)samp, (float2 *)grid, weight, (hipStream_t)stream, LowRank{}, ws);
+}
+
 int tn_interp_lowrank(tn_plan *plan, const void *grid, void *samp, int64_t nbatch, int32_t ncoeff,
                       int32_t frame_npts, const void *basis_adjoint, const float *weight,
                       tn_stream stream) {

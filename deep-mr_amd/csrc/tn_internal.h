@@ -45,6 +45,8 @@ constexpr int kMatrixChunk = 16;     // default coils per item (record width of 
 constexpr uint64_t kMatrixAutoBytes = 4ull << 30;  // largest gridding matrix the automatic mode builds
 constexpr int kMatrixPasses = 4;     // cell passes per item of the coil-minor matrix gridding
 constexpr int kLongGroups = 32;      // lane groups summing one long row of the coil-minor matrix gridding
+constexpr int64_t kStaticScratchFloats = 8ll << 20;  // fixed gridding workspace: 32 MiB per device
+constexpr int kMaxDevices = 64;
 constexpr int kXcds = 8;          // accelerator complex dies (each with its own L2)
 constexpr int kRunsMaxTaps = 36;  // register budget of the run sums of one z plane (W <= 6)
 constexpr int kRunsBatch = 256;                 // entries per staged batch (= kBlock)
@@ -200,6 +202,9 @@ int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp,
 int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid,
                 const float *weight, hipStream_t stream, LowRank lr = LowRank{},
                 float *scratch = nullptr);
+// Device address of the fixed gridding workspace on `device` (looked up once
+// per device with hipGetSymbolAddress; no allocation).
+int static_scratch(int device, float **out);
 int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *samp,
                           const float *weight, hipStream_t stream, LowRank lr);
 int launch_fold(const LaunchTables &t, float2 *grid, hipStream_t stream);

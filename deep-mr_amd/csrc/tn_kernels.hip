@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "tn_internal.h"
 #include "tn_kb.h"
@@ -1516,6 +1517,30 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                            t.d_fold, scratch, t.d_slot_base, grid);
         TN_HIP(hipGetLastError());
     }
+    return TN_OK;
+}
+
+// Fixed device workspace of the allocation-free gridding entry point: the
+// partial records of long rows summed in parts (and the partial tiles of
+// split tiles). One copy per device, held until the process ends; host code
+// uses the per-device address from hipGetSymbolAddress, never the array's
+// name.
+static __device__ __align__(256) float scratch[kStaticScratchFloats];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int static_scratch(int device, float **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(TN_E_RANGE, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        TN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = p;
+    }
+    *out = (float *)g_scratch_ptr[device];
     return TN_OK;
 }
 

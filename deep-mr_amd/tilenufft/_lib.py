@@ -51,6 +51,7 @@ _SIGNATURES = {
     "tn_plan_prepare": (c_int32, [c_void_p, c_int64]),
     "tn_interp": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "tn_grid": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "tn_grid_static": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "tn_interp_lowrank": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p,
                                     c_void_p, c_void_p]),
     "tn_grid_lowrank": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p,

@@ -132,6 +132,20 @@ int tn_interp(tn_plan *plan, const void *grid, void *samp, int64_t nbatch,
 int tn_grid(tn_plan *plan, const void *samp, void *grid, int64_t nbatch,
             const float *weight, tn_stream stream);
 
+/* Gridding exactly as tn_grid, without allocating: no counterpart in
+ * deepmr; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops). The launch tables for `nbatch` must
+ * already exist (tn_plan_prepare, or an earlier tn_grid with this nbatch),
+ * else TN_E_ARG. Partial sums (long matrix rows summed in parts, split
+ * tiles) go to a fixed device workspace of 32 MiB per device instead of
+ * the tables' own; TN_E_RANGE when a plan needs more. Each partial is
+ * written before it is read, so nothing has to start at zero and the
+ * previous call's contents do no harm. Every call on one device shares
+ * that workspace, so calls that overlap on one device (concurrent streams
+ * or host threads) race; calls ordered on one stream do not. */
+int tn_grid_static(tn_plan *plan, const void *samp, void *grid, int64_t nbatch,
+                   const float *weight, tn_stream stream);
+
 /* Low-rank (subspace) interpolation. The plan must hold every frame's
  * samples as one trajectory (nframes = 1, npts = nframes_lr * frame_npts,
  * frame f = samples [f*frame_npts, (f+1)*frame_npts)).
