@@ -63,6 +63,100 @@ def variant_key(plan, kind):
     return kind
 
 
+# kernels of each timed stage (short names as fold_counters derives them)
+STAGE_KERNELS = {
+    "grid": ("samp_transpose_kernel", "grid_matrix_t_kernel", "grid_matrix_kernel", "matrix_fold_kernel",
+             "grid_runs_kernel", "grid_kernel", "fold_kernel"),
+    "interp": ("interp_staged_kernel", "interp_kernel"),
+    "fft_fwd": ("fft_cols_fwd_kernel", "fft_rows_kernel|false"),
+    "fft_adj": ("fft_rows_kernel|true", "fft_cols_inv_kernel"),
+}
+
+
+def short_kernel_name(name):
+    """`void ns::(anonymous namespace)::k<1, 2>(args)` -> `k<1, 2>`."""
+    n = name.strip().replace("(anonymous namespace)::", "")
+    if n.startswith("void "):
+        n = n[5:]
+    depth = 0
+    for i, ch in enumerate(n):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            n = n[:i]
+            break
+    base, _, targs = n.partition("<")
+    return base.split("::")[-1] + ("<" + targs if targs else "")
+
+
+def _stage_of(kname):
+    base = kname.split("<")[0]
+    for stage, names in STAGE_KERNELS.items():
+        for entry in names:
+            nm, _, flag = entry.partition("|")
+            if base == nm and (not flag or kname.rstrip(">").endswith(flag)):
+                return stage
+    return None
+
+
+def fold_counters(dirs, bench_line, copy_mib):
+    """Per-kernel means of rocprofv3 counter runs of ``bench.py --kernels-only
+    --calibrate-copy MIB`` (one directory per run), folded into the HBM bytes
+    per launch of each stage. FETCH_SIZE / WRITE_SIZE are KiB per dispatch;
+    each is scaled by what it reads for the plain copy of ``copy_mib`` MiB
+    (the three largest-writing elementwise dispatches), so the bytes are the
+    copy-calibrated estimate. ``bench_line`` is the JSON line of one of those
+    runs: it names the launch variants the traffic is keyed by."""
+    import csv
+    import glob
+
+    per = {}  # (kernel, counter) -> [values per dispatch]
+    for d in dirs:
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        for fn in files:
+            acc = {}
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    key = (row["Dispatch_Id"], short_kernel_name(row["Kernel_Name"]), row["Counter_Name"])
+                    acc[key] = acc.get(key, 0.0) + float(row["Counter_Value"])
+            for (_, k, c), v in acc.items():
+                per.setdefault((k, c), []).append(v)
+    copy_bytes = copy_mib * (1 << 20)
+    scale = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = sorted(v for (k, cc), vs in per.items() if cc == c and "elementwise" in k for v in vs)[-3:]
+        if cal:
+            scale[c] = copy_bytes / (float(np.mean(cal)) * 1024.0)
+    with open(bench_line) as f:
+        line = json.loads([ln for ln in f if ln.startswith("{")][-1])
+    variants = line["kernels"]["variants"]
+    stages = {}
+    for (k, c), vs in sorted(per.items()):
+        st = _stage_of(k)
+        if st is None:
+            continue
+        e = stages.setdefault(st, {"kernels": {}})
+        e["kernels"].setdefault(k, {})[c] = float(np.mean(vs))
+    out = {"workload": line["config"]["workload"], "copy_mib": copy_mib,
+           "fetch_scale": scale.get("FETCH_SIZE"), "write_scale": scale.get("WRITE_SIZE"),
+           "variants": {}, "stages": {}}
+    for st, e in stages.items():
+        # raw counters (upper case) add up over the stage's kernels; derived
+        # ratios (LdsBankConflict, MeanOccupancyPerActiveCU) stay per kernel
+        counters = sorted({c for kv in e["kernels"].values() for c in kv if c.isupper()})
+        tot = {c: sum(kv.get(c, 0.0) for kv in e["kernels"].values()) for c in counters}
+        rec = {"kernels": e["kernels"], "totals": tot}
+        if "FETCH_SIZE" in tot and "WRITE_SIZE" in tot and len(scale) == 2:
+            rec["fetch_bytes"] = tot["FETCH_SIZE"] * 1024.0 * scale["FETCH_SIZE"]
+            rec["write_bytes"] = tot["WRITE_SIZE"] * 1024.0 * scale["WRITE_SIZE"]
+            rec["bytes"] = rec["fetch_bytes"] + rec["write_bytes"]
+            out["variants"][variants[st]] = rec["bytes"]
+        out["stages"][variants[st]] = rec
+    return out
+
+
 def time_kernel(fn, reps, device):
     """Average duration of fn() from HIP events recorded on the launch stream.
     Python's cyclic garbage collector is run before and held off during the
@@ -241,7 +335,7 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
 def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
     """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
     256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3): the
-    automatic plan's gridding (tile matrix) and staged interpolation, and with
+    automatic plan's gridding (the gridding matrix) and staged interpolation, and with
     ``compare_runs`` the register-run gridding too. Durations from HIP events
     on the launch stream; algorithmic bytes as for the 2D workload. Returns a
     dict."""
@@ -359,6 +453,11 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after measuring, run 3 plain device copies of this many MiB (a known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--fold-counters", nargs="+", default=None, metavar="DIR",
+                    help="instead (no GPU): fold rocprofv3 --pmc output directories of --kernels-only runs into "
+                         "per-stage bytes per launch; needs --bench-line and --calibrate-copy; writes --traffic-out")
+    ap.add_argument("--bench-line", default=None, help="JSON line printed by one of the folded runs")
+    ap.add_argument("--traffic-out", default=None, help="where --fold-counters writes its JSON (default: stdout)")
     ap.add_argument("--kooshball", action="store_true",
                     help="instead: time 3D kooshball gridding / interpolation (128^3, 8 coils, 20000 x 256, "
                          "W 4, osf 2.0 and 1.5) with register runs and the gridding matrix; one JSON line")
@@ -366,6 +465,16 @@ def main():
                     help="instead: time 10 unrolled CG-SENSE iterations forward + backward (256^2, 8 coils, "
                          "201 radial spokes x 512, osf 1.25); one JSON line")
     args = ap.parse_args()
+    if args.fold_counters:
+        if not args.bench_line or args.calibrate_copy <= 0:
+            raise SystemExit("--fold-counters needs --bench-line and --calibrate-copy")
+        res = fold_counters(args.fold_counters, args.bench_line, args.calibrate_copy)
+        text = json.dumps(res, indent=1, sort_keys=True)
+        if args.traffic_out:
+            with open(args.traffic_out, "w") as f:
+                f.write(text + "\n")
+        print(text)
+        return
     if args.kooshball:
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
@@ -537,6 +646,7 @@ def main():
             "image_copy_ms": t_copy,
             **{f"{k}_ms": v for k, v in t_fft.items()},
             **({"fft_alg_bytes": b_fft} if t_fft else {}),
+            "variants": {k: variant_key(plans[0], k) for k in ("grid", "interp", *t_fft)},
             **{f"{k}_frac": b_fft / (v * 1e-3) / 1e9 / HBM_PEAK_GBS for k, v in t_fft.items()},
         },
         "cpu_baseline": cpu,

@@ -39,6 +39,13 @@ namespace {
 // x 64 / C rows) on 16 distinct bank pairs, four lanes each.
 constexpr int fft_pad(int C) { return 64 / C; }
 
+// The Stockham passes keep a line in LDS with one float2 of padding after
+// every 8: the scattered writes of a radix-8 pass (lane j at 8 j + r, or
+// in runs of 8 at stride 64) then fall on distinct banks instead of 8 or 16
+// lanes per bank. A line of L elements takes fft_line(L) float2.
+constexpr int fft_line(int L) { return L + L / 8; }
+__device__ __forceinline__ int lds_at(int a) { return a + (a >> 3); }
+
 // Radix of pass p of a length-L transform: radix 8 while possible, then 4,
 // 2, 3 and 5; 0 past the last pass.
 constexpr int fft_radix_at(int L, int p) {
@@ -138,7 +145,8 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 // W_{NS R}^{r (j mod NS)}, runs the R-point DFT and stores element r at
 // (j / NS) NS R + (j mod NS) + r NS. Pass 0 reads line l through
 // src(l, i), the last pass writes through dst(l, i, v), the others use
-// buf[l] (the wave's LDS lines; src / dst may address them too).
+// buf[l] (the wave's LDS lines of fft_line(L) float2, addressed through
+// lds_at; src / dst may address them too, unpadded).
 template <int L, int NL, int P, int NS, bool INV, class Src, class Dst>
 __device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ tw, float2 *const *buf, Src &src,
                                            Dst &dst) {
@@ -168,7 +176,7 @@ __device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ 
                     if constexpr (P == 0)
                         v[l][q][r] = src(l, j + r * M);
                     else
-                        v[l][q][r] = buf[l][j + r * M];
+                        v[l][q][r] = buf[l][lds_at(j + r * M)];
                 }
             }
 #pragma unroll
@@ -194,7 +202,7 @@ __device__ __forceinline__ void fft_passes(int lane, const float2 *__restrict__ 
                     if constexpr (P == NP - 1)
                         dst(l, d + r * NS, v[l][q][r]);
                     else
-                        buf[l][d + r * NS] = v[l][q][r];
+                        buf[l][lds_at(d + r * NS)] = v[l][q][r];
                 }
             }
         }
@@ -226,7 +234,7 @@ template <int L, bool INV>
 __global__ __launch_bounds__(kBlock) void fft_rows_kernel(float2 *__restrict__ data, int64_t nlines, int Nx,
                                                           const float2 *__restrict__ tw) {
     constexpr int RL = rows_per_wave(L);
-    __shared__ float2 lds[kBlock / kWave * RL][L];
+    __shared__ float2 lds[kBlock / kWave * RL][fft_line(L)];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t line0 = ((int64_t)blockIdx.x * (kBlock / kWave) + wave) * RL;
     if (line0 >= nlines) return;  // no barriers below
@@ -278,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__re
                                                               int Ny, int Gx, const float *__restrict__ apx,
                                                               const float *__restrict__ apy,
                                                               const float2 *__restrict__ tw) {
-    constexpr int LP = L + fft_pad(C);
+    constexpr int LP = fft_line(L) + fft_pad(C);
     extern __shared__ float2 lds[];  // C * LP
     int64_t b;
     int mx0;
@@ -343,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
                                                               int Ny, int Gx, const float *__restrict__ apx,
                                                               const float *__restrict__ apy,
                                                               const float2 *__restrict__ tw, float scale) {
-    constexpr int LP = L + fft_pad(C);
+    constexpr int LP = fft_line(L) + fft_pad(C);
     extern __shared__ float2 lds[];  // C * LP
     int64_t b;
     int mx0;
@@ -411,7 +419,9 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
 // 66 KiB of LDS, two workgroups per CU) measured slower than 8 columns at
 // four workgroups per CU.
 constexpr int col_group(int L) { return L <= 256 ? 16 : L <= 512 ? 8 : 4; }
-constexpr size_t col_lds_bytes(int L) { return sizeof(float2) * (size_t)col_group(L) * (L + fft_pad(col_group(L))); }
+constexpr size_t col_lds_bytes(int L) {
+    return sizeof(float2) * (size_t)col_group(L) * (fft_line(L) + fft_pad(col_group(L)));
+}
 
 using RowFn = void (*)(float2 *, int64_t, int, const float2 *);
 using ColFwdFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, const float *, const float *,
