@@ -292,14 +292,14 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
         return serve.make_stream(n, rate_hz, seed=s, device="cpu")
 
     # warm-up (kernel loading, rocFFT plans of the common sizes), untimed
-    warm_srv = serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32)
+    warm_srv = serve.ReconServer(dev, hbm_budget=64 << 30)
     warm_srv.run(stream(warm, seed + 1000), realtime=False)
     warm_srv.close()
     serve.run_plain(stream(warm, seed + 1000), dev, realtime=False)
     torch.cuda.synchronize(dev)
 
     reqs_s = stream(nreq, seed)
-    srv = serve.ReconServer(dev, hbm_budget=64 << 30, max_active=32)
+    srv = serve.ReconServer(dev, hbm_budget=64 << 30)
     srv.run(reqs_s, realtime=True)
     srv.close()
     reqs_p = stream(nreq, seed)
@@ -317,7 +317,8 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
         stream=f"{nreq} requests, Poisson {rate_hz:g} Hz (seed {seed}), N in {{128,192,256,320}}, "
                "4-32 coils, radial/spiral, 10k-200k samples/coil, 10 CG-SENSE iterations, osf 1.25, W 4",
         served=dict(p50_s=st_s["p50"], p99_s=st_s["p99"], recon_per_s=st_s["recon_per_s"],
-                    completed=st_s["completed"], server_steps=srv.steps, stats=srv.stats),
+                    completed=st_s["completed"], server_steps=srv.steps, max_active=srv.max_active,
+                    admission=srv.admission, stats=srv.stats),
         plain=dict(p50_s=st_p["p50"], p99_s=st_p["p99"], recon_per_s=st_p["recon_per_s"],
                    completed=st_p["completed"],
                    path="one request at a time, apply_nufft / apply_nufft_adj with the request's plan "

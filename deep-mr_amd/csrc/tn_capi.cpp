@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -91,6 +92,23 @@ void free_tables(LaunchTables &t) {
     t = LaunchTables{};
 }
 
+// Launch tables go to the device on a library-owned non-blocking stream of
+// the current device (default priority), waited for before the tables are
+// used: a hipMemcpy would run on the null stream and first wait for all the
+// work the caller has queued there.
+static std::mutex g_upload_mu;
+static hipStream_t g_upload[kMaxDevices] = {};
+
+static int upload_stream(hipStream_t *out) {
+    int dev = 0;
+    TN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TN_E_RANGE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_upload_mu);
+    if (!g_upload[dev]) TN_HIP(hipStreamCreateWithFlags(&g_upload[dev], hipStreamNonBlocking));
+    *out = g_upload[dev];
+    return TN_OK;
+}
+
 static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &interp,
                          const std::vector<Item> &grid, const std::vector<Item> &fold,
                          const std::vector<int64_t> &slot_base, LaunchTables &t) {
@@ -105,18 +123,22 @@ static int upload_tables(const std::vector<Job> &jobs, const std::vector<Item> &
     TN_HIP(hipMalloc(&t.d_fold, sizeof(Item) * std::max<size_t>(fold.size(), 1)));
     TN_HIP(hipMalloc(&t.d_slot_base, sizeof(int64_t) * std::max<size_t>(slot_base.size(), 1)));
     TN_HIP(hipMalloc(&t.d_scratch, sizeof(float) * std::max<int64_t>(t.scratch_floats, 1)));
+    hipStream_t us = nullptr;
+    int rc = upload_stream(&us);
+    if (rc) return rc;
     if (!fold.empty())
-        TN_HIP(hipMemcpy(t.d_fold, fold.data(), sizeof(Item) * fold.size(), hipMemcpyHostToDevice));
+        TN_HIP(hipMemcpyAsync(t.d_fold, fold.data(), sizeof(Item) * fold.size(), hipMemcpyHostToDevice, us));
     if (!slot_base.empty())
-        TN_HIP(hipMemcpy(t.d_slot_base, slot_base.data(), sizeof(int64_t) * slot_base.size(),
-                         hipMemcpyHostToDevice));
+        TN_HIP(hipMemcpyAsync(t.d_slot_base, slot_base.data(), sizeof(int64_t) * slot_base.size(),
+                              hipMemcpyHostToDevice, us));
     if (!jobs.empty())
-        TN_HIP(hipMemcpy(t.d_jobs, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice));
+        TN_HIP(hipMemcpyAsync(t.d_jobs, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice, us));
     if (!interp.empty())
-        TN_HIP(hipMemcpy(t.d_interp, interp.data(), sizeof(Item) * interp.size(),
-                         hipMemcpyHostToDevice));
+        TN_HIP(hipMemcpyAsync(t.d_interp, interp.data(), sizeof(Item) * interp.size(), hipMemcpyHostToDevice,
+                              us));
     if (!grid.empty())
-        TN_HIP(hipMemcpy(t.d_grid, grid.data(), sizeof(Item) * grid.size(), hipMemcpyHostToDevice));
+        TN_HIP(hipMemcpyAsync(t.d_grid, grid.data(), sizeof(Item) * grid.size(), hipMemcpyHostToDevice, us));
+    TN_HIP(hipStreamSynchronize(us));
     t.device_bytes = (int64_t)(sizeof(Job) * std::max<size_t>(jobs.size(), 1) +
                                sizeof(Item) * (std::max<size_t>(interp.size(), 1) + std::max<size_t>(grid.size(), 1) +
                                                std::max<size_t>(fold.size(), 1)) +
@@ -301,7 +323,13 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         t.st_elems = off;
         t.n_tr = (int64_t)tr.size();
         TN_HIP(hipMalloc(&t.d_tr, sizeof(Item) * std::max<size_t>(tr.size(), 1)));
-        if (!tr.empty()) TN_HIP(hipMemcpy(t.d_tr, tr.data(), sizeof(Item) * tr.size(), hipMemcpyHostToDevice));
+        if (!tr.empty()) {
+            hipStream_t us = nullptr;
+            int rc = upload_stream(&us);
+            if (rc) return rc;
+            TN_HIP(hipMemcpyAsync(t.d_tr, tr.data(), sizeof(Item) * tr.size(), hipMemcpyHostToDevice, us));
+            TN_HIP(hipStreamSynchronize(us));
+        }
         TN_HIP(hipMalloc(&t.d_st, sizeof(float2) * std::max<int64_t>(off, 1)));
     }
     const int64_t copy_bytes = t.d_st ? (int64_t)(sizeof(Item) * std::max<size_t>(tr.size(), 1) +
@@ -513,13 +541,14 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
         const size_t n = (size_t)params->shape[d];
         if (hipMalloc(&pl->d_apod[d], sizeof(float) * n) != hipSuccess)
             return cleanup(fail(TN_E_NOMEM, "hipMalloc of apodization failed"));
-        if (hipMemcpy(pl->d_apod[d], params->apod[d], sizeof(float) * n, hipMemcpyHostToDevice) !=
-            hipSuccess)
+        if (hipMemcpyAsync(pl->d_apod[d], params->apod[d], sizeof(float) * n, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
             return cleanup(fail(TN_E_HIP, "apodization upload failed"));
     }
     rc = build_bins(pl, s);
     if (rc) return cleanup(rc);
-    rc = build_fft(pl);
+    rc = build_fft(pl, s);
     if (rc) return cleanup(rc);
     *out = pl;
     return TN_OK;

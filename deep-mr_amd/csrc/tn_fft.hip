@@ -491,7 +491,7 @@ bool fft_supported(const tn_plan *plan) {
     return fft_fns(p.grid[2]).L != 0 && fft_fns(p.grid[1]).L != 0;
 }
 
-int build_fft(tn_plan *plan) {
+int build_fft(tn_plan *plan, hipStream_t stream) {
     if (!fft_supported(plan)) return TN_OK;
     {   // dynamic LDS of the column tiles (up to 64 KiB with these column counts)
         const FftFns fy = fft_fns(plan->p.grid[1]);
@@ -508,7 +508,8 @@ int build_fft(tn_plan *plan) {
         }
         TN_HIP(hipMalloc(&plan->d_fft_tw[a], sizeof(float2) * L));
         plan->fft_tw_bytes += (int64_t)sizeof(float2) * L;
-        TN_HIP(hipMemcpy(plan->d_fft_tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+        TN_HIP(hipMemcpyAsync(plan->d_fft_tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice, stream));
+        TN_HIP(hipStreamSynchronize(stream));  // before the host table goes out of scope
     }
     return TN_OK;
 }

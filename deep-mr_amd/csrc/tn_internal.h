@@ -216,7 +216,7 @@ int build_bins(tn_plan *plan, hipStream_t stream);
 int build_matrix(tn_plan *plan, hipStream_t stream);
 // Pruned, fused oversampled-grid FFT stages (2D plans with supported lengths).
 bool fft_supported(const tn_plan *plan);
-int build_fft(tn_plan *plan);
+int build_fft(tn_plan *plan, hipStream_t stream);
 int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, int64_t nimg, hipStream_t stream);
 int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t nimg, hipStream_t stream);
 int count_footprint(tn_plan *plan, hipStream_t stream);

@@ -471,8 +471,9 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
                               plan->grid_mode == kGridRuns ? kOrderOrigin : kOrderSpread);
         if (rc) return rc;
         plan->h_bin_off.assign(nbins + 1, 0);
-        TN_HIP(hipMemcpy(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1),
-                         hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpyAsync(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1),
+                              hipMemcpyDeviceToHost, stream));
+        TN_HIP(hipStreamSynchronize(stream));
     }
     // ---- interpolation bins: every sample once -----------------------------
     {
@@ -485,8 +486,9 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
         int rc = sort_entries(pgeo, plan->d_coord, keys.p, vals.p, total, plan->d_pbin_off,
                               &plan->d_pentries, stream);
         if (rc) return rc;
-        TN_HIP(hipMemcpy(plan->h_pbin_off.data(), plan->d_pbin_off, sizeof(int32_t) * (npbins + 1),
-                         hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpyAsync(plan->h_pbin_off.data(), plan->d_pbin_off, sizeof(int32_t) * (npbins + 1),
+                              hipMemcpyDeviceToHost, stream));
+        TN_HIP(hipStreamSynchronize(stream));
     }
     return TN_OK;
 }

@@ -43,14 +43,15 @@ def small_stream(n=6, seed=0, sizes=(64, 128, 192), niter=4):
                              samples=(3000, 12000), niter=niter)
 
 
-def test_alone_equals_mix_equals_plain():
-    print("server: alone vs mix vs plain", flush=True)
+@pytest.mark.parametrize("max_active, admission", [(32, "fifo"), (3, "sjf")])
+def test_alone_equals_mix_equals_plain(max_active, admission):
+    print(f"server: alone vs mix vs plain ({max_active} active, {admission})", flush=True)
     reqs = small_stream(8)
     alone = copy.deepcopy(reqs[2])
     srv = serve.ReconServer(DEV, hbm_budget=4 << 30)
     srv.run([alone], realtime=False)
     srv.close()
-    srv = serve.ReconServer(DEV, hbm_budget=4 << 30)
+    srv = serve.ReconServer(DEV, hbm_budget=4 << 30, max_active=max_active, admission=admission)
     srv.run(reqs, realtime=False)
     srv.close()
     assert all(r.error is None for r in reqs), [r.error for r in reqs]
@@ -103,7 +104,7 @@ def test_full_size_stream_alone_mix_plain():
     srv = serve.ReconServer(DEV, hbm_budget=32 << 30)
     srv.run([alone], realtime=False)
     srv.close()
-    srv = serve.ReconServer(DEV, hbm_budget=32 << 30)
+    srv = serve.ReconServer(DEV, hbm_budget=32 << 30, max_active=32)  # all six in one mix
     srv.run(reqs, realtime=False)
     srv.close()
     assert all(r.error is None for r in reqs), [r.error for r in reqs]
