@@ -173,15 +173,27 @@ class NativePlan:
             raise ValueError("coordinates out of range")
         self.ndim, self.nframes, self.npts = ndim, nframes, npts
         self.device = device
-        pad = 3 - ndim
         self.shape = tuple(int(s) for s in shape)
         self.grid = tuple(int(g) for g in grid)
         self.width = tuple(int(w) for w in width)
+        if ndim == 1:
+            # a 1D transform is the 2D one of a unit y axis: one grid row,
+            # width 1 (the Cartesian path: one tap of weight 1 at y = 0), the
+            # same x taps, scale and apodization (deepmr's kernel scaling and
+            # apodization give a width-1 axis the factor 1)
+            coord = np.ascontiguousarray(np.concatenate([coord, np.zeros_like(coord)], -1))
+            shape, grid, width = (1, *self.shape), (1, *self.grid), (1, *self.width)
+            beta, shift = (0.0, *beta), (0, *shift)
+            tile = None if tile is None else (1, *tile)
+            itile = None if itile is None else (1, *itile)
+            apod = None if apod is None else (None, *apod)
+            ndim = 2
+        pad = 3 - ndim
         p = _lib.PlanParams()
         p.ndim, p.nframes, p.npts = ndim, nframes, npts
-        p.shape[:] = [1] * pad + list(self.shape)
-        p.grid[:] = [1] * pad + list(self.grid)
-        p.width[:] = [1] * pad + list(self.width)
+        p.shape[:] = [1] * pad + [int(s) for s in shape]
+        p.grid[:] = [1] * pad + [int(g) for g in grid]
+        p.width[:] = [1] * pad + [int(w) for w in width]
         p.beta[:] = [0.0] * pad + [float(b) for b in beta]
         p.shift[:] = [0] * pad + [int(s) for s in shift]
         p.tile[:] = [0, 0, 0] if tile is None else [1] * pad + [int(t) for t in tile]

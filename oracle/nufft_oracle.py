@@ -326,10 +326,36 @@ def _tables(interp, ndim):
     return idx, val
 
 
+def _interp1(grid, idx, val):
+    """dm/fft/_interp/interp.py:125-151 (_interpolate1), vectorised over the
+    points: out[f, b, p] = sum_i val[f, p, i] * grid[f, b, idx[f, p, i]],
+    taps added in order as the reference's loop does."""
+    nframes, batch, _ = grid.shape
+    out = np.zeros((nframes, batch, idx.shape[1]), np.complex64)
+    for f in range(nframes):
+        for i in range(idx.shape[-1]):
+            out[f] += (val[f, :, i][None, :] * grid[f][:, idx[f, :, i]]).astype(np.complex64)
+    return out
+
+
+def _grid1(samp, idx, val, G):
+    """dm/fft/_interp/grid.py:182-207 (_gridding1): cart[f, b, idx[f, p, i]] +=
+    val[f, p, i] * samp[f, b, p] (np.add.at, tap-major rather than the
+    reference's point-major order: the same sums up to float rounding)."""
+    nframes, batch, _ = samp.shape
+    out = np.zeros((nframes, batch, G), np.complex64)
+    for f in range(nframes):
+        for b in range(batch):
+            for i in range(idx.shape[-1]):
+                np.add.at(out[f, b], idx[f, :, i], (val[f, :, i] * samp[f, b]).astype(np.complex64))
+    return out
+
+
 def interpolate_raw(grid, interp, fast=None):
     """Kernel call on (nframes, batch, *G) -> (nframes, batch, npts).
 
-    dm/fft/_interp/interp.py:154-227 (through oracle/nufft_loops.c).
+    dm/fft/_interp/interp.py:125-227 (2D / 3D through oracle/nufft_loops.c,
+    1D in numpy).
     """
     lib = load_loops(_FAST if fast is None else fast)
     ndim = interp["ndim"]
@@ -341,7 +367,9 @@ def interpolate_raw(grid, interp, fast=None):
     if nframes != idx[0].shape[0] or tuple(G) != tuple(interp["dshape"]):
         raise ValueError("grid shape does not match the interpolator")
     out = np.zeros((nframes, batch, npts), np.complex64)
-    if ndim == 2:
+    if ndim == 1:
+        out = _interp1(grid, idx[0], val[0])
+    elif ndim == 2:
         lib.orc_interp2(_ptr(out), _ptr(grid), _ptr(val[0]), _ptr(idx[0]), _ptr(val[1]),
                         _ptr(idx[1]), nframes, batch, npts, G[0], G[1],
                         idx[0].shape[-1], idx[1].shape[-1])
@@ -351,14 +379,15 @@ def interpolate_raw(grid, interp, fast=None):
                         G[0], G[1], G[2], idx[0].shape[-1], idx[1].shape[-1],
                         idx[2].shape[-1])
     else:
-        raise NotImplementedError("oracle covers 2D and 3D")
+        raise NotImplementedError("oracle covers 1D, 2D and 3D")
     return out
 
 
 def grid_raw(samp, interp, fast=None):
     """Kernel call on (nframes, batch, npts) -> (nframes, batch, *G).
 
-    dm/fft/_interp/grid.py:211-284 (through oracle/nufft_loops.c).
+    dm/fft/_interp/grid.py:182-284 (2D / 3D through oracle/nufft_loops.c,
+    1D in numpy).
     """
     lib = load_loops(_FAST if fast is None else fast)
     ndim = interp["ndim"]
@@ -369,7 +398,9 @@ def grid_raw(samp, interp, fast=None):
     if nframes != idx[0].shape[0] or npts != idx[0].shape[1]:
         raise ValueError("sample shape does not match the interpolator")
     out = np.zeros((nframes, batch, *G), np.complex64)
-    if ndim == 2:
+    if ndim == 1:
+        out = _grid1(samp, idx[0], val[0], G[0])
+    elif ndim == 2:
         lib.orc_grid2(_ptr(out), _ptr(samp), _ptr(val[0]), _ptr(idx[0]), _ptr(val[1]),
                       _ptr(idx[1]), nframes, batch, npts, G[0], G[1],
                       idx[0].shape[-1], idx[1].shape[-1])
@@ -379,7 +410,7 @@ def grid_raw(samp, interp, fast=None):
                       G[0], G[1], G[2], idx[0].shape[-1], idx[1].shape[-1],
                       idx[2].shape[-1])
     else:
-        raise NotImplementedError("oracle covers 2D and 3D")
+        raise NotImplementedError("oracle covers 1D, 2D and 3D")
     return out
 
 
@@ -399,8 +430,21 @@ def _lowrank_args(interp):
 def interpolate_lowrank_raw(grid, interp, basis_adjoint):
     """(ncoeff, batch, *G) -> (nframes, batch, npts) with basis_adjoint (nframes, ncoeff).
 
-    dm/fft/_interp/interp.py:268-357.
+    dm/fft/_interp/interp.py:231-357 (1D: _interpolate_lowrank1, :231-265, in
+    numpy: out[f, b, p] = sum_i val[f, p, i] sum_k ba[f, k] grid[k, b, idx]).
     """
+    if interp["ndim"] == 1:
+        idx, val = _tables(interp, 1)
+        idx, val = idx[0], val[0]
+        grid = np.asarray(grid, np.complex64)
+        ba = np.asarray(basis_adjoint, np.complex64)
+        nframes, npts = idx.shape[:2]
+        out = np.zeros((nframes, grid.shape[1], npts), np.complex64)
+        for f in range(nframes):
+            mixed = np.tensordot(ba[f], grid, axes=(0, 0)).astype(np.complex64)  # (batch, G)
+            for i in range(idx.shape[-1]):
+                out[f] += (val[f, :, i][None, :] * mixed[:, idx[f, :, i]]).astype(np.complex64)
+        return out
     lib = load_loops()
     z_v, z_i, y_v, y_i, x_v, x_i, wz = _lowrank_args(interp)
     grid = np.ascontiguousarray(grid, np.complex64)
@@ -424,8 +468,22 @@ def interpolate_lowrank_raw(grid, interp, basis_adjoint):
 def grid_lowrank_raw(samp, interp, basis):
     """(nframes, batch, npts) -> (ncoeff, batch, *G) with basis (ncoeff, nframes).
 
-    dm/fft/_interp/grid.py:326-417.
+    dm/fft/_interp/grid.py:288-417 (1D: _gridding_lowrank1, :288-322, in
+    numpy: cart[k, b, idx] += val * basis[k, f] * samp[f, b, p]).
     """
+    if interp["ndim"] == 1:
+        idx, val = _tables(interp, 1)
+        idx, val = idx[0], val[0]
+        samp = np.asarray(samp, np.complex64)
+        bs = np.asarray(basis, np.complex64)
+        nframes, batch, _ = samp.shape
+        G = interp["dshape"][0]
+        out = np.zeros((bs.shape[0], batch, G), np.complex64)
+        for k in range(bs.shape[0]):
+            for f in range(nframes):
+                g = _grid1((bs[k, f] * samp[f:f + 1]).astype(np.complex64), idx[f:f + 1], val[f:f + 1], G)
+                out[k] += g[0]
+        return out
     lib = load_loops()
     z_v, z_i, y_v, y_i, x_v, x_i, wz = _lowrank_args(interp)
     samp = np.ascontiguousarray(samp, np.complex64)

@@ -201,6 +201,29 @@ def test_plain_interpolator_matches_oracle():
     assert rel(gg, orc.apply_gridding(ks, oi)) < RTOL
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_interpolate_and_gridding_wrappers_match_oracle(ndim):
+    """deepmr's one-call wrappers (plan + apply, _interp/__init__.py:15-156)
+    with their default width 2 and beta 1 and with width 3: two frames, a
+    coil batch, numpy in and a tensor on the input's device out, as deepmr's,
+    against the oracle's plan + apply."""
+    n = 24 if ndim == 2 else 12
+    print(f"interpolate / gridding wrappers {ndim}D", flush=True)
+    rng = np.random.default_rng(14 + ndim)
+    coord = rng.uniform(0, n, (2, 4, 25, ndim)).astype(np.float32)
+    g = cnormal((3, 2) + (n,) * ndim, 15)
+    ks = cnormal((3, 2, 4, 25), 16)
+    for kw in ({}, {"width": 3, "beta": 2.5}):
+        w, b = kw.get("width", 2), kw.get("beta", 1.0)
+        oi = orc.plan_interpolator(coord, [n] * ndim, [w] * ndim, [np.float32(b)] * ndim)
+        s = tilenufft._interp.interpolate(g, torch.from_numpy(coord), device=DEV, **kw)
+        assert s.device.type == "cpu" and tuple(s.shape) == (3, 2, 4, 25)
+        assert rel(s.numpy(), orc.apply_interpolation(g, oi)) < RTOL
+        gg = tilenufft._interp.gridding(ks, torch.from_numpy(coord), [n] * ndim, device=DEV, **kw)
+        assert gg.device.type == "cpu" and tuple(gg.shape) == (3, 2) + (n,) * ndim
+        assert rel(gg.numpy(), orc.apply_gridding(ks, oi)) < RTOL
+
+
 def test_numpy_in_numpy_out_and_real_input():
     n = 16
     coord = testing.golden_angle_radial(8, 32, n)
@@ -431,3 +454,72 @@ def test_empty_trajectory_plan(mode):
     b.gridding(b.grid)
     torch.cuda.synchronize()
     assert float(b.grid.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("grid_mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("osf", [2.0, 1.25])
+def test_nufft1d_matches_oracle(grid_mode, osf):
+    """1D NUFFT (deepmr's _interpolate1 / _gridding1 path, interp.py:125,
+    grid.py:182): two frames of 60 random samples, three coils, every
+    gridding variant, forward and adjoint against the oracle."""
+    n = 40
+    print(f"1D NUFFT grid_mode={grid_mode} osf={osf}", flush=True)
+    rng = np.random.default_rng(21)
+    coord = rng.uniform(-n / 2, n / 2, (2, 1, 60, 1)).astype(np.float32)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, osf, device=DEV, grid_mode=grid_mode)
+    assert plan.ndim == 1 and plan.shape == (n,) and plan.os_shape == (int(np.ceil(osf * n)),)
+    oplan = orc.plan_nufft(coord, n, 4, osf)
+    x = cnormal((3, 2, n), 22)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    assert y.shape == (3, 2, 1, 60)
+    assert rel(y, orc.apply_nufft(x, oplan)) < RTOL
+    k = cnormal(y.shape, 23)
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    assert xa.shape == (3, 2, n)
+    assert rel(xa, orc.apply_nufft_adj(k, oplan)) < RTOL
+
+
+@pytest.mark.parametrize("ncontrasts, ncoils, nslices", [(1, 1, 1), (1, 2, 2), (2, 1, 2), (2, 2, 1)])
+def test_known_answers_1d(ncontrasts, ncoils, nslices):
+    """Restates deepmr's tests/fft/test_nufft.py test_nufft1 (:28-62): a
+    centred delta maps to all ones and back on integer coordinates."""
+    npix = 4
+    print(f"1D known answer ncontrasts={ncontrasts}", flush=True)
+    coord = (np.arange(npix) - npix // 2).astype(np.float32)[None, :, None]
+    if ncontrasts > 1:
+        coord = np.repeat(coord[None], ncontrasts, 0)
+    lead = (nslices, ncoils) if ncontrasts == 1 else (nslices, ncoils, ncontrasts)
+    img = torch.zeros((*lead, npix), dtype=torch.complex64)
+    img[..., npix // 2] = 1.0
+    k = tilenufft.fft.nufft(img, coord=torch.from_numpy(coord), device=DEV, width=12)
+    np.testing.assert_allclose(k.numpy(), np.ones((*lead, 1, npix)), rtol=1e-4, atol=1e-4)
+    back = tilenufft.fft.nufft_adj(torch.ones((*lead, 1, npix), dtype=torch.complex64),
+                                   coord=torch.from_numpy(coord), shape=[npix], device=DEV, width=12)
+    np.testing.assert_allclose(back.numpy(), img.numpy(), rtol=1e-4, atol=1e-4)
+
+
+def test_lowrank_1d_matches_oracle():
+    """1D subspace NUFFT (deepmr's _interpolate_lowrank1 / _gridding_lowrank1,
+    interp.py:231, grid.py:288) against the oracle, and the reference's
+    identity-basis known answer (tests/fft/test_nufft.py:65-97)."""
+    nf, nc, n = 4, 2, 32
+    print("1D low rank", flush=True)
+    rng = np.random.default_rng(24)
+    coord = rng.uniform(-n / 2, n / 2, (nf, 1, 30, 1)).astype(np.float32)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV)
+    oplan = orc.plan_nufft(coord, n, 4, 2.0)
+    basis = cnormal((nc, nf), 25)
+    ba = basis.conj().T.copy()
+    coef = cnormal((3, nc, n), 26)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(coef).to(DEV), plan, basis_adjoint=torch.from_numpy(ba))
+    assert rel(y.cpu().numpy(), orc.apply_nufft(coef, oplan, basis_adjoint=ba)) < RTOL
+    k = cnormal((3, nf, 1, 30), 27)
+    g = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan, basis=torch.from_numpy(basis))
+    assert rel(g.cpu().numpy(), orc.apply_nufft_adj(k, oplan, basis=basis)) < RTOL
+    npix = 4
+    cc = np.repeat((np.arange(npix) - npix // 2).astype(np.float32)[None, None, :, None], 3, 0)
+    img = torch.zeros((2, 2, 3, npix), dtype=torch.complex64)
+    img[..., npix // 2] = 1.0
+    kk = tilenufft.fft.nufft(img, coord=torch.from_numpy(cc), device=DEV, width=12,
+                             basis_adjoint=torch.eye(3, dtype=torch.complex64))
+    np.testing.assert_allclose(kk.numpy(), np.ones((2, 2, 3, 1, npix)), rtol=1e-4, atol=1e-4)
