@@ -230,28 +230,33 @@ __global__ __launch_bounds__(kBlock) void interp_lowrank_kernel(const Job *__res
 template <int NDIM, int WC, bool FULL>
 __device__ __forceinline__ float2 staged_taps(const float2 *r, const float (&w)[3][WC], int nx, int ny, int nz,
                                               int Rx, int Ry) {
-    constexpr int KZ = NDIM > 2 ? WC : 1;
     float re = 0.0f, im = 0.0f;
+    auto plane = [&](int kz, float wz) {
 #pragma unroll
-    for (int kz = 0; kz < KZ; ++kz) {
-        if (FULL || kz < nz) {
+        for (int ky = 0; ky < WC; ++ky) {
+            if (FULL || ky < ny) {
+                const float vzy = wz * w[1][ky];
+                const float2 *row = r + (kz * Ry + ky) * Rx;
 #pragma unroll
-            for (int ky = 0; ky < WC; ++ky) {
-                if (FULL || ky < ny) {
-                    const float vzy = w[2][kz] * w[1][ky];
-                    const float2 *row = r + (kz * Ry + ky) * Rx;
-#pragma unroll
-                    for (int kx = 0; kx < WC; ++kx) {
-                        if (FULL || kx < nx) {
-                            const float v = vzy * w[0][kx];
-                            const float2 sv = row[kx];
-                            re += v * sv.x;
-                            im += v * sv.y;
-                        }
+                for (int kx = 0; kx < WC; ++kx) {
+                    if (FULL || kx < nx) {
+                        const float v = vzy * w[0][kx];
+                        const float2 sv = row[kx];
+                        re += v * sv.x;
+                        im += v * sv.y;
                     }
                 }
             }
         }
+    };
+    if constexpr (NDIM > 2) {
+        // one z plane of the footprint at a time: its W^2 LDS reads are in
+        // flight together, not all W^3 (which would hold 2 W^3 registers)
+#pragma unroll 1
+        for (int kz = 0; kz < WC; ++kz)
+            if (FULL || kz < nz) plane(kz, w[2][kz]);
+    } else {
+        plane(0, w[2][0]);
     }
     return make_float2(re, im);
 }
