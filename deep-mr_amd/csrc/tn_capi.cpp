@@ -170,12 +170,12 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         max_region = std::max<int64_t>(max_region, (int64_t)(j.IT[0] + j.W[0] - 1) *
                                                        (j.IT[1] + j.W[1] - 1) * (j.IT[2] + j.W[2] - 1));
     }
-    const int wc = t.kind_grid % 16;
+    const int wc = kind_width(t.kind_grid);
     const int gmode = plan_of.empty() ? kGridSpread : plan_of[0]->grid_mode;
     for (const tn_plan *pl : plan_of)
         if (pl->grid_mode != gmode) return fail(TN_E_ARG, "batched plans must share the gridding variant");
     t.grid_mode = gmode;
-    const int gndim = t.kind_grid / 16;
+    const int gndim = kind_ndim(t.kind_grid);
     if (gmode != kGridMatrix && grid_lds_bytes(1, max_plane, wc, gmode, gndim) > kGridLdsBudget)
         return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
                     grid_lds_bytes(1, max_plane, wc, gmode, gndim));
@@ -189,6 +189,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     int mode = plan_of.empty() ? kInterpGather : plan_of[0]->p.interp_mode;
     if (mode == kInterpAuto) mode = kInterpStaged;
     if (mode == kInterpStaged && region_coil > kInterpLdsBudget) mode = kInterpGather;
+    if (kind_width(t.kind_interp) > kStagedMaxWidth) mode = kInterpGather;  // wide kernels gather
     if (force_gather) mode = kInterpGather;
     t.interp_mode = mode;
 
@@ -209,7 +210,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         // coils per lane: two for 16-coil records, and for 4-8 coil records
         // in 2D (half the gather instructions); one for 4-8 coil records in
         // 3D, where twice the cells per wave measured slower (divergence)
-        mat_cpl = (cwmax == 16 || (cwmax >= 4 && t.kind_grid / 16 == 2)) ? 2 : 1;
+        mat_cpl = (cwmax == 16 || (cwmax >= 4 && kind_ndim(t.kind_grid) == 2)) ? 2 : 1;
         if (!inplace && cwmax > 0) mat_cpi = std::min(kBlock, kBlock / (cwmax / mat_cpl) * kMatrixPasses);
     }
     std::vector<Item> interp, grid, fold, tr;
@@ -363,8 +364,8 @@ int validate(const tn_plan_params *p) {
         }
         if (p->shape[d] < 1 || p->grid[d] < p->shape[d])
             return fail(TN_E_ARG, "axis %d: need 1 <= shape (%d) <= grid (%d)", d, p->shape[d], p->grid[d]);
-        if (p->width[d] < 1 || p->width[d] > 8 || p->width[d] > p->grid[d])
-            return fail(TN_E_ARG, "axis %d: width %d unsupported (1..8, <= grid)", d, p->width[d]);
+        if (p->width[d] < 1 || p->width[d] > kMaxWidth || p->width[d] > p->grid[d])
+            return fail(TN_E_ARG, "axis %d: width %d unsupported (1..%d, <= grid)", d, p->width[d], kMaxWidth);
         if (p->shift[d] < 0 || p->shift[d] >= p->grid[d])
             return fail(TN_E_ARG, "axis %d: shift %d outside [0, grid)", d, p->shift[d]);
         if (p->tile[d] < 0 || p->itile[d] < 0) return fail(TN_E_ARG, "axis %d: negative tile", d);
@@ -524,6 +525,15 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
                         : matrix_ok                        ? kGridMatrix
                         : runs_ok                          ? kGridRuns
                                                            : kGridSpread;
+        // kernels wider than kStagedMaxWidth grid only through the matrix
+        const int wmax = std::max(params->width[0], std::max(params->width[1], params->width[2]));
+        if (wmax > kStagedMaxWidth && pl->grid_mode != kGridMatrix) {
+            delete pl;
+            return fail(TN_E_ARG,
+                        "width %d: kernels wider than %d grid through the gridding matrix (grid_mode 3), "
+                        "which needs %lld entries here (at most %lld bytes)",
+                        wmax, kStagedMaxWidth, (long long)nnz, (long long)kMatrixAutoBytes);
+        }
     }
     const int64_t ncoord = (int64_t)params->nframes * params->npts * params->ndim;
     auto cleanup = [&](int code) {

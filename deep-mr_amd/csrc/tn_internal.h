@@ -16,6 +16,12 @@ namespace tn {
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWave = 64;
 constexpr int kMaxWidth = 16;      // largest kernel width supported per axis
+constexpr int kStagedMaxWidth = 8; // widest kernel of the staged interpolation and the tile gridding
+                                   // variants; wider kernels gather and grid through the matrix
+// launch "kind" of a table: ndim * kKindStride + widest kernel width
+constexpr int kKindStride = 32;
+constexpr int kind_ndim(int kind) { return kind / kKindStride; }
+constexpr int kind_width(int kind) { return kind % kKindStride; }
 constexpr int kInterpBlockPts = 256;  // samples per interpolation work item
 constexpr size_t kGridLdsBudget = 80 * 1024;  // LDS per gridding workgroup (two per CU)
 constexpr size_t kInterpLdsBudget = 64 * 1024;  // LDS per staged-interpolation workgroup

@@ -1341,6 +1341,8 @@ InterpLrFn pick_interp_lowrank(int wc) {
         case 6: return interp_lowrank_kernel<NDIM, 6>;
         case 7: return interp_lowrank_kernel<NDIM, 7>;
         case 8: return interp_lowrank_kernel<NDIM, 8>;
+        case 9: case 10: case 11: case 12: return interp_lowrank_kernel<NDIM, 12>;
+        case 13: case 14: case 15: case 16: return interp_lowrank_kernel<NDIM, 16>;
         default: return nullptr;
     }
 }
@@ -1356,6 +1358,8 @@ InterpFn pick_interp(int wc) {
         case 6: return interp_kernel<NDIM, 6>;
         case 7: return interp_kernel<NDIM, 7>;
         case 8: return interp_kernel<NDIM, 8>;
+        case 9: case 10: case 11: case 12: return interp_kernel<NDIM, 12>;
+        case 13: case 14: case 15: case 16: return interp_kernel<NDIM, 16>;
         default: return nullptr;
     }
 }
@@ -1405,20 +1409,20 @@ GridFn pick_grid(int wc) {
 
 }  // namespace
 
-// kind = ndim * 16 + max width
+// kind = ndim * kKindStride + max width
 int interp_kind(const std::vector<Job> &jobs) {
     int ndim = 0, wc = 1;
     for (const Job &j : jobs) {
         ndim = std::max(ndim, j.ndim);
         for (int d = 0; d < 3; ++d) wc = std::max(wc, j.W[d]);
     }
-    return ndim * 16 + wc;
+    return ndim * kKindStride + wc;
 }
 
 int launch_interp(const LaunchTables &t, const float2 *grid, float2 *samp, const float *weight,
                   hipStream_t stream) {
     if (t.n_interp == 0) return TN_OK;
-    const int ndim = t.kind_interp / 16, wc = t.kind_interp % 16;
+    const int ndim = kind_ndim(t.kind_interp), wc = kind_width(t.kind_interp);
     if (t.interp_mode == kInterpStaged) {
         StagedFn fn = ndim == 2 ? pick_staged<2>(wc) : ndim == 3 ? pick_staged<3>(wc) : nullptr;
         if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
@@ -1465,7 +1469,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         MtFn mt = nullptr;
         const bool pair = t.st_cpl == 2;  // two coils per lane (the table's cells per item assume it)
         // entries in flight per lane: 4 in 2D, 8 in 3D (each measured fastest)
-        const bool d3 = t.kind_grid / 16 == 3;
+        const bool d3 = kind_ndim(t.kind_grid) == 3;
         switch (t.st_cw) {
             case 2:
                 tr = samp_transpose_kernel<1>;
@@ -1507,7 +1511,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         TN_HIP(hipGetLastError());
         return launch_matrix_fold(t, grid, scratch, stream);
     }
-    const int ndim = t.kind_grid / 16, wc = t.kind_grid % 16;
+    const int ndim = kind_ndim(t.kind_grid), wc = kind_width(t.kind_grid);
     GridFn fn = nullptr;
     if (t.grid_mode == kGridRuns)
         fn = ndim == 2 ? pick_runs<2>(wc) : ndim == 3 ? pick_runs<3>(wc) : nullptr;
@@ -1553,7 +1557,7 @@ int launch_interp_lowrank(const LaunchTables &t, const float2 *grid, float2 *sam
                           hipStream_t stream, LowRank lr) {
     if (t.n_interp == 0) return TN_OK;
     if (t.interp_mode != kInterpGather) return fail(TN_E_ARG, "low-rank interpolation needs gather tables");
-    const int ndim = t.kind_interp / 16, wc = t.kind_interp % 16;
+    const int ndim = kind_ndim(t.kind_interp), wc = kind_width(t.kind_interp);
     InterpLrFn fn = ndim == 2 ? pick_interp_lowrank<2>(wc) : ndim == 3 ? pick_interp_lowrank<3>(wc) : nullptr;
     if (!fn) return fail(TN_E_ARG, "unsupported kernel (ndim=%d, width=%d)", ndim, wc);
     hipLaunchKernelGGL(fn, dim3(xcd_blocks(t.n_interp)), dim3(kBlock), 0, stream, t.d_jobs, t.d_interp,

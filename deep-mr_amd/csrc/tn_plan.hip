@@ -142,10 +142,10 @@ __global__ void matrix_emit_kernel(BinGeom geo, const float *__restrict__ coord,
         const int64_t frame = i / geo.npts;
         const int64_t p = i - frame * geo.npts;
         int base[3] = {0, 0, 0};
-        float w[3][8];
+        float w[3][kMaxWidth];
         for (int d = 0; d < 3; ++d) {
             if (d < geo.ndim) {
-                axis_taps<8>(coord[i * geo.ndim + d], geo.ax[d].W, geo.ax[d].G, geo.ax[d].shift, beta_of(geo, d),
+                axis_taps<kMaxWidth>(coord[i * geo.ndim + d], geo.ax[d].W, geo.ax[d].G, geo.ax[d].shift, beta_of(geo, d),
                              base[d], w[d]);
             } else {
                 w[d][0] = 1.0f;

@@ -52,7 +52,9 @@ typedef struct tn_plan_params {
     int64_t npts;       /* samples per frame (nviews * nsamples)                */
     int32_t shape[3];   /* image size N                                         */
     int32_t grid[3];    /* oversampled grid G = ceil(osf * N)                   */
-    int32_t width[3];   /* kernel width; 1 marks a Cartesian axis               */
+    int32_t width[3];   /* kernel width 1..16; 1 marks a Cartesian axis; widths
+                           above 8 gather (interp_mode 1) and grid through
+                           the matrix (grid_mode 3)                          */
     float beta[3];      /* Kaiser-Bessel beta                                   */
     int32_t shift[3];   /* output index = (tap index + shift) mod G; the fused
                            NUFFT uses G - G/2 (folds the centring shifts), the

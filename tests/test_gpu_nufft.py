@@ -523,3 +523,33 @@ def test_lowrank_1d_matches_oracle():
     kk = tilenufft.fft.nufft(img, coord=torch.from_numpy(cc), device=DEV, width=12,
                              basis_adjoint=torch.eye(3, dtype=torch.complex64))
     np.testing.assert_allclose(kk.numpy(), np.ones((2, 2, 3, 1, npix)), rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("ndim, n, width", [(2, 24, 12), (2, 20, 16), (3, 12, 10)])
+def test_wide_kernels_match_oracle(ndim, n, width):
+    """Kernel widths above 8 (deepmr accepts any width): the gathering
+    interpolation and the matrix gridding, forward and adjoint, plus the
+    low-rank pair in 2D, against the oracle; the tile gridding variants
+    refuse them with an error."""
+    print(f"wide kernel ndim={ndim} N={n} W={width}", flush=True)
+    rng = np.random.default_rng(31 + width)
+    coord = rng.uniform(-n / 2, n / 2, (2, 1, 50, ndim)).astype(np.float32)
+    plan = tilenufft.fft.plan_nufft(coord, n, width, 2.0, device=DEV)
+    oplan = orc.plan_nufft(coord, n, width, 2.0)
+    x = cnormal((2, 2) + (n,) * ndim, 32)
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    assert rel(y, orc.apply_nufft(x, oplan)) < RTOL
+    k = cnormal(y.shape, 33)
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    assert rel(xa, orc.apply_nufft_adj(k, oplan)) < RTOL
+    if ndim == 2:
+        basis = cnormal((1, 2), 34)  # (ncoeff, nframes)
+        ba = basis.conj().T.copy()
+        coef = cnormal((2, 1, n, n), 35)
+        yl = tilenufft.fft.apply_nufft(torch.from_numpy(coef).to(DEV), plan, basis_adjoint=torch.from_numpy(ba))
+        assert rel(yl.cpu().numpy(), orc.apply_nufft(coef, oplan, basis_adjoint=ba)) < RTOL
+        gl = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan, basis=torch.from_numpy(basis))
+        assert rel(gl.cpu().numpy(), orc.apply_nufft_adj(k, oplan, basis=basis)) < RTOL
+    for mode in (1, 2):
+        with pytest.raises(RuntimeError):
+            tilenufft.fft.plan_nufft(coord, n, width, 2.0, device=DEV, grid_mode=mode)

@@ -236,3 +236,18 @@ def test_lowrank_toeplitz_kernel_is_basis_combination_of_frame_kernels():
     per_frame = orc.apply_selfadj(np.einsum("cf,bc...->bf...", basis.conj(), x).astype(np.complex64), kf, ks)
     ref = s * np.einsum("cf,bf...->bc...", basis, per_frame)
     assert rel(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("ndim, n, width", [(2, 24, 12), (2, 20, 16), (3, 12, 10)])
+def test_wide_kernels_vs_ndft(ndim, n, width):
+    """Widths above 8 (deepmr accepts any width): at osf 2 the KB NUFFT is
+    then accurate to float32 round-off against the FP64 NDFT."""
+    rng = np.random.default_rng(40 + width)
+    coord = rng.uniform(-n / 2, n / 2, (120, ndim)).astype(np.float32)
+    x = cn(rng, (n,) * ndim)
+    y = orc.nufft(x, coord, n, width, 2.0)
+    assert rel(y, ndft.ndft(x, coord, (n,) * ndim).reshape(y.shape)) < 1e-5
+    plan = orc.plan_nufft(coord, n, width, 2.0)
+    k = cn(rng, y.shape)
+    exact = ndft.ndft_adjoint(k.reshape(-1), coord, (n,) * ndim) / np.prod(plan["os_shape"])
+    assert rel(orc.apply_nufft_adj(k, plan), exact) < 1e-5
