@@ -53,8 +53,17 @@ def make_spiral_workload(device, nframes=8, ncoils=16, n=256, nsamp=2048, osf=2.
                 n=n, ncoils=ncoils, osf=osf, width=width, nsamp=nsamp)
 
 
-def variant_key(plan, kind):
-    """Launch variant of a workload's dominant kernel, as the traffic file keys it."""
+def variant_key(plan, kind, ncoils):
+    """Launch variant of a workload's dominant kernel, as the traffic file
+    keys it: the effective launch configuration (requested 0 = automatic
+    values resolved as the library resolves them), so equal launches get
+    equal keys."""
+    if kind == "grid" and plan.grid_mode in (0, 3) and plan.interpolator.grid_mode_effective() == 3:
+        mc = min(plan.grid_chunk if plan.grid_chunk > 0 else 16, 16, ncoils)
+        record = 2 if mc <= 2 else 4 if mc <= 4 else 8 if mc <= 8 else 16
+        split = plan.grid_split
+        long_row = 128 if split == 0 else 1024 if split < 0 else min(split, 1024)
+        return f"grid|matrix|record={record}|long={long_row}"
     if kind == "grid":
         return (f"grid|mode={plan.grid_mode}|tile={list(plan.interpolator.tile())}|chunk={plan.grid_chunk}"
                 f"|split={plan.grid_split}")
@@ -131,7 +140,7 @@ def fold_counters(dirs, bench_line, copy_mib):
             scale[c] = copy_bytes / (float(np.mean(cal)) * 1024.0)
     with open(bench_line) as f:
         line = json.loads([ln for ln in f if ln.startswith("{")][-1])
-    variants = line["kernels"]["variants"]
+    variants = line.get("kernels", {}).get("variants", {})
     stages = {}
     for (k, c), vs in sorted(per.items()):
         st = _stage_of(k)
@@ -139,7 +148,7 @@ def fold_counters(dirs, bench_line, copy_mib):
             continue
         e = stages.setdefault(st, {"kernels": {}})
         e["kernels"].setdefault(k, {})[c] = float(np.mean(vs))
-    out = {"workload": line["config"]["workload"], "copy_mib": copy_mib,
+    out = {"workload": line.get("config", line)["workload"], "copy_mib": copy_mib,
            "fetch_scale": scale.get("FETCH_SIZE"), "write_scale": scale.get("WRITE_SIZE"),
            "variants": {}, "stages": {}}
     for st, e in stages.items():
@@ -152,9 +161,21 @@ def fold_counters(dirs, bench_line, copy_mib):
             rec["fetch_bytes"] = tot["FETCH_SIZE"] * 1024.0 * scale["FETCH_SIZE"]
             rec["write_bytes"] = tot["WRITE_SIZE"] * 1024.0 * scale["WRITE_SIZE"]
             rec["bytes"] = rec["fetch_bytes"] + rec["write_bytes"]
-            out["variants"][variants[st]] = rec["bytes"]
-        out["stages"][variants[st]] = rec
+            out["variants"][variants.get(st, st)] = rec["bytes"]
+        out["stages"][variants.get(st, st)] = rec
     return out
+
+
+def calibration_copy(dev, mib):
+    """Three plain elementwise copies of ``mib`` MiB (read once, written
+    once): the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
+    counters of a profiled run to bytes."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize(dev)
 
 
 def time_kernel(fn, reps, device):
@@ -479,7 +500,10 @@ def main():
     if args.kooshball:
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        print(json.dumps(kooshball_bench(dev, compare_runs=True)), flush=True)
+        res = kooshball_bench(dev, compare_runs=not args.kernels_only)  # --kernels-only: the automatic plan only
+        if args.calibrate_copy > 0:
+            calibration_copy(dev, args.calibrate_copy)
+        print(json.dumps(res), flush=True)
         return
     if args.cgsense:
         cgsense_bench()
@@ -563,7 +587,7 @@ def main():
     # HBM bytes per launch of the timed variant, from rocprofv3 FETCH_SIZE /
     # WRITE_SIZE runs of this workload (scaled by a copy kernel's counters);
     # null when the autotuner picked a variant that was not profiled
-    variant = variant_key(plans[0], dominant)
+    variant = variant_key(plans[0], dominant, ncoils)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_spiral2d.json")
     if os.path.exists(tpath):
@@ -580,13 +604,7 @@ def main():
         cpu, parity = cpu_baseline(wl, batch)
 
     if args.calibrate_copy > 0:
-        n = args.calibrate_copy * (1 << 20) // 4
-        src = torch.ones(n, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.mul(src, 1.0, out=dst)
-        torch.cuda.synchronize(dev)
-        del src, dst
+        calibration_copy(dev, args.calibrate_copy)
 
     config1 = None
     if not args.no_config1 and not args.kernels_only:
@@ -647,7 +665,7 @@ def main():
             "image_copy_ms": t_copy,
             **{f"{k}_ms": v for k, v in t_fft.items()},
             **({"fft_alg_bytes": b_fft} if t_fft else {}),
-            "variants": {k: variant_key(plans[0], k) for k in ("grid", "interp", *t_fft)},
+            "variants": {k: variant_key(plans[0], k, ncoils) for k in ("grid", "interp", *t_fft)},
             **{f"{k}_frac": b_fft / (v * 1e-3) / 1e9 / HBM_PEAK_GBS for k, v in t_fft.items()},
         },
         "cpu_baseline": cpu,

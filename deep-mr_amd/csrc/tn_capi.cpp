@@ -593,6 +593,12 @@ int64_t tn_plan_device_bytes(const tn_plan *plan) {
 
 int64_t tn_plan_entries(const tn_plan *plan) { return plan ? plan->nentries : -1; }
 
+int tn_plan_grid_mode(const tn_plan *plan) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    return plan->grid_mode;
+}
+
 int64_t tn_plan_footprint_cells(const tn_plan *plan_c) {
     clear_error();
     tn_plan *plan = const_cast<tn_plan *>(plan_c);

@@ -221,6 +221,37 @@ int check_group(const tn_sense_job *jobs, int64_t njobs, int64_t npix) {
     return TN_OK;
 }
 
+// Coil combination of SenseAdjointOp (src/deepmr/linops/coil.py:101-205):
+// x[b, k, p] = sum_s sum_c conj(S[s, b, c, p]) * y[s, b, c, k, p], coils
+// summed first, then sets (deepmr's .sum(coil).sum(0) order). One thread per
+// output element; the maps are read once per contrast.
+__global__ __launch_bounds__(kBlock) void coil_combine_kernel(const float2 *__restrict__ smaps,
+                                                             const float2 *__restrict__ y,
+                                                             float2 *__restrict__ x, int64_t nsets,
+                                                             int64_t nb, int64_t ncoils, int64_t nk,
+                                                             int64_t npix) {
+    const int64_t total = nb * nk * npix;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t p = i % npix;
+        const int64_t k = (i / npix) % nk;
+        const int64_t b = i / (npix * nk);
+        float2 acc = make_float2(0.0f, 0.0f);
+        for (int64_t st = 0; st < nsets; ++st) {
+            float2 cs = make_float2(0.0f, 0.0f);
+            const float2 *sm = smaps + ((st * nb + b) * ncoils) * npix + p;
+            const float2 *yy = y + (((st * nb + b) * ncoils) * nk + k) * npix + p;
+            for (int64_t c = 0; c < ncoils; ++c) {
+                const float2 v = cmulc(sm[c * npix], yy[c * nk * npix]);
+                cs.x += v.x;
+                cs.y += v.y;
+            }
+            acc.x += cs.x;
+            acc.y += cs.y;
+        }
+        x[i] = acc;
+    }
+}
+
 }  // namespace
 }  // namespace tn
 
@@ -270,6 +301,21 @@ int tn_cg_step(const tn_sense_job *jobs, int64_t njobs, int64_t npix, float *wor
     hipLaunchKernelGGL(cg_update_kernel, grid, dim3(kBlock), 0, s, jobs, npix, work, nblk);
     TN_HIP(hipGetLastError());
     hipLaunchKernelGGL(cg_direction_kernel, grid, dim3(kBlock), 0, s, jobs, npix, (const float *)work, nblk);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+int tn_coil_combine(const void *smaps, const void *y, void *x, int64_t nsets, int64_t nb, int64_t ncoils,
+                    int64_t nk, int64_t npix, tn_stream stream) {
+    clear_error();
+    if (nsets < 1 || nb < 1 || ncoils < 1 || nk < 1 || npix < 0)
+        return fail(TN_E_ARG, "coil combination sizes must be >= 1");
+    const int64_t total = nb * nk * npix;
+    if (total == 0) return TN_OK;
+    if (!smaps || !y || !x) return fail(TN_E_ARG, "NULL data pointer");
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + kBlock - 1) / kBlock, 65535);
+    hipLaunchKernelGGL(coil_combine_kernel, dim3(blocks), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const float2 *)smaps, (const float2 *)y, (float2 *)x, nsets, nb, ncoils, nk, npix);
     TN_HIP(hipGetLastError());
     return TN_OK;
 }

@@ -235,6 +235,13 @@ class NativePlan:
             _lib.check(n, "tn_plan_footprint_cells")
         return n
 
+    def grid_mode_effective(self):
+        """The gridding variant the library chose (1, 2 or 3)."""
+        m = int(_lib.load().tn_plan_grid_mode(self.handle))
+        if m < 0:
+            _lib.check(m, "tn_plan_grid_mode")
+        return m
+
     def tile(self):
         t = (ctypes.c_int32 * 3)()
         _lib.check(_lib.load().tn_plan_tile(self.handle, t), "tn_plan_tile")

@@ -33,8 +33,10 @@ class CGStep(nn.Module):
         self.tol = tol
 
     def dot(self, s1, s2):
-        d = (s1.conj() * s2).reshape(*s1.shape[: -self.ndim], -1).sum(dim=-1)
-        return d.reshape(*d.shape, *([1] * self.ndim))
+        # sum(conj(s1) * s2) over the last ndim axes, as one reduction
+        n = self.ndim
+        d = torch.linalg.vecdot(s1.reshape(*s1.shape[:-n], -1), s2.reshape(*s2.shape[:-n], -1), dim=-1)
+        return d.reshape(*d.shape, *([1] * n))
 
     def forward(self, x):
         AHAp = self.AHA(self.p)

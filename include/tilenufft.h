@@ -109,6 +109,10 @@ int64_t tn_plan_entries(const tn_plan *plan);
 int64_t tn_plan_footprint_cells(const tn_plan *plan);
 /* Effective gridding tile (z, y, x). */
 int tn_plan_tile(const tn_plan *plan, int32_t tile_out[3]);
+/* Effective gridding variant (1, 2 or 3, with grid_mode 0 resolved), or a
+ * negative status for a NULL plan. No counterpart in deepmr (introspection
+ * for tuners and benchmarks). */
+int tn_plan_grid_mode(const tn_plan *plan);
 
 /* Builds (once) the device launch tables for `nbatch` coils so that later
  * apply calls with that nbatch allocate nothing (needed before graph
@@ -251,6 +255,14 @@ int tn_sense_expand(const tn_sense_job *jobs, int64_t njobs, int64_t npix, void 
  * (src/deepmr/linops/coil.py:101-205). */
 int tn_sense_combine(const tn_sense_job *jobs, int64_t njobs, int64_t npix, const void *img,
                      tn_stream stream);
+
+/* Coil combination of SenseAdjointOp (src/deepmr/linops/coil.py:101-205) on
+ * contiguous complex64 arrays: smaps (nsets, nb, ncoils, npix), y (nsets, nb,
+ * ncoils, nk, npix) -> x (nb, nk, npix) with x[b, k, p] = sum over sets of
+ * the sum over coils of conj(smaps[s, b, c, p]) * y[s, b, c, k, p] (coils
+ * first, in order). One launch, no allocation. */
+int tn_coil_combine(const void *smaps, const void *y, void *x, int64_t nsets, int64_t nb, int64_t ncoils,
+                    int64_t nk, int64_t npix, tn_stream stream);
 
 /* Floats of `work` that tn_cg_step needs for njobs records of npix pixels. */
 int64_t tn_cg_work_floats(int64_t njobs, int64_t npix);

@@ -429,6 +429,7 @@ def test_grid_runs_refuses_wide_kernels():
     with pytest.raises(RuntimeError, match="taps"):
         tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV, grid_mode=2)
     plan = tilenufft.fft.plan_nufft(coord, 24, 7, 2.0, device=DEV)  # automatic: the gridding matrix (49 taps)
+    assert plan.interpolator.grid_mode_effective() == 3
     k = torch.from_numpy(cnormal((1,) + coord.shape[:-1], 52)).to(DEV)
     ref = orc.apply_nufft_adj(k.cpu().numpy(), orc.plan_nufft(coord, 24, 7, 2.0))
     assert rel(tilenufft.fft.apply_nufft_adj(k, plan).cpu().numpy(), ref) < RTOL
