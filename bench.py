@@ -62,7 +62,8 @@ def variant_key(plan, kind, ncoils):
         mc = min(plan.grid_chunk if plan.grid_chunk > 0 else 16, 16, ncoils)
         record = 2 if mc <= 2 else 4 if mc <= 4 else 8 if mc <= 8 else 16
         split = plan.grid_split
-        long_row = 128 if split == 0 else 1024 if split < 0 else min(split, 1024)
+        auto_long = 256 if plan.ndim == 3 else 128  # the library's defaults
+        long_row = auto_long if split == 0 else 1024 if split < 0 else min(split, 1024)
         return f"grid|matrix|record={record}|long={long_row}"
     if kind == "grid":
         return (f"grid|mode={plan.grid_mode}|tile={list(plan.interpolator.tile())}|chunk={plan.grid_chunk}"

@@ -60,7 +60,9 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.pbin_off = plan->d_pbin_off + (int64_t)f * plan->nitiles_total;
         j.mrow = plan->d_mrow ? plan->d_mrow + (int64_t)f * gvol : nullptr;
         j.mnz = plan->d_mnz;
-        j.mlong = p.grid_split == 0 ? kMatrixLongRow : p.grid_split < 0 ? kMatrixStage : std::min(p.grid_split, kMatrixStage);
+        j.mlong = p.grid_split == 0  ? (p.ndim == 3 ? kMatrixLongRow3d : kMatrixLongRow)
+                  : p.grid_split < 0 ? kMatrixStage
+                                     : std::min(p.grid_split, kMatrixStage);
         // deepmr layout (batch, nframes, ...): frame f of coil b
         j.grid_off = grid_off + (int64_t)f * gvol;
         j.grid_bstride = (int64_t)p.nframes * gvol;
