@@ -27,11 +27,13 @@ int main() {
         const int mlong = 16 + rng() % (tn::kMatrixStage - 15);
         const int nbatch = 1 + rng() % 40;
         const int mc = 1 + rng() % tn::kMatrixCoils;
-        // row lengths: mostly empty or short, a few long (a dense centre)
+        // row lengths: mostly empty or short, a few long (a dense centre);
+        // every seventh case has no entries at all (a plan without samples)
+        const bool empty = cs % 7 == 3;
         std::vector<int32_t> row(ncell + 1, 0);
         for (int64_t c = 0; c < ncell; ++c) {
             const int r = rng() % 100;
-            const int len = r < 30 ? 0 : r < 95 ? rng() % 40 : r < 98 ? rng() % (2 * mlong) : rng() % 9000;
+            const int len = empty ? 0 : r < 30 ? 0 : r < 95 ? rng() % 40 : r < 98 ? rng() % (2 * mlong) : rng() % 9000;
             row[c + 1] = row[c] + len;
         }
         std::vector<tn::MatrixRun> runs;
@@ -112,6 +114,7 @@ int main() {
             }
         }
         if (gi != grid.size()) return fail("extra grid items", cs, -1);
+        if (empty && (!fold.empty() || scratch != 0)) return fail("an empty matrix needs no parts", cs, -1);
         if (fi != fold.size()) return fail("extra fold items", cs, -1);
         ++cases;
     }

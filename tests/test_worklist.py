@@ -4,7 +4,8 @@ The matrix-gridding items are built on the host from the plan's CSR row
 offsets (deep-mr_amd/csrc/tn_worklist.h). A host-only program compiled from
 that header with g++ checks, over 400 random row-length / threshold / cell
 limit / coil-record configurations, that the runs cover every cell exactly
-once (long rows first), that short runs respect the kernel's cells per item
+once (long rows first; all-empty matrices of sample-less plans included),
+that short runs respect the kernel's cells per item
 and LDS entry stage and carry the entry range of their rows, and that long
 rows are split into in-order parts whose partial records are disjoint and
 inside the scratch size, with one fold item each.
