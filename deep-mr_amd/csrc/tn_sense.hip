@@ -221,6 +221,211 @@ int check_group(const tn_sense_job *jobs, int64_t njobs, int64_t npix) {
     return TN_OK;
 }
 
+// ---- one differentiable CG iteration (CGStep.forward, cg.py:203-212) ------
+// B independent systems of n complex elements, (B, n) contiguous; rho, rho1
+// real per system; ad = (alpha, d) per system, kept for the backward. The
+// block partition (kSenseEpb elements per block) and the order of every sum
+// depend only on n.
+constexpr int kCgStride = 8;  // floats per block partial record
+
+__device__ __forceinline__ float2 cdiv(float2 a, float2 b) {  // a / b
+    const float den = b.x * b.x + b.y * b.y;
+    return make_float2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+
+// d = <p, q> = sum conj(p) q, block partials
+__global__ __launch_bounds__(kBlock) void cgf_dot_kernel(const float2 *__restrict__ p, const float2 *__restrict__ q,
+                                                         int64_t n, float *__restrict__ work, int nblk) {
+    const int64_t off = (int64_t)blockIdx.y * n;
+    const int64_t i0 = (int64_t)blockIdx.x * kSenseEpb + threadIdx.x;
+    float a = 0.0f, b = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kSenseEpl; ++k) {
+        const int64_t i = i0 + k * kBlock;
+        if (i < n) {
+            const float2 t = cmulc(p[off + i], q[off + i]);
+            a += t.x;
+            b += t.y;
+        }
+    }
+    const float2 s = block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        float *w = work + ((int64_t)blockIdx.y * nblk + blockIdx.x) * kCgStride;
+        w[0] = s.x;
+        w[1] = s.y;
+    }
+}
+
+// alpha = rho / d, x1 = x + alpha p, r1 = r - alpha q; block partials of |r1|^2
+__global__ __launch_bounds__(kBlock) void cgf_update_kernel(const float2 *__restrict__ x, const float2 *__restrict__ r,
+                                                            const float2 *__restrict__ p, const float2 *__restrict__ q,
+                                                            const float *__restrict__ rho, float2 *__restrict__ x1,
+                                                            float2 *__restrict__ r1, float2 *__restrict__ ad,
+                                                            int64_t n, float *__restrict__ work, int nblk) {
+    const int64_t bsys = blockIdx.y;
+    float *wrow = work + bsys * nblk * kCgStride;
+    const float2 d = total_of(wrow, nblk, kCgStride, 2);
+    const float2 alpha = cdiv(make_float2(rho[bsys], 0.0f), d);
+    const int64_t off = bsys * n;
+    const int64_t i0 = (int64_t)blockIdx.x * kSenseEpb + threadIdx.x;
+    float a = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kSenseEpl; ++k) {
+        const int64_t i = i0 + k * kBlock;
+        if (i < n) {
+            const float2 ax = cmul(alpha, p[off + i]);
+            const float2 aq = cmul(alpha, q[off + i]);
+            const float2 xo = x[off + i], ro = r[off + i];
+            x1[off + i] = make_float2(xo.x + ax.x, xo.y + ax.y);
+            const float2 rn = make_float2(ro.x - aq.x, ro.y - aq.y);
+            r1[off + i] = rn;
+            a += rn.x * rn.x + rn.y * rn.y;
+        }
+    }
+    const float2 s = block_sum2(a, 0.0f);
+    if (threadIdx.x == 0) wrow[(int64_t)blockIdx.x * kCgStride + 2] = s.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ad[2 * bsys] = alpha;
+        ad[2 * bsys + 1] = d;
+    }
+}
+
+// rho1 = |r1|^2, p1 = r1 + (rho1 / rho) p
+__global__ __launch_bounds__(kBlock) void cgf_direction_kernel(const float2 *__restrict__ r1,
+                                                               const float2 *__restrict__ p,
+                                                               const float *__restrict__ rho,
+                                                               float2 *__restrict__ p1, float *__restrict__ rho1,
+                                                               int64_t n, const float *__restrict__ work, int nblk) {
+    const int64_t bsys = blockIdx.y;
+    const float rs = total_of(work + bsys * nblk * kCgStride + 2, nblk, kCgStride, 1).x;
+    const float beta = rs / rho[bsys];
+    const int64_t off = bsys * n;
+    const int64_t i0 = (int64_t)blockIdx.x * kSenseEpb + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kSenseEpl; ++k) {
+        const int64_t i = i0 + k * kBlock;
+        if (i < n) {
+            const float2 rv = r1[off + i], pv = p[off + i];
+            p1[off + i] = make_float2(rv.x + beta * pv.x, rv.y + beta * pv.y);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) rho1[bsys] = rs;
+}
+
+// Backward, pass 1: block partials of
+//   S1 = Re sum gp1 conj(p), S2 = sum (gr1 + gp1) conj(q), S3 = sum r1 conj(q),
+//   S4 = sum gx1 conj(p)   (missing output gradients are zeros)
+__global__ __launch_bounds__(kBlock) void cgb_sums_kernel(const float2 *__restrict__ p, const float2 *__restrict__ q,
+                                                          const float2 *__restrict__ r1,
+                                                          const float2 *__restrict__ gx1,
+                                                          const float2 *__restrict__ gr1,
+                                                          const float2 *__restrict__ gp1, int64_t n,
+                                                          float *__restrict__ work, int nblk) {
+    const int64_t off = (int64_t)blockIdx.y * n;
+    const int64_t i0 = (int64_t)blockIdx.x * kSenseEpb + threadIdx.x;
+    float s1 = 0.0f, s2r = 0.0f, s2i = 0.0f, s3r = 0.0f, s3i = 0.0f, s4r = 0.0f, s4i = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kSenseEpl; ++k) {
+        const int64_t i = i0 + k * kBlock;
+        if (i < n) {
+            const float2 pv = p[off + i], qv = q[off + i], rv = r1[off + i];
+            const float2 gp = gp1 ? gp1[off + i] : make_float2(0.0f, 0.0f);
+            const float2 gr = gr1 ? gr1[off + i] : make_float2(0.0f, 0.0f);
+            const float2 gx = gx1 ? gx1[off + i] : make_float2(0.0f, 0.0f);
+            s1 += gp.x * pv.x + gp.y * pv.y;
+            const float2 g = make_float2(gr.x + gp.x, gr.y + gp.y);
+            const float2 t2 = cmulc(qv, g);  // g conj(q)
+            s2r += t2.x;
+            s2i += t2.y;
+            const float2 t3 = cmulc(qv, rv);
+            s3r += t3.x;
+            s3i += t3.y;
+            const float2 t4 = cmulc(pv, gx);
+            s4r += t4.x;
+            s4i += t4.y;
+        }
+    }
+    float *w = work + ((int64_t)blockIdx.y * nblk + blockIdx.x) * kCgStride;
+    float2 t = block_sum2(s1, s2r);
+    if (threadIdx.x == 0) {
+        w[0] = t.x;
+        w[1] = t.y;
+    }
+    __syncthreads();  // block_sum2's LDS is reused
+    t = block_sum2(s2i, s3r);
+    if (threadIdx.x == 0) {
+        w[2] = t.x;
+        w[3] = t.y;
+    }
+    __syncthreads();
+    t = block_sum2(s3i, s4r);
+    if (threadIdx.x == 0) {
+        w[4] = t.x;
+        w[5] = t.y;
+    }
+    __syncthreads();
+    t = block_sum2(s4i, 0.0f);
+    if (threadIdx.x == 0) w[6] = t.x;
+}
+
+// Backward, pass 2: the scalar gradients from the totals, then
+//   gr = gr1 + gp1 + 2 g_rho1 r1,  gq = -gr conj(alpha) + g_d p,
+//   gp = beta gp1 + conj(alpha) gx1 + conj(g_d) q,  grho (real)
+// with g_rho1 = grho1 + S1 / rho, g_alpha = S4 - (S2 + 2 g_rho1 S3),
+// g_d = g_alpha conj(-rho / d^2), grho = -S1 rho1 / rho^2 + Re(g_alpha conj(1 / d))
+// (torch's conjugate-Wirtinger convention for complex gradients).
+__global__ __launch_bounds__(kBlock) void cgb_apply_kernel(const float2 *__restrict__ p, const float2 *__restrict__ q,
+                                                           const float2 *__restrict__ r1,
+                                                           const float2 *__restrict__ gx1,
+                                                           const float2 *__restrict__ gr1,
+                                                           const float2 *__restrict__ gp1,
+                                                           const float *__restrict__ rho,
+                                                           const float *__restrict__ rho1,
+                                                           const float2 *__restrict__ ad,
+                                                           const float *__restrict__ grho1,
+                                                           float2 *__restrict__ gr, float2 *__restrict__ gp,
+                                                           float2 *__restrict__ gq, float *__restrict__ grho, int64_t n,
+                                                           const float *__restrict__ work, int nblk) {
+    const int64_t bsys = blockIdx.y;
+    const float *wrow = work + bsys * nblk * kCgStride;
+    const float2 t01 = total_of(wrow, nblk, kCgStride, 2);
+    const float2 t23 = total_of(wrow + 2, nblk, kCgStride, 2);
+    const float2 t45 = total_of(wrow + 4, nblk, kCgStride, 2);
+    const float t6 = total_of(wrow + 6, nblk, kCgStride, 1).x;
+    const float S1 = t01.x;
+    const float2 S2 = make_float2(t01.y, t23.x), S3 = make_float2(t23.y, t45.x), S4 = make_float2(t45.y, t6);
+    const float rh = rho[bsys], rh1 = rho1[bsys];
+    const float2 alpha = ad[2 * bsys], d = ad[2 * bsys + 1];
+    const float beta = rh1 / rh;
+    const float g_rho1 = (grho1 ? grho1[bsys] : 0.0f) + S1 / rh;
+    const float2 g_alpha = make_float2(S4.x - (S2.x + 2.0f * g_rho1 * S3.x), S4.y - (S2.y + 2.0f * g_rho1 * S3.y));
+    const float2 d2 = cmul(d, d);
+    const float2 da = cdiv(make_float2(-rh, 0.0f), d2);               // d alpha / d d
+    const float2 g_d = make_float2(g_alpha.x * da.x + g_alpha.y * da.y, g_alpha.y * da.x - g_alpha.x * da.y);  // g_alpha conj(da)
+    const float2 inv_d = cdiv(make_float2(1.0f, 0.0f), d);
+    const float g_rho = -S1 * rh1 / (rh * rh) + (g_alpha.x * inv_d.x + g_alpha.y * inv_d.y);
+    const float2 ca = make_float2(alpha.x, -alpha.y);
+    const int64_t off = bsys * n;
+    const int64_t i0 = (int64_t)blockIdx.x * kSenseEpb + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kSenseEpl; ++k) {
+        const int64_t i = i0 + k * kBlock;
+        if (i < n) {
+            const float2 pv = p[off + i], qv = q[off + i], rv = r1[off + i];
+            const float2 gpo = gp1 ? gp1[off + i] : make_float2(0.0f, 0.0f);
+            const float2 gro = gr1 ? gr1[off + i] : make_float2(0.0f, 0.0f);
+            const float2 gxo = gx1 ? gx1[off + i] : make_float2(0.0f, 0.0f);
+            const float2 g = make_float2(gro.x + gpo.x + 2.0f * g_rho1 * rv.x, gro.y + gpo.y + 2.0f * g_rho1 * rv.y);
+            gr[off + i] = g;
+            const float2 a1 = cmul(g, ca), a2 = cmul(g_d, pv);
+            gq[off + i] = make_float2(a2.x - a1.x, a2.y - a1.y);
+            const float2 b2 = cmul(gxo, ca), b3 = cmulc(g_d, qv);  // conj(g_d) q
+            gp[off + i] = make_float2(beta * gpo.x + b2.x + b3.x, beta * gpo.y + b2.y + b3.y);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) grho[bsys] = g_rho;
+}
+
 // Coil combination of SenseAdjointOp (src/deepmr/linops/coil.py:101-205):
 // x[b, k, p] = sum_s sum_c conj(S[s, b, c, p]) * y[s, b, c, k, p], coils
 // summed first, then sets (deepmr's .sum(coil).sum(0) order). One thread per
@@ -301,6 +506,57 @@ int tn_cg_step(const tn_sense_job *jobs, int64_t njobs, int64_t npix, float *wor
     hipLaunchKernelGGL(cg_update_kernel, grid, dim3(kBlock), 0, s, jobs, npix, work, nblk);
     TN_HIP(hipGetLastError());
     hipLaunchKernelGGL(cg_direction_kernel, grid, dim3(kBlock), 0, s, jobs, npix, (const float *)work, nblk);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+int64_t tn_cg_grad_work_floats(int64_t nsys, int64_t n) {
+    if (nsys < 1 || n < 1) return 0;
+    return nsys * sense_blocks(n) * kCgStride;
+}
+
+int tn_cg_iterate(const void *x, const void *r, const void *p, const void *q, const float *rho, void *x1, void *r1,
+                  void *p1, float *rho1, void *ad, float *work, int64_t nsys, int64_t n, tn_stream stream) {
+    clear_error();
+    if (nsys < 1 || nsys > 65535 || n < 1 || n > ((int64_t)1 << 30))
+        return fail(TN_E_ARG, "CG sizes out of range (nsys %lld, n %lld)", (long long)nsys, (long long)n);
+    if (!x || !r || !p || !q || !rho || !x1 || !r1 || !p1 || !rho1 || !ad || !work)
+        return fail(TN_E_ARG, "NULL data pointer");
+    const int nblk = sense_blocks(n);
+    const dim3 grid((unsigned)nblk, (unsigned)nsys);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(cgf_dot_kernel, grid, dim3(kBlock), 0, s, (const float2 *)p, (const float2 *)q, n, work, nblk);
+    TN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(cgf_update_kernel, grid, dim3(kBlock), 0, s, (const float2 *)x, (const float2 *)r,
+                       (const float2 *)p, (const float2 *)q, rho, (float2 *)x1, (float2 *)r1, (float2 *)ad, n, work,
+                       nblk);
+    TN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(cgf_direction_kernel, grid, dim3(kBlock), 0, s, (const float2 *)r1, (const float2 *)p, rho,
+                       (float2 *)p1, rho1, n, (const float *)work, nblk);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+int tn_cg_iterate_backward(const void *p, const void *q, const void *r1, const float *rho, const float *rho1,
+                           const void *ad, const void *gx1, const void *gr1, const void *gp1, const float *grho1,
+                           void *gr, void *gp, void *gq, float *grho, float *work, int64_t nsys, int64_t n,
+                           tn_stream stream) {
+    clear_error();
+    if (nsys < 1 || nsys > 65535 || n < 1 || n > ((int64_t)1 << 30))
+        return fail(TN_E_ARG, "CG sizes out of range (nsys %lld, n %lld)", (long long)nsys, (long long)n);
+    if (!p || !q || !r1 || !rho || !rho1 || !ad || !gr || !gp || !gq || !grho || !work)
+        return fail(TN_E_ARG, "NULL data pointer");
+    const int nblk = sense_blocks(n);
+    const dim3 grid((unsigned)nblk, (unsigned)nsys);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(cgb_sums_kernel, grid, dim3(kBlock), 0, s, (const float2 *)p, (const float2 *)q,
+                       (const float2 *)r1, (const float2 *)gx1, (const float2 *)gr1, (const float2 *)gp1, n, work,
+                       nblk);
+    TN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(cgb_apply_kernel, grid, dim3(kBlock), 0, s, (const float2 *)p, (const float2 *)q,
+                       (const float2 *)r1, (const float2 *)gx1, (const float2 *)gr1, (const float2 *)gp1, rho, rho1,
+                       (const float2 *)ad, grho1, (float2 *)gr, (float2 *)gp, (float2 *)gq, grho, n,
+                       (const float *)work, nblk);
     TN_HIP(hipGetLastError());
     return TN_OK;
 }

@@ -49,6 +49,9 @@ _SIGNATURES = {
     "tn_plan_footprint_cells": (c_int64, [c_void_p]),
     "tn_plan_tile": (c_int32, [c_void_p, ctypes.POINTER(c_int32)]),
     "tn_plan_grid_mode": (c_int32, [c_void_p]),
+    "tn_cg_grad_work_floats": (c_int64, [c_int64, c_int64]),
+    "tn_cg_iterate": (c_int32, [c_void_p] * 11 + [c_int64, c_int64, c_void_p]),
+    "tn_cg_iterate_backward": (c_int32, [c_void_p] * 15 + [c_int64, c_int64, c_void_p]),
     "tn_coil_combine": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64,
                                   c_void_p]),
     "tn_plan_prepare": (c_int32, [c_void_p, c_int64]),

@@ -6,11 +6,60 @@
 
 __all__ = ["cg_solve", "CGStep"]
 
+import math
+
 import numpy as np
 import torch
 import torch.nn as nn
 
+from .. import _core, _lib
 from .. import linops as _linops
+
+
+class _CGIterate(torch.autograd.Function):
+    """One CG iteration's vector arithmetic on (nsys, n) complex64 device
+    arrays given q = A p (tn_cg_iterate / tn_cg_iterate_backward): returns
+    (x1, r1, p1, rho1); the backward returns the gradients of (x, r, p, q,
+    rho) in torch's convention, so autograd carries q's gradient through A."""
+
+    @staticmethod
+    def forward(ctx, x, r, p, q, rho):
+        rho = rho.to(torch.float32).contiguous()  # torch.real() of a complex tensor is a strided view
+        nsys, n = r.shape
+        dev = r.device
+        x1, r1, p1 = torch.empty_like(r), torch.empty_like(r), torch.empty_like(r)
+        rho1 = torch.empty((nsys,), dtype=torch.float32, device=dev)
+        ad = torch.empty((nsys, 2), dtype=torch.complex64, device=dev)
+        lib = _lib.load()
+        work = torch.empty((int(lib.tn_cg_grad_work_floats(nsys, n)),), dtype=torch.float32, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(lib.tn_cg_iterate(_core.ptr(x), _core.ptr(r), _core.ptr(p), _core.ptr(q), _core.ptr(rho),
+                                         _core.ptr(x1), _core.ptr(r1), _core.ptr(p1), _core.ptr(rho1),
+                                         _core.ptr(ad), _core.ptr(work), nsys, n, _core.stream_ptr(dev)),
+                       "tn_cg_iterate")
+        ctx.save_for_backward(p, q, r1, rho, rho1, ad)
+        ctx.set_materialize_grads(False)
+        return x1, r1, p1, rho1
+
+    @staticmethod
+    def backward(ctx, gx1, gr1, gp1, grho1):
+        p, q, r1, rho, rho1, ad = ctx.saved_tensors
+        nsys, n = p.shape
+        dev = p.device
+        c = lambda g: None if g is None else g.to(torch.complex64).resolve_conj().contiguous()  # noqa: E731
+        gx1, gr1, gp1 = c(gx1), c(gr1), c(gp1)
+        grho1 = None if grho1 is None else grho1.to(torch.float32).contiguous()
+        gr, gp, gq = torch.empty_like(p), torch.empty_like(p), torch.empty_like(p)
+        grho = torch.empty((nsys,), dtype=torch.float32, device=dev)
+        lib = _lib.load()
+        work = torch.empty((int(lib.tn_cg_grad_work_floats(nsys, n)),), dtype=torch.float32, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(lib.tn_cg_iterate_backward(_core.ptr(p), _core.ptr(q), _core.ptr(r1), _core.ptr(rho),
+                                                  _core.ptr(rho1), _core.ptr(ad), _core.ptr(gx1), _core.ptr(gr1),
+                                                  _core.ptr(gp1), _core.ptr(grho1), _core.ptr(gr), _core.ptr(gp),
+                                                  _core.ptr(gq), _core.ptr(grho), _core.ptr(work), nsys, n,
+                                                  _core.stream_ptr(dev)), "tn_cg_iterate_backward")
+        return gx1, gr, gp, gq, grho
 
 
 class CGStep(nn.Module):
@@ -31,6 +80,15 @@ class CGStep(nn.Module):
         self.rsold = self.dot(self.r, self.r)
         self.rsnew = None
         self.tol = tol
+        # complex64 systems on a HIP device take the native iteration; rs is
+        # then kept real (<r, r> has an exactly zero imaginary part, so its
+        # gradient is the same)
+        self._native = (isinstance(AHy, torch.Tensor) and AHy.device.type == "cuda"
+                        and AHy.dtype == torch.complex64 and self.ndim is not None and AHy.dim() >= self.ndim)
+        if self._native:
+            self._lead = tuple(AHy.shape[:AHy.dim() - self.ndim])
+            self._nsys = max(1, math.prod(self._lead))
+            self._rho = torch.real(self.rsold).reshape(self._nsys).to(torch.float32).contiguous()
 
     def dot(self, s1, s2):
         # sum(conj(s1) * s2) over the last ndim axes, as one reduction
@@ -39,13 +97,29 @@ class CGStep(nn.Module):
         return d.reshape(*d.shape, *([1] * n))
 
     def forward(self, x):
-        AHAp = self.AHA(self.p)
+        if self._native and isinstance(x, torch.Tensor) and x.shape == self.r.shape and x.dtype == torch.complex64:
+            AHAp = self.AHA(self.p)
+            if AHAp.shape == self.p.shape and AHAp.dtype == torch.complex64:
+                shape, ns = self.p.shape, self._nsys
+                flat = lambda t: t.reshape(ns, -1).resolve_conj().contiguous()  # noqa: E731
+                x1, r1, p1, rho1 = _CGIterate.apply(flat(x), flat(self.r), flat(self.p), flat(AHAp), self._rho)
+                self.r, self.p = r1.reshape(shape), p1.reshape(shape)
+                self._rho = rho1
+                self.rsnew = rho1.reshape(*self._lead, *([1] * self.ndim))
+                self.rsold = self.rsnew
+                return x1.reshape(shape)
+            return self._torch_step(x, AHAp)
+        return self._torch_step(x, self.AHA(self.p))
+
+    def _torch_step(self, x, AHAp):
         alpha = self.rsold / self.dot(self.p, AHAp)
         out = x + self.p * alpha
         self.r = self.r + AHAp * (-alpha)
         self.rsnew = torch.real(self.dot(self.r, self.r))
         self.p = self.r + self.p * (self.rsnew / self.rsold)
         self.rsold = self.rsnew
+        if self._native:
+            self._rho = self.rsnew.reshape(self._nsys).to(torch.float32).contiguous()
         return out
 
     def check_convergence(self):

@@ -256,6 +256,27 @@ int tn_sense_expand(const tn_sense_job *jobs, int64_t njobs, int64_t npix, void 
 int tn_sense_combine(const tn_sense_job *jobs, int64_t njobs, int64_t npix, const void *img,
                      tn_stream stream);
 
+/* ---- one differentiable CG iteration, for autograd callers ----------------
+ * nsys independent systems of n complex64 elements, arrays (nsys, n)
+ * contiguous in device memory; rho, rho1 (nsys) float32; ad (nsys, 2)
+ * complex64. Replaces CGStep.forward (src/deepmr/optim/cg.py:203-212)
+ * given q = A p (A Hermitian):
+ *   d = <p, q>, alpha = rho / d, x1 = x + alpha p, r1 = r - alpha q,
+ *   rho1 = <r1, r1>, p1 = r1 + (rho1 / rho) p,   ad = (alpha, d)
+ * Three launches; `work` holds tn_cg_grad_work_floats(nsys, n) floats.
+ * Every sum has a fixed order that depends only on n. */
+int64_t tn_cg_grad_work_floats(int64_t nsys, int64_t n);
+int tn_cg_iterate(const void *x, const void *r, const void *p, const void *q, const float *rho, void *x1, void *r1,
+                  void *p1, float *rho1, void *ad, float *work, int64_t nsys, int64_t n, tn_stream stream);
+/* Its vector-Jacobian product in torch's convention for complex gradients
+ * (conjugate Wirtinger): from the gradients of x1, r1, p1 (each may be NULL
+ * for zero) and rho1 (may be NULL), the gradients of r, p, q and rho (the
+ * gradient of x is that of x1). Two launches, same `work` size. */
+int tn_cg_iterate_backward(const void *p, const void *q, const void *r1, const float *rho, const float *rho1,
+                           const void *ad, const void *gx1, const void *gr1, const void *gp1, const float *grho1,
+                           void *gr, void *gp, void *gq, float *grho, float *work, int64_t nsys, int64_t n,
+                           tn_stream stream);
+
 /* Coil combination of SenseAdjointOp (src/deepmr/linops/coil.py:101-205) on
  * contiguous complex64 arrays: smaps (nsets, nb, ncoils, npix), y (nsets, nb,
  * ncoils, nk, npix) -> x (nb, nk, npix) with x[b, k, p] = sum over sets of
