@@ -453,7 +453,7 @@ def cgsense_bench(steps=10, warmup=3):
            "ms_per_step_eager": t_eager, "ms_per_step_graphed": t_graph,
            "ms_per_iteration_fwd_bwd_graphed": t_graph / niter, "plan_s": plan_s, "steps": steps,
            "warmup": warmup, "graphed_vs_eager_rel_l2": {"image": rel(x_g, x_ref), "grad": rel(g_g, g_ref)}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -468,6 +468,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--no-config1", action="store_true", help="skip the plain single-coil 2D leg")
     ap.add_argument("--no-kooshball", action="store_true", help="skip the 3D kooshball kernel leg")
+    ap.add_argument("--no-cgsense", action="store_true", help="skip the unrolled CG-SENSE (autograd) leg")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only replay the step and time the kernels (short runs under a profiler)")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -507,7 +508,7 @@ def main():
         print(json.dumps(res), flush=True)
         return
     if args.cgsense:
-        cgsense_bench()
+        print(json.dumps(cgsense_bench()), flush=True)
         return
     if args.tune_cache:
         os.environ["TILENUFFT_CACHE"] = args.tune_cache
@@ -615,6 +616,11 @@ def main():
     if not args.no_kooshball and not args.kernels_only:
         kooshball = kooshball_bench(dev)
 
+    cgsense = None
+    if not args.no_cgsense and not args.kernels_only:
+        log("bench: unrolled CG-SENSE with autograd (256^2, 8 coils), eager and graphed")
+        cgsense = cgsense_bench()
+
     serving = None
     if not args.no_serve and not args.kernels_only:
         log(f"bench: serving stream ({args.serve_requests} requests) served vs plain")
@@ -673,6 +679,7 @@ def main():
         "parity_rel_l2_frame0_vs_oracle": parity,
         "config1_plain_2d": config1,
         "kooshball_3d": kooshball,
+        "cgsense_2d": cgsense,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
