@@ -61,8 +61,8 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.mrow = plan->d_mrow ? plan->d_mrow + (int64_t)f * gvol : nullptr;
         j.mnz = plan->d_mnz;
         j.mlong = p.grid_split == 0  ? (p.ndim == 3 ? kMatrixLongRow3d : kMatrixLongRow)
-                  : p.grid_split < 0 ? kMatrixStage
-                                     : std::min(p.grid_split, kMatrixStage);
+                  : p.grid_split < 0 ? matrix_stage(p.ndim)
+                                     : std::min(p.grid_split, matrix_stage(p.ndim));
         // deepmr layout (batch, nframes, ...): frame f of coil b
         j.grid_off = grid_off + (int64_t)f * gvol;
         j.grid_bstride = (int64_t)p.nframes * gvol;
@@ -155,7 +155,8 @@ static const std::vector<MatrixRun> &matrix_runs(const tn_plan *pl, int64_t f, i
     auto &per_frame = pl->mruns[cpi];
     if (per_frame.empty()) per_frame.resize(pl->p.nframes);
     std::vector<MatrixRun> &out = per_frame[f];
-    if (out.empty() && ncell > 0) matrix_runs_of(pl->h_mrow.data() + f * ncell, ncell, cpi, mlong, out);
+    if (out.empty() && ncell > 0)
+        matrix_runs_of(pl->h_mrow.data() + f * ncell, ncell, cpi, mlong, out, matrix_stage(pl->p.ndim));
     return out;
 }
 

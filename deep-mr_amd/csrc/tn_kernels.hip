@@ -1006,6 +1006,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_kernel(const Job *__restri
     if ((int)threadIdx.x < ncoil) {
         const int c = threadIdx.x;
         float re = red[0][2 * c], im = red[0][2 * c + 1];
+#pragma unroll 1
         for (int k = 1; k < kLongGroups; ++k) {
             re += red[k][2 * c];
             im += red[k][2 * c + 1];
@@ -1064,13 +1065,13 @@ __global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__res
 // coils instead of one line per coil; with CPL = 2 each lane reads 16 bytes).
 // Items as for grid_matrix_kernel (it.e0 = chunk index). Part 0, a run of
 // (it.e1 & 0xffff) cells from it.unit whose entries (it.e1 >> 16 of them,
-// at most kMatrixStage, contiguous in the CSR order from it.slot) and row
+// at most STAGE, contiguous in the CSR order from it.slot) and row
 // offsets are first staged in LDS with independent coalesced loads, so that
 // the only dependent global loads left are the sample records; kBlock / CW
 // cells per pass, each row U entries at a time. Part 1 strides one long
 // row's entries over kLongGroups cell groups and adds the group sums in
 // group order.
-template <int LCW, int CPL, int U>
+template <int LCW, int CPL, int U, int STAGE>
 __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,
                                                                const float2 *__restrict__ st,
@@ -1085,7 +1086,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     // and stored as contiguous runs (row pitch CPI + 1 against bank conflicts)
     constexpr int CPI = CPP * kMatrixPasses < kBlock ? CPP * kMatrixPasses : kBlock;
     __shared__ float2 sout[RC * (CPI + 1)];
-    __shared__ i32x2 sent[kMatrixStage];
+    __shared__ i32x2 sent[STAGE];
     __shared__ int srow[kBlock + 1];
     const int64_t idx = xcd_item(nitems);
     if (idx < 0) return;
@@ -1133,7 +1134,7 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
         const int64_t cell0 = it.unit;
         const int ncells = min(it.e1 & 0xffff, CPI);
         const int rb = it.slot;
-        const int nent = min(it.e1 >> 16, kMatrixStage);  // <= kMatrixStage by construction
+        const int nent = min(it.e1 >> 16, STAGE);  // <= STAGE by construction (matrix_stage)
         for (int i = threadIdx.x; i <= ncells; i += kBlock) srow[i] = min(max(J.mrow[cell0 + i] - rb, 0), nent);
         for (int i0 = threadIdx.x; i0 < nent; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
             i32x2 m[kLoadBatch];
@@ -1185,7 +1186,8 @@ __global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__rest
     const int c = threadIdx.x;  // coil of the record
     if (c < RC && it.c0 + c < it.c1) {
         float2 s = sout[c];
-        for (int k = 1; k < kLongGroups; ++k) {
+#pragma unroll 1
+        for (int k = 1; k < kLongGroups; ++k) {  // one read at a time: a full unroll holds 62 registers
             s.x += sout[k * RC + c].x;
             s.y += sout[k * RC + c].y;
         }
@@ -1468,26 +1470,27 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
         TrFn tr = nullptr;
         MtFn mt = nullptr;
         const bool pair = t.st_cpl == 2;  // two coils per lane (the table's cells per item assume it)
-        // entries in flight per lane: 4 in 2D, 8 in 3D (each measured fastest)
+        // entries in flight per lane: 4 in 2D, 8 in 3D; staged entries per
+        // item: matrix_stage (each measured fastest)
         const bool d3 = kind_ndim(t.kind_grid) == 3;
         switch (t.st_cw) {
             case 2:
                 tr = samp_transpose_kernel<1>;
-                mt = d3 ? grid_matrix_t_kernel<1, 1, 8> : grid_matrix_t_kernel<1, 1, 4>;
+                mt = d3 ? grid_matrix_t_kernel<1, 1, 8, kMatrixStage> : grid_matrix_t_kernel<1, 1, 4, kMatrixStage2d>;
                 break;
             case 4:
                 tr = samp_transpose_kernel<2>;
-                mt = pair ? (d3 ? grid_matrix_t_kernel<1, 2, 8> : grid_matrix_t_kernel<1, 2, 4>)
-                          : (d3 ? grid_matrix_t_kernel<2, 1, 8> : grid_matrix_t_kernel<2, 1, 4>);
+                mt = pair ? (d3 ? grid_matrix_t_kernel<1, 2, 8, kMatrixStage> : grid_matrix_t_kernel<1, 2, 4, kMatrixStage2d>)
+                          : (d3 ? grid_matrix_t_kernel<2, 1, 8, kMatrixStage> : grid_matrix_t_kernel<2, 1, 4, kMatrixStage2d>);
                 break;
             case 8:
                 tr = samp_transpose_kernel<3>;
-                mt = pair ? (d3 ? grid_matrix_t_kernel<2, 2, 8> : grid_matrix_t_kernel<2, 2, 4>)
-                          : (d3 ? grid_matrix_t_kernel<3, 1, 8> : grid_matrix_t_kernel<3, 1, 4>);
+                mt = pair ? (d3 ? grid_matrix_t_kernel<2, 2, 8, kMatrixStage> : grid_matrix_t_kernel<2, 2, 4, kMatrixStage2d>)
+                          : (d3 ? grid_matrix_t_kernel<3, 1, 8, kMatrixStage> : grid_matrix_t_kernel<3, 1, 4, kMatrixStage2d>);
                 break;
             case 16:
                 tr = samp_transpose_kernel<4>;  // 8 lanes x 2 coils
-                mt = d3 ? grid_matrix_t_kernel<3, 2, 8> : grid_matrix_t_kernel<3, 2, 4>;
+                mt = d3 ? grid_matrix_t_kernel<3, 2, 8, kMatrixStage> : grid_matrix_t_kernel<3, 2, 4, kMatrixStage2d>;
                 break;
             default: return fail(TN_E_ARG, "matrix gridding: %d coils per record", t.st_cw);
         }

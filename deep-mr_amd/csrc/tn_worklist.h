@@ -10,9 +10,14 @@
 namespace tn {
 
 constexpr int kMatrixCoils = 16;   // widest record of the matrix gridding (coils per item)
-constexpr int kMatrixStage = 1024; // entries of a run of cells staged in LDS (also the longest short row)
+constexpr int kMatrixStage = 1024; // entries of a run of cells staged in LDS (also the longest short row), 3D
+constexpr int kMatrixStage2d = 512; // the same in 2D: a smaller stage leaves LDS for more workgroups per CU
+                                    // (spiral 8x16: 0.175 against 0.181 ms; in 3D 512 measured slower)
 constexpr int kMatrixPart = 2048;  // entries per workgroup of a long row; longer rows are summed
                                    // in parts (kMatrixCoils-wide partial records, folded in order)
+
+// LDS-staged entries per matrix-gridding item for plans of `ndim` dimensions.
+inline int matrix_stage(int ndim) { return ndim == 3 ? kMatrixStage : kMatrixStage2d; }
 
 // Work item. Gathering interpolation: unit = sample block. Staged
 // interpolation: unit = tile, [e0, e1) = slice of its entries. Gridding:
@@ -33,11 +38,12 @@ struct MatrixRun {
 };
 
 // Runs of one frame from its CSR row offsets row[0 .. ncell]: consecutive
-// cells with at most cpi cells and at most kMatrixStage entries per run, so
-// that a run's entries (contiguous in the CSR order) fit the kernel's LDS
-// stage; a row longer than mlong (<= kMatrixStage) ends the run and becomes
-// a long-row run of ceil(len / kMatrixPart) parts. Long rows are listed first.
-inline void matrix_runs_of(const int32_t *row, int64_t ncell, int cpi, int mlong, std::vector<MatrixRun> &out) {
+// cells with at most cpi cells and at most `stage` entries per run, so that
+// a run's entries (contiguous in the CSR order) fit the kernel's LDS stage;
+// a row longer than mlong (<= stage) ends the run and becomes a long-row run
+// of ceil(len / kMatrixPart) parts. Long rows are listed first.
+inline void matrix_runs_of(const int32_t *row, int64_t ncell, int cpi, int mlong, std::vector<MatrixRun> &out,
+                           int stage = kMatrixStage) {
     std::vector<MatrixRun> shorts;
     int64_t start = 0;
     int ncells = 0, nent = 0;
@@ -53,7 +59,7 @@ inline void matrix_runs_of(const int32_t *row, int64_t ncell, int cpi, int mlong
             out.push_back(MatrixRun{(int32_t)c, -((len + kMatrixPart - 1) / kMatrixPart), row[c], len});
             continue;
         }
-        if (ncells == cpi || nent + len > kMatrixStage) close();
+        if (ncells == cpi || nent + len > stage) close();
         if (ncells == 0) start = c;
         ++ncells;
         nent += len;

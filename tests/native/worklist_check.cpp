@@ -24,7 +24,8 @@ int main() {
     for (int cs = 0; cs < 400; ++cs) {
         const int64_t ncell = cs % 50 == 0 ? 0 : 1 + rng() % 5000;
         const int cpi = cpis[rng() % 5];
-        const int mlong = 16 + rng() % (tn::kMatrixStage - 15);
+        const int stage = cs % 2 ? tn::kMatrixStage : tn::kMatrixStage2d;
+        const int mlong = 16 + rng() % (stage - 15);
         const int nbatch = 1 + rng() % 40;
         const int mc = 1 + rng() % tn::kMatrixCoils;
         // row lengths: mostly empty or short, a few long (a dense centre);
@@ -37,7 +38,7 @@ int main() {
             row[c + 1] = row[c] + len;
         }
         std::vector<tn::MatrixRun> runs;
-        tn::matrix_runs_of(row.data(), ncell, cpi, mlong, runs);
+        tn::matrix_runs_of(row.data(), ncell, cpi, mlong, runs, stage);
         std::vector<int> seen(ncell, 0);
         bool in_shorts = false;
         for (const tn::MatrixRun &r : runs) {
@@ -55,7 +56,7 @@ int main() {
             in_shorts = true;
             if (r.cell < 0 || r.cell + r.ncells > ncell) return fail("run past the grid", cs, r.cell);
             if (r.ncells > cpi) return fail("run longer than cells per item", cs, r.cell);
-            if (r.ne > tn::kMatrixStage || r.ne < 0) return fail("run entries exceed the LDS stage", cs, r.cell);
+            if (r.ne > stage || r.ne < 0) return fail("run entries exceed the LDS stage", cs, r.cell);
             if (r.e0 != row[r.cell] || r.ne != row[r.cell + r.ncells] - row[r.cell]) return fail("run entries", cs, r.cell);
             for (int i = 0; i < r.ncells; ++i) {
                 if (row[r.cell + i + 1] - row[r.cell + i] > mlong) return fail("long row inside a run", cs, r.cell + i);
