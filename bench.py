@@ -474,6 +474,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--tune-cache", default=None, help="directory of the autotuner's cache file")
+    ap.add_argument("--grid-split", type=int, default=None,
+                    help="after tuning, force this gridding split (for the matrix: the long-row threshold), "
+                         "to profile each variant the tuner may pick")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after measuring, run 3 plain device copies of this many MiB (a known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -493,6 +496,11 @@ def main():
         if not args.bench_line or args.calibrate_copy <= 0:
             raise SystemExit("--fold-counters needs --bench-line and --calibrate-copy")
         res = fold_counters(args.fold_counters, args.bench_line, args.calibrate_copy)
+        if args.traffic_out and os.path.exists(args.traffic_out):  # add to the variants already profiled
+            with open(args.traffic_out) as f:
+                old = json.load(f)
+            for key in ("variants", "stages"):
+                res[key] = {**old.get(key, {}), **res[key]}
         text = json.dumps(res, indent=1, sort_keys=True)
         if args.traffic_out:
             with open(args.traffic_out, "w") as f:
@@ -532,6 +540,11 @@ def main():
         tune = tilenufft.autotune.tune_batch(plans, ncoils)
         tune = {k: v for k, v in tune.items() if k != "trials"}
         log(f"bench: autotune -> {tune}")
+    if args.grid_split is not None:
+        from tilenufft.fft.nufft import retune
+
+        for p in plans:
+            retune(p, grid_split=args.grid_split)
     batch = BatchedNUFFT(plans, ncoils)
     for f in range(len(plans)):
         batch.image_view(f)[:, 0].copy_(wl["smaps"] * wl["frames"][f])
