@@ -1070,9 +1070,11 @@ __global__ __launch_bounds__(kBlock) void samp_transpose_kernel(const Job *__res
 // the only dependent global loads left are the sample records; kBlock / CW
 // cells per pass, each row U entries at a time. Part 1 strides one long
 // row's entries over kLongGroups cell groups and adds the group sums in
-// group order.
+// group order. One coil per lane fits 64 VGPRs, so eight waves per SIMD are
+// asked for (kooshball 3D, 8 coils: 3.48 -> 3.35 ms); two coils per lane
+// would spill at that bound.
 template <int LCW, int CPL, int U, int STAGE>
-__global__ __launch_bounds__(kBlock) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
+__global__ __launch_bounds__(kBlock, CPL == 1 ? 8 : 1) void grid_matrix_t_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,
                                                                const float2 *__restrict__ st,
                                                                float2 *__restrict__ grid,
