@@ -359,7 +359,7 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
     """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
     256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3): the
     automatic plan's gridding (the gridding matrix) and staged interpolation, and with
-    ``compare_runs`` the register-run gridding too. Durations from HIP events
+    ``compare_runs`` the register-tile and register-run griddings too. Durations from HIP events
     on the launch stream; algorithmic bytes as for the 2D workload. Returns a
     dict."""
     import tilenufft
@@ -389,6 +389,13 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
         case["interp_frac"] = b_interp / (ti * 1e-3) / 1e9 / HBM_PEAK_GBS
         b = None
         if compare_runs:
+            retune(plan, grid_mode=4)  # register tiles
+            torch.cuda.synchronize(dev)
+            case["tiles_plan_bytes"] = plan.interpolator.device_bytes()
+            b = BatchedNUFFT([plan], 8)
+            b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(dev))
+            case["grid_tiles_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)
+            b = None
             retune(plan, grid_mode=2, tile=(16, 16, 16))
             b = BatchedNUFFT([plan], 8)
             case["grid_runs_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)

@@ -60,6 +60,7 @@ void make_jobs(const tn_plan *plan, int64_t nbatch, int64_t grid_off, int64_t sa
         j.pbin_off = plan->d_pbin_off + (int64_t)f * plan->nitiles_total;
         j.mrow = plan->d_mrow ? plan->d_mrow + (int64_t)f * gvol : nullptr;
         j.mnz = plan->d_mnz;
+        j.taps = plan->d_taps ? plan->d_taps + (int64_t)f * p.npts * 4 : nullptr;
         j.mlong = p.grid_split == 0  ? (p.ndim == 3 ? kMatrixLongRow3d : kMatrixLongRow)
                   : p.grid_split < 0 ? matrix_stage(p.ndim)
                                      : std::min(p.grid_split, matrix_stage(p.ndim));
@@ -179,12 +180,15 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         if (pl->grid_mode != gmode) return fail(TN_E_ARG, "batched plans must share the gridding variant");
     t.grid_mode = gmode;
     const int gndim = kind_ndim(t.kind_grid);
-    if (gmode != kGridMatrix && grid_lds_bytes(1, max_plane, wc, gmode, gndim) > kGridLdsBudget)
+    // LDS sizing of the LDS gridding kernels (register tiles: of the LDS-atomic
+    // kernel that serves their low-rank launches)
+    const int lmode = gmode == kGridTiles ? kGridSpread : gmode;
+    if (gmode != kGridMatrix && grid_lds_bytes(1, max_plane, wc, lmode, gndim) > kGridLdsBudget)
         return fail(TN_E_ARG, "gridding tile too large for LDS (%zu bytes for one coil)",
-                    grid_lds_bytes(1, max_plane, wc, gmode, gndim));
+                    grid_lds_bytes(1, max_plane, wc, lmode, gndim));
     int lds_coils = 1;
     while (lds_coils < (gmode == kGridRuns ? kRunsMaxCoils : 64) &&
-           grid_lds_bytes(lds_coils + 1, max_plane, wc, gmode, gndim) <= kGridLdsBudget)
+           grid_lds_bytes(lds_coils + 1, max_plane, wc, lmode, gndim) <= kGridLdsBudget)
         ++lds_coils;
     const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
     const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
@@ -225,8 +229,11 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
         // automatic coil chunk: as many as fit in LDS; for register-run gridding
         // one coil per wave (more coils per item cost occupancy, which hides
         // the latency of its LDS read-add-write chains)
-        const int gc_auto = gmode == kGridRuns ? std::min(lds_coils, kBlock / kWave) : lds_coils;
-        const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, gc_auto), lds_coils);
+        const int gc_auto = gmode == kGridRuns    ? std::min(lds_coils, kBlock / kWave)
+                            : gmode == kGridTiles ? std::min(lds_coils, kTilesCoils)
+                                                  : lds_coils;
+        const int gc = std::min(chunk_for(pl->p.grid_chunk, j.nbatch, gc_auto),
+                                gmode == kGridTiles ? std::min(lds_coils, kTilesCoils) : lds_coils);
         if (mode == kInterpStaged) {
             // one item per (tile, segment) covers every coil; the region is
             // loaded `ic` coils at a time
@@ -285,10 +292,24 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
             continue;
         }
         max_gc = std::max(max_gc, gc);
+        if (gmode == kGridTiles) {
+            // coil-minor sample copy in chunks of gc coils padded to cw
+            // (scalar-loaded records); none for low-rank or gather tables,
+            // whose gridding takes the LDS-atomic kernel over the same bins
+            if (no_copy || force_gather) {
+                st_none = true;
+            } else {
+                st_cw = std::max(st_cw, gc <= 2 ? 2 : gc <= 4 ? 4 : 8);
+                const int64_t pblk = (j.npts + kBlock - 1) / kBlock;
+                for (int c0 = 0; c0 < j.nbatch; c0 += gc)
+                    for (int64_t b = 0; b < pblk; ++b)
+                        tr.push_back(Item{(int)ji, (int)b, c0, std::min(j.nbatch, c0 + gc), c0 / gc, 0, 0, -1});
+            }
+        }
         const int64_t ntiles = (int64_t)j.NT[0] * j.NT[1] * j.NT[2];
         const int64_t fg = (j.bin_off - pl->d_bin_off) / std::max<int64_t>(pl->ntiles_total, 1);
         const int32_t *goff = pl->h_bin_off.data() + fg * pl->ntiles_total;
-        const int split = pl->p.grid_split == 0 ? kGridSplitDefault
+        const int split = pl->p.grid_split == 0 ? (gmode == kGridTiles ? kTilesSplit : kGridSplitDefault)
                           : pl->p.grid_split < 0 ? INT32_MAX : pl->p.grid_split;
         const int64_t cells = (int64_t)j.T[0] * j.T[1] * j.T[2];  // partial size per coil
         for (int64_t tl = 0; tl < ntiles; ++tl) {
@@ -315,11 +336,18 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     t.scratch_floats = scratch;
     t.grid_coils = max_gc;
     std::vector<Job> jobs_st(jobs);
-    if (gmode == kGridMatrix && st_cw > 0 && !st_none) {  // every job gets its copy at st_cw coils per record
+    if ((gmode == kGridMatrix || gmode == kGridTiles) && st_cw > 0 && !st_none) {
+        // every job gets its copy at st_cw coils per record
         int64_t off = 0;
         for (size_t ji = 0; ji < jobs.size(); ++ji) {
-            const int mc = std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch, kMatrixChunk), kMatrixCoils);
+            const int mc = gmode == kGridTiles ? std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch,
+                                                                    std::min(lds_coils, kTilesCoils)),
+                                                          std::min(lds_coils, kTilesCoils))
+                                               : std::min(chunk_for(plan_of[ji]->p.grid_chunk, jobs[ji].nbatch,
+                                                                    kMatrixChunk),
+                                                          kMatrixCoils);
             jobs_st[ji].st_off = off;
+            jobs_st[ji].st_chunk = mc;
             off += (int64_t)((jobs[ji].nbatch + mc - 1) / mc) * jobs[ji].npts * st_cw;
         }
         t.st_cw = st_cw;
@@ -339,7 +367,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     const int64_t copy_bytes = t.d_st ? (int64_t)(sizeof(Item) * std::max<size_t>(tr.size(), 1) +
                                                   sizeof(float2) * std::max<int64_t>(t.st_elems, 1))
                                       : 0;
-    t.grid_lds = gmode == kGridMatrix ? 0 : grid_lds_bytes(max_gc, max_plane, wc, gmode, gndim);
+    t.grid_lds = gmode == kGridMatrix ? 0 : grid_lds_bytes(max_gc, max_plane, wc, lmode, gndim);
     t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
     t.interp_coils = max_ic;
     const int rc = upload_tables(jobs_st, interp, grid, fold, slot_base, t);
@@ -376,7 +404,7 @@ int validate(const tn_plan_params *p) {
     if ((int64_t)p->grid[0] * p->grid[1] * p->grid[2] >= INT32_MAX)
         return fail(TN_E_RANGE, "grid too large");
     if (p->interp_mode < 0 || p->interp_mode > 2) return fail(TN_E_ARG, "interp_mode must be 0, 1 or 2");
-    if (p->grid_mode < 0 || p->grid_mode > 3) return fail(TN_E_ARG, "grid_mode must be 0, 1, 2 or 3");
+    if (p->grid_mode < 0 || p->grid_mode > 4) return fail(TN_E_ARG, "grid_mode must be 0, 1, 2, 3 or 4");
     return TN_OK;
 }
 
@@ -425,6 +453,7 @@ void destroy(tn_plan *pl) {
     if (pl->d_pbin_off) (void)hipFree(pl->d_pbin_off);
     if (pl->d_mrow) (void)hipFree(pl->d_mrow);
     if (pl->d_mnz) (void)hipFree(pl->d_mnz);
+    if (pl->d_taps) (void)hipFree(pl->d_taps);
     for (int a = 0; a < 2; ++a)
         if (pl->d_fft_tw[a]) (void)hipFree(pl->d_fft_tw[a]);
     for (int d = 0; d < 3; ++d)
@@ -515,19 +544,39 @@ int tn_plan_create(const tn_plan_params *params, const float *coord_dev, tn_stre
             return fail(TN_E_ARG, "grid_mode 2 supports at most %d taps per z plane (got %d)", kRunsMaxTaps, taps);
         }
         // automatic: the sparse gridding matrix when it fits kMatrixAutoBytes
-        // (8 bytes per sample and tap), else register runs when the taps of a
-        // z plane fit the registers, else per-sample LDS atomics
+        // (8 bytes per sample and tap), else register tiles (3D), else
+        // register runs when the taps of a z plane fit the registers, else
+        // per-sample LDS atomics
         const int64_t nnz = (int64_t)params->nframes * params->npts * params->width[0] * params->width[1] *
                             params->width[2];
         const int64_t nrows = (int64_t)params->nframes * params->grid[0] * params->grid[1] * params->grid[2];
         const bool matrix_ok = nnz < INT32_MAX && (uint64_t)nnz * sizeof(int2) <= kMatrixAutoBytes &&
                                nrows < (int64_t)UINT32_MAX && params->npts <= ((int64_t)1 << 28);
+        // then register tiles where they fit (3D)
+        const bool tiles_ok = tiles_fit(*params);
+        if (params->grid_mode == kGridTiles && !tiles_ok) {
+            delete pl;
+            return fail(TN_E_ARG,
+                        "grid_mode 4 (register tiles) needs a 3D plan with at most %d taps per axis and grid "
+                        "lengths that are multiples of the %dx%dx%d tile with at least two tiles per axis",
+                        kTilesMaxWidth, kTilesT[0], kTilesT[1], kTilesT[2]);
+        }
         pl->grid_mode = params->grid_mode == kGridSpread   ? kGridSpread
                         : params->grid_mode == kGridMatrix ? kGridMatrix
                         : params->grid_mode == kGridRuns   ? kGridRuns
+                        : params->grid_mode == kGridTiles  ? kGridTiles
                         : matrix_ok                        ? kGridMatrix
+                        : tiles_ok                         ? kGridTiles
                         : runs_ok                          ? kGridRuns
                                                            : kGridSpread;
+        if (pl->grid_mode == kGridTiles) {  // the tile the kernel is built for, whatever was asked
+            pl->ntiles_total = 1;
+            for (int d = 0; d < 3; ++d) {
+                pl->tile[d] = kTilesT[2 - d];
+                pl->ntiles[d] = params->grid[d] / pl->tile[d];
+                pl->ntiles_total *= pl->ntiles[d];
+            }
+        }
         // kernels wider than kStagedMaxWidth grid only through the matrix
         const int wmax = std::max(params->width[0], std::max(params->width[1], params->width[2]));
         if (wmax > kStagedMaxWidth && pl->grid_mode != kGridMatrix) {
@@ -585,6 +634,7 @@ int64_t tn_plan_device_bytes(const tn_plan *plan) {
     if (plan->d_mrow)                                                                      // gridding matrix
         b += sizeof(int32_t) * ((int64_t)p.grid[0] * p.grid[1] * p.grid[2] * p.nframes + 1) +
              sizeof(int2) * std::max<int64_t>(plan->mnnz, 1);
+    if (plan->d_taps) b += 4 * sizeof(float4) * total;                                    // tile gridding taps
     for (int d = 0; d < 3; ++d)
         if (plan->d_apod[d]) b += sizeof(float) * p.shape[d];
     b += plan->fft_tw_bytes;

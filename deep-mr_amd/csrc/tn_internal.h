@@ -44,8 +44,16 @@ constexpr int kGridPad = 4;      // accumulator row padding (floats)
 // across footprint origins, or per-lane register runs over origin-sorted
 // entries (for at most kRunsMaxTaps taps per sample).
 // Matrix: per-cell sums over the plan's sparse gridding matrix (CSR by grid
-// cell), one lane per cell.
-enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2, kGridMatrix = 3 };
+// cell), one lane per cell. Tiles (3D): one wave per gridding tile of
+// kTilesT cells, its cells' sums in registers (lane = (x, y), one register
+// set per z plane), over the tile's entries in sample order with per-sample
+// taps from a plan table.
+enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2, kGridMatrix = 3, kGridTiles = 4 };
+constexpr int kTilesT[3] = {8, 8, 4};  // register-tile gridding tile (x, y, z): 64 lanes x 4 planes
+constexpr int kTilesMaxWidth = 4;      // taps per axis the register tiles hold
+constexpr int kTilesCoils = 8;         // coils per register-tile item (4 planes x 8 coils x 2 floats)
+constexpr int kTilesSplit = 1024;      // default entries per register-tile item (kooshball 128^3, osf 2:
+                                       // 4.30 ms against 4.69 at 2048 and 5.83 at 4096)
 constexpr int kMatrixLongRow = 128;  // default (2D): rows with more entries get a whole workgroup
 constexpr int kMatrixLongRow3d = 256;  // default (3D): kooshball 128^3 at osf 2: 3.74 ms against 4.33 (128), 4.30 (512)
 constexpr int kMatrixChunk = 16;     // default coils per item (record width of the coil-minor copy)
@@ -85,7 +93,8 @@ struct alignas(16) Job {
     int32_t NT[3];      // tiles per axis
     int32_t IT[3];      // staged-interpolation tile
     int32_t INT_[3];    // staged-interpolation tiles per axis
-    int32_t pad1[2];
+    int32_t st_chunk;   // tile gridding: coils per chunk of the coil-minor copy
+    int32_t pad1;
     float beta[3];
     float inv_scale;
     int32_t ndim;
@@ -105,7 +114,9 @@ struct alignas(16) Job {
     int64_t samp_off;
     int64_t samp_bstride;
     int64_t w_off;            // offset into the weight base (-1: no weight)
-    int64_t st_off;           // matrix gridding: this job's coil-minor sample copy in the table's buffer
+    int64_t st_off;           // matrix / tile gridding: this job's coil-minor sample copy in the table's buffer
+    const float4 *taps;       // tile gridding: per sample of this frame, (x taps), (y taps), (z taps),
+                              // (footprint origin x, y, z as int bits, 0)
 };
 
 // Low-rank subspace arguments of a launch (ncoeff == 0: plain transform).
@@ -181,6 +192,7 @@ struct tn_plan {
     float2 *d_fft_tw[2] = {nullptr, nullptr};  // pruned FFT twiddles exp(-2 pi i m / G), (x, y); null: no pruned FFT
     int64_t fft_tw_bytes = 0;
     int64_t mnnz = 0;
+    float4 *d_taps = nullptr;   // tile gridding: 4 float4 per sample (taps per axis, origin)
     std::map<int64_t, tn::LaunchTables> cache;
     std::map<int64_t, tn::LaunchTables> cache_gather;  // gather-mode tables (low-rank interpolation)
     std::map<int64_t, tn::LaunchTables> cache_lowrank; // tables of the low-rank gridding (no coil-minor copy)
@@ -220,6 +232,17 @@ int launch_apod_pad(const tn_plan *plan, const float2 *image, float2 *grid,
 int launch_crop_apod(const tn_plan *plan, const float2 *grid, float2 *image,
                      int64_t nimg, hipStream_t stream);
 int build_bins(tn_plan *plan, hipStream_t stream);
+// Whether register-tile gridding can serve a plan of these parameters: 3D,
+// at most kTilesMaxWidth taps per axis, grids of whole tiles with at least
+// two tiles per axis (a footprint then meets a tile at most once).
+inline bool tiles_fit(const tn_plan_params &p) {
+    if (p.ndim != 3) return false;
+    for (int d = 0; d < 3; ++d) {
+        const int T = kTilesT[2 - d];  // params are (z, y, x)
+        if (p.width[d] > kTilesMaxWidth || p.grid[d] % T != 0 || p.grid[d] < 2 * T) return false;
+    }
+    return true;
+}
 int build_matrix(tn_plan *plan, hipStream_t stream);
 // Pruned, fused oversampled-grid FFT stages (2D plans with supported lengths).
 bool fft_supported(const tn_plan *plan);

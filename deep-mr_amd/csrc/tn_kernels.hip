@@ -1254,6 +1254,160 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jo
     }
 }
 
+// Register-tile gridding (3D), the sums of one item's entries: lane (x, y)
+// of the tile face (origin x0, y0, z0) adds, entry by entry, its cells' terms
+// in the tile's 4 z planes for CW coils. Pointers are global-memory words
+// (loads at wave-uniform addresses become scalar loads) and restrict: the
+// kernel writes nothing they address. The loads of the next two entries are
+// issued before the current entry's sums, then all are waited for.
+using gfloat = const __attribute__((address_space(1))) float *;
+template <int CW>
+__device__ __forceinline__ void tiles_sum(gfloat __restrict__ tf, gfloat __restrict__ ef, gfloat __restrict__ sf,
+                                          int e0, int e1, int x, int y, int x0, int y0, int z0, const int (&G)[3],
+                                          const int (&W)[3], float (&ar)[kTilesT[2]][CW],
+                                          float (&ai)[kTilesT[2]][CW]) {
+    constexpr int TX = kTilesT[0], TY = kTilesT[1], TZ = kTilesT[2];
+    // tile-local footprint origin along one axis: the origin is wrapped into
+    // [0, G); a footprint that wraps onto the tile at 0 starts at origin - G
+    // (grids hold at least two tiles per axis)
+    auto local = [](int o, int t0, int T, int g) {
+        const int l = o - t0;
+        return l >= T ? l - g : l;
+    };
+    struct Stage {
+        int smp;       // sample
+        float z[4];    // z taps
+        int o[3];      // footprint origin
+        float wx, wy;  // this lane's x and y taps
+        float v[2 * CW];
+    };
+    // taps table: 16 words per sample (x, y, z taps, origin); entries: 4
+    // words, the sample's bits last; copy: CW complex per sample
+    auto fetch_head = [&](int smp, Stage &g) {  // scalar loads: origin, z taps
+        const int64_t b16 = 16 * (int64_t)smp;
+        g.smp = smp;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g.z[k] = tf[b16 + 8 + k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g.o[d] = __float_as_int(tf[b16 + 12 + d]);
+    };
+    auto fetch_body = [&](Stage &g) {  // the record (scalar), the lane's x and y taps (vector)
+        const int64_t b16 = 16 * (int64_t)g.smp;
+        const int lx = local(g.o[0], x0, TX, G[0]), ly = local(g.o[1], y0, TY, G[1]);
+        g.wx = tf[b16 + ((x - lx) & 3)];
+        g.wy = tf[b16 + 4 + ((y - ly) & 3)];
+#pragma unroll
+        for (int c = 0; c < 2 * CW; ++c) g.v[c] = sf[(int64_t)g.smp * 2 * CW + c];
+    };
+    auto sample_of = [&](int e) { return __float_as_int(ef[4 * (int64_t)e + 3]); };
+    auto accumulate = [&](const Stage &g) {
+        const int lx = local(g.o[0], x0, TX, G[0]), ly = local(g.o[1], y0, TY, G[1]);
+        const int lz = local(g.o[2], z0, TZ, G[2]);
+        if ((unsigned)(x - lx) < (unsigned)W[0] && (unsigned)(y - ly) < (unsigned)W[1]) {
+#pragma unroll
+            for (int p = 0; p < TZ; ++p) {
+                const int k = p - lz;
+                if ((unsigned)k < (unsigned)W[2]) {
+                    const float wz = k == 0 ? g.z[0] : k == 1 ? g.z[1] : k == 2 ? g.z[2] : g.z[3];
+                    const float w = (wz * g.wy) * g.wx;
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        ar[p][c] += w * g.v[2 * c];
+                        ai[p][c] += w * g.v[2 * c + 1];
+                    }
+                }
+            }
+        }
+    };
+    // Three entries in flight: e complete (a), e + 1 with its origin and z
+    // taps (b: its record and lane taps are loaded in this step), e + 2
+    // (c: origin and z taps loaded in this step); entries past the end
+    // reload the last one.
+    const int elast = e1 - 1;
+    Stage A, B, C;
+    fetch_head(sample_of(e0), A);
+    __builtin_amdgcn_s_waitcnt(0);
+    fetch_body(A);
+    fetch_head(sample_of(min(e0 + 1, elast)), B);
+    int i2 = sample_of(min(e0 + 2, elast));
+    __builtin_amdgcn_s_waitcnt(0);
+    int e = e0;
+    auto step = [&](const Stage &a, Stage &b, Stage &c) {
+        fetch_body(b);
+        fetch_head(i2, c);
+        i2 = sample_of(min(e + 3, elast));
+        accumulate(a);
+        __builtin_amdgcn_s_waitcnt(0);
+        return ++e < e1;
+    };
+    while (step(A, B, C) && step(B, C, A) && step(C, A, B)) {
+    }
+}
+
+// Register-tile gridding (3D): one wave per item, four items per workgroup.
+// Item: unit = tile of kTilesT cells, entries [e0, e1) of its bin (the
+// samples whose footprints meet the tile, ascending), coils c0 .. c1 - 1 of
+// chunk c0 / st_chunk of the coil-minor copy (CW coils per record, scaled
+// and weighted), part / slot as for the LDS kernels. Lane (x, y) of the
+// tile's 8x8 face sums its cells in the tile's 4 z planes in registers with
+// the weight (wz wy) wx the other variants use: a cell's sum runs over its
+// samples in ascending order, as the matrix rows do. The wave-uniform data
+// of an entry (sample record, z taps, footprint origin) come through scalar
+// loads, each lane's x and y taps through one vector load per axis.
+template <int CW>
+__global__ __launch_bounds__(kBlock) void grid_tiles_kernel(const Job *__restrict__ jobs,
+                                                            const Item *__restrict__ items, int64_t nitems,
+                                                            const float2 *__restrict__ st,
+                                                            float2 *__restrict__ grid, float *__restrict__ scratch,
+                                                            const int64_t *__restrict__ slot_base) {
+    constexpr int TX = kTilesT[0], TY = kTilesT[1], TZ = kTilesT[2];
+    static_assert(TX * TY == kWave, "one lane per (x, y) of the tile face");
+    // items in list order, four per workgroup: the hardware deals workgroups
+    // to the XCDs in turn, which spreads the hot tiles' parts (adjacent in
+    // the list) over all of them; contiguous per-XCD ranges measured 2x
+    // slower on a kooshball (its centre's parts landed on one XCD)
+    const int64_t idx = (int64_t)blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (idx >= nitems) return;
+    const Item it = items[idx];
+    const Job &J = jobs[it.job];
+    const int lane = threadIdx.x % kWave, x = lane % TX, y = lane / TX;
+    const int tile = it.unit;
+    const int x0 = (tile % J.NT[0]) * TX, y0 = ((tile / J.NT[0]) % J.NT[1]) * TY, z0 = tile / (J.NT[0] * J.NT[1]) * TZ;
+    float ar[TZ][CW], ai[TZ][CW];
+#pragma unroll
+    for (int p = 0; p < TZ; ++p)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) ar[p][c] = ai[p][c] = 0.0f;
+    if (it.e0 < it.e1) {
+        const int G[3] = {J.G[0], J.G[1], J.G[2]}, W[3] = {J.W[0], J.W[1], J.W[2]};
+        tiles_sum<CW>((gfloat)J.taps, (gfloat)J.entries,
+                      (gfloat)(st + J.st_off + (int64_t)(it.c0 / J.st_chunk) * J.npts * CW), it.e0, it.e1, x, y,
+                      x0, y0, z0, G, W, ar, ai);
+    }
+    const int nc = it.c1 - it.c0;
+    if (it.slot < 0) {
+        const int64_t Gx = J.G[0], Gy = J.G[1];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            if (c >= nc) break;
+            float2 *g = grid + J.grid_off + (int64_t)(it.c0 + c) * J.grid_bstride + (int64_t)(y0 + y) * Gx + x0 + x;
+#pragma unroll
+            for (int p = 0; p < TZ; ++p) g[(int64_t)(z0 + p) * Gy * Gx] = make_float2(ar[p][c], ai[p][c]);
+        }
+    } else {  // partial of a split tile, in fold_kernel's (coil, z, y, x) order
+        float *dst = scratch + slot_base[it.slot] + (int64_t)it.part * 2 * nc * (TX * TY * TZ);
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            if (c >= nc) break;
+#pragma unroll
+            for (int p = 0; p < TZ; ++p) {
+                const int i = (c * TZ + p) * (TX * TY) + lane;
+                *(float2 *)(dst + 2 * i) = make_float2(ar[p][c], ai[p][c]);
+            }
+        }
+    }
+}
+
 // ---- fused pre/post processing -------------------------------------------
 struct PrePost {
     int N[3];   // image (x, y, z)
@@ -1507,6 +1661,35 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                            (const float2 *)t.d_st, grid, (float2 *)scratch);
         TN_HIP(hipGetLastError());
         return launch_matrix_fold(t, grid, scratch, stream);
+    }
+    if (t.grid_mode == kGridTiles && lr.ncoeff == 0 && t.st_cw > 0) {
+        using TrFn = void (*)(const Job *, const Item *, const float2 *, const float *, float2 *);
+        using TlFn = void (*)(const Job *, const Item *, int64_t, const float2 *, float2 *, float *, const int64_t *);
+        TrFn tr = nullptr;
+        TlFn tl = nullptr;
+        switch (t.st_cw) {
+            case 2: tr = samp_transpose_kernel<1>; tl = grid_tiles_kernel<2>; break;
+            case 4: tr = samp_transpose_kernel<2>; tl = grid_tiles_kernel<4>; break;
+            case 8: tr = samp_transpose_kernel<3>; tl = grid_tiles_kernel<8>; break;
+            default: return fail(TN_E_ARG, "tile gridding: %d coils per record", t.st_cw);
+        }
+        if (kind_ndim(t.kind_grid) != 3 || kind_width(t.kind_grid) > kTilesMaxWidth)
+            return fail(TN_E_ARG, "tile gridding: 3D tables with at most %d taps per axis", kTilesMaxWidth);
+        if (t.n_tr > 0) {
+            hipLaunchKernelGGL(tr, dim3((unsigned)t.n_tr), dim3(kBlock), 0, stream, t.d_jobs, t.d_tr, samp, weight,
+                               t.d_st);
+            TN_HIP(hipGetLastError());
+        }
+        const int64_t groups = (t.n_grid + kBlock / kWave - 1) / (kBlock / kWave);
+        hipLaunchKernelGGL(tl, dim3((unsigned)groups), dim3(kBlock), 0, stream, t.d_jobs, t.d_grid, t.n_grid,
+                           (const float2 *)t.d_st, grid, scratch, t.d_slot_base);
+        TN_HIP(hipGetLastError());
+        if (t.n_fold > 0) {
+            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
+                               t.d_fold, scratch, t.d_slot_base, grid);
+            TN_HIP(hipGetLastError());
+        }
+        return TN_OK;
     }
     if (t.grid_mode == kGridMatrix) {
         if (t.grid_coils > kMatrixCoils) return fail(TN_E_ARG, "matrix gridding: %d coils per item", t.grid_coils);

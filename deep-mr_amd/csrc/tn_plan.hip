@@ -165,6 +165,24 @@ __global__ void matrix_emit_kernel(BinGeom geo, const float *__restrict__ coord,
     }
 }
 
+// Per-sample taps of the register-tile gridding (3D, at most
+// kTilesMaxWidth taps per axis): float4 x taps, y taps, z taps (zeros past
+// the axis width) and the footprint origin (x, y, z) as int bits, the same
+// taps and wrapped origin the other gridding variants evaluate.
+__global__ void tiles_taps_kernel(BinGeom geo, const float *__restrict__ coord, float4 *__restrict__ taps) {
+    const int64_t total = geo.npts * geo.nframes;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int base[3];
+        float w[3][kTilesMaxWidth];
+        for (int d = 0; d < 3; ++d)
+            axis_taps<kTilesMaxWidth>(coord[i * 3 + d], geo.ax[d].W, geo.ax[d].G, geo.ax[d].shift, beta_of(geo, d),
+                                      base[d], w[d]);
+        for (int d = 0; d < 3; ++d) taps[4 * i + d] = make_float4(w[d][0], w[d][1], w[d][2], w[d][3]);
+        taps[4 * i + 3] = make_float4(__int_as_float(base[0]), __int_as_float(base[1]), __int_as_float(base[2]), 0.0f);
+    }
+}
+
 __global__ void bin_gather_kernel(BinGeom geo, const float *__restrict__ coord,
                                   const uint32_t *__restrict__ keys,
                                   const int32_t *__restrict__ vals, int64_t n,
@@ -466,10 +484,20 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
                            plan->d_coord, (const int32_t *)offsets.p, (uint32_t *)keys.p,
                            (int32_t *)vals.p);
         TN_HIP(hipGetLastError());
+        // register tiles: each tile's entries in sample order (the stable
+        // sort's), the sums' fixed order
         int rc = sort_entries(geo, plan->d_coord, keys.p, vals.p, n, plan->d_bin_off,
                               &plan->d_entries, stream,
-                              plan->grid_mode == kGridRuns ? kOrderOrigin : kOrderSpread);
+                              plan->grid_mode == kGridRuns    ? kOrderOrigin
+                              : plan->grid_mode == kGridTiles ? kOrderSample
+                                                              : kOrderSpread);
         if (rc) return rc;
+        if (plan->grid_mode == kGridTiles) {
+            TN_HIP(hipMalloc(&plan->d_taps, sizeof(float4) * 4 * total));
+            hipLaunchKernelGGL(tiles_taps_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream, geo,
+                               plan->d_coord, plan->d_taps);
+            TN_HIP(hipGetLastError());
+        }
         plan->h_bin_off.assign(nbins + 1, 0);
         TN_HIP(hipMemcpyAsync(plan->h_bin_off.data(), plan->d_bin_off, sizeof(int32_t) * (nbins + 1),
                               hipMemcpyDeviceToHost, stream));

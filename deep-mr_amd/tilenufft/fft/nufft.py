@@ -171,8 +171,10 @@ def plan_nufft(coord, shape, width=4, oversamp=1.25, device="cpu", tile=None, in
     gridding workgroup before a hot tile is split; < 0 never) and
     ``grid_mode`` (1 per-sample LDS atomics, 2 register runs over
     origin-sorted samples, 3 per-cell sums over a plan-time sparse gridding
-    matrix, the automatic choice when that matrix fits in 4 GiB) optionally
-    override the launch parameters (0 / None = automatic).
+    matrix, the automatic choice when that matrix fits in 4 GiB, 4 (3D,
+    widths <= 4, grids of whole 8x8x4 tiles) one wave per tile summing its
+    cells in registers, the automatic choice when the matrix does not fit)
+    optionally override the launch parameters (0 / None = automatic).
     """
     coord = torch.as_tensor(coord).detach().cpu().to(torch.float32).numpy().copy()
     ndim = coord.shape[-1]

@@ -68,8 +68,8 @@ typedef struct tn_plan_params {
                              is split across workgroups (partials folded in a
                              fixed order); 0 = automatic, < 0 = never split   */
     int32_t grid_mode;    /* gridding variant: 0 automatic (3 when its matrix
-                             fits in 4 GiB, else 2 when W_y * W_x <= 36,
-                             else 1), 1 one LDS atomic per
+                             fits in 4 GiB, else 4 where it fits, else 2
+                             when W_y * W_x <= 36, else 1), 1 one LDS atomic per
                              tap and sample (entries spread over footprint
                              origins), 2 per-lane runs of origin-sorted
                              entries summed in registers, one z plane of the
@@ -79,7 +79,14 @@ typedef struct tn_plan_params {
                              tap) reading coil-minor sample records;
                              grid_chunk = coils per record, <= 16 (0: 16),
                              grid_split = row length above which one cell
-                             is summed by a workgroup of its own          */
+                             is summed by a workgroup of its own; 4 (3D,
+                             widths <= 4, grid lengths multiples of the
+                             8x8x4 (x, y, z) tile with two or more tiles
+                             per axis) one wave per tile summing its cells
+                             in registers over the tile's samples in
+                             ascending order, per-sample taps from a plan
+                             table (64 bytes per sample), coil-minor
+                             records; grid_chunk <= 8 (0: 8); tile ignored */
     float inv_scale;    /* 1 / kernel scaling (src/deepmr/fft/_interp/plan.py:190) */
     const float *apod[3]; /* host apodization vectors of length shape[d], or
                              NULL for ones (src/deepmr/fft/nufft.py:423-447)   */
@@ -109,7 +116,7 @@ int64_t tn_plan_entries(const tn_plan *plan);
 int64_t tn_plan_footprint_cells(const tn_plan *plan);
 /* Effective gridding tile (z, y, x). */
 int tn_plan_tile(const tn_plan *plan, int32_t tile_out[3]);
-/* Effective gridding variant (1, 2 or 3, with grid_mode 0 resolved), or a
+/* Effective gridding variant (1 to 4, with grid_mode 0 resolved), or a
  * negative status for a NULL plan. No counterpart in deepmr (introspection
  * for tuners and benchmarks). */
 int tn_plan_grid_mode(const tn_plan *plan);
