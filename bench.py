@@ -76,7 +76,7 @@ def variant_key(plan, kind, ncoils):
 # kernels of each timed stage (short names as fold_counters derives them)
 STAGE_KERNELS = {
     "grid": ("samp_transpose_kernel", "grid_matrix_t_kernel", "grid_matrix_kernel", "matrix_fold_kernel",
-             "grid_runs_kernel", "grid_kernel", "fold_kernel"),
+             "grid_tiles_kernel", "grid_runs_kernel", "grid_kernel", "fold_kernel"),
     "interp": ("interp_staged_kernel", "interp_kernel"),
     "fft_fwd": ("fft_cols_fwd_kernel", "fft_rows_kernel|false"),
     "fft_adj": ("fft_rows_kernel|true", "fft_cols_inv_kernel"),
@@ -355,11 +355,12 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
     )
 
 
-def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
+def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, compare_tiles=True, reps=3, grid_mode=0):
     """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
     256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3): the
-    automatic plan's gridding (the gridding matrix) and staged interpolation, and with
-    ``compare_runs`` the register-tile and register-run griddings too. Durations from HIP events
+    automatic plan's gridding (the gridding matrix) and staged interpolation, with
+    ``compare_tiles`` the register-tile gridding and with ``compare_runs`` the
+    register-run gridding too. Durations from HIP events
     on the launch stream; algorithmic bytes as for the 2D workload. Returns a
     dict."""
     import tilenufft
@@ -373,7 +374,7 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
     for osf in osfs:
         log(f"bench: kooshball osf {osf}")
         t0 = time.perf_counter()
-        plan = tilenufft.fft.plan_nufft(coord, 128, 4, osf, device=dev)
+        plan = tilenufft.fft.plan_nufft(coord, 128, 4, osf, device=dev, grid_mode=grid_mode)
         torch.cuda.synchronize(dev)
         case = {"osf": osf, "grid": list(plan.os_shape), "plan_s": time.perf_counter() - t0,
                 "plan_bytes": plan.interpolator.device_bytes()}
@@ -388,7 +389,7 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
         case["interp_ms"] = ti
         case["interp_frac"] = b_interp / (ti * 1e-3) / 1e9 / HBM_PEAK_GBS
         b = None
-        if compare_runs:
+        if compare_tiles:
             retune(plan, grid_mode=4)  # register tiles
             torch.cuda.synchronize(dev)
             case["tiles_plan_bytes"] = plan.interpolator.device_bytes()
@@ -396,6 +397,7 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, reps=3):
             b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(dev))
             case["grid_tiles_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)
             b = None
+        if compare_runs:
             retune(plan, grid_mode=2, tile=(16, 16, 16))
             b = BatchedNUFFT([plan], 8)
             case["grid_runs_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)
@@ -481,6 +483,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--tune-cache", default=None, help="directory of the autotuner's cache file")
+    ap.add_argument("--grid-mode", type=int, default=0,
+                    help="with --kooshball: gridding variant of the timed plan (0 automatic, 4 register tiles)")
     ap.add_argument("--grid-split", type=int, default=None,
                     help="after tuning, force this gridding split (for the matrix: the long-row threshold), "
                          "to profile each variant the tuner may pick")
@@ -517,7 +521,9 @@ def main():
     if args.kooshball:
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        res = kooshball_bench(dev, compare_runs=not args.kernels_only)  # --kernels-only: the automatic plan only
+        # --kernels-only: the automatic plan only
+        res = kooshball_bench(dev, compare_runs=not args.kernels_only, compare_tiles=not args.kernels_only,
+                              grid_mode=args.grid_mode)
         if args.calibrate_copy > 0:
             calibration_copy(dev, args.calibrate_copy)
         print(json.dumps(res), flush=True)

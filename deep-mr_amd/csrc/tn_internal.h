@@ -52,8 +52,8 @@ enum GridMode { kGridAuto = 0, kGridSpread = 1, kGridRuns = 2, kGridMatrix = 3, 
 constexpr int kTilesT[3] = {8, 8, 4};  // register-tile gridding tile (x, y, z): 64 lanes x 4 planes
 constexpr int kTilesMaxWidth = 4;      // taps per axis the register tiles hold
 constexpr int kTilesCoils = 8;         // coils per register-tile item (4 planes x 8 coils x 2 floats)
-constexpr int kTilesSplit = 1024;      // default entries per register-tile item (kooshball 128^3, osf 2:
-                                       // 4.30 ms against 4.69 at 2048 and 5.83 at 4096)
+constexpr int kTilesSplit = 384;       // default entries per register-tile item (kooshball 128^3, osf 2:
+                                       // 3.30 ms against 3.52 at 256, 3.38 at 512, 3.86 at 1024)
 constexpr int kMatrixLongRow = 128;  // default (2D): rows with more entries get a whole workgroup
 constexpr int kMatrixLongRow3d = 256;  // default (3D): kooshball 128^3 at osf 2: 3.74 ms against 4.33 (128), 4.30 (512)
 constexpr int kMatrixChunk = 16;     // default coils per item (record width of the coil-minor copy)

@@ -1220,6 +1220,7 @@ __global__ __launch_bounds__(kWave) void matrix_fold_kernel(const Job *__restric
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
+constexpr int kFoldBatch = 8;
 __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jobs,
                                                       const Item *__restrict__ items,
                                                       const float *__restrict__ scratch,
@@ -1243,13 +1244,27 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jo
     Idx4 c(threadIdx.x, kBlock, Tx, Ty, Tz);
     for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, c.next()) {
         if (c.x < ex && c.y < ey && c.z < ez) {
-            float re = src[2 * i], im = src[2 * i + 1];
-            for (int q = 1; q < nparts; ++q) {
-                re += src[q * pstride + 2 * i];
-                im += src[q * pstride + 2 * i + 1];
+            const float2 *s2 = (const float2 *)(src + 2 * (int64_t)i);
+            float2 a = s2[0];
+            int q = 1;
+            // kFoldBatch parts' loads in flight, added in part order
+            for (; q + kFoldBatch <= nparts; q += kFoldBatch) {
+                float2 v[kFoldBatch];
+#pragma unroll
+                for (int u = 0; u < kFoldBatch; ++u) v[u] = s2[(int64_t)(q + u) * (pstride / 2)];
+#pragma unroll
+                for (int u = 0; u < kFoldBatch; ++u) {
+                    a.x += v[u].x;
+                    a.y += v[u].y;
+                }
+            }
+            for (; q < nparts; ++q) {
+                const float2 v = s2[(int64_t)q * (pstride / 2)];
+                a.x += v.x;
+                a.y += v.y;
             }
             grid[J.grid_off + (int64_t)(it.c0 + c.b) * J.grid_bstride +
-                 ((int64_t)(oz + c.z) * Gy + (oy + c.y)) * Gx + (ox + c.x)] = make_float2(re, im);
+                 ((int64_t)(oz + c.z) * Gy + (oy + c.y)) * Gx + (ox + c.x)] = a;
         }
     }
 }

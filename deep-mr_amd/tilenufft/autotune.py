@@ -27,9 +27,11 @@ def default_candidates(ndim):
     else:
         tiles = [(8, 8, 8), (8, 16, 16), (16, 16, 16), (8, 8, 32), (16, 16, 32)]
         itiles = [(4, 8, 8), (8, 8, 8), (8, 8, 16), (8, 16, 16)]
+    # register tiles (grid_mode 4) are 3D only; plans they do not fit skip them
     return dict(tiles=tiles, grid_chunks=[0, 1, 2, 4, 8, 16], interp_chunks=[1, 2, 4, 8, 16],
                 itiles=itiles, staged_chunks=[2, 4, 8, 16], grid_splits=[512, 1024, 2048, 4096],
-                matrix_splits=[64, 128, 256], grid_modes=[3, 2, 1])
+                matrix_splits=[64, 128, 256], tiles_splits=[256, 384, 512, 1024],
+                grid_modes=[3, 2, 1] if ndim == 2 else [3, 4, 2, 1])
 
 
 def _cache_path():
@@ -94,8 +96,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         # 1) gridding variant x tile, automatic chunks
         best_tile, best_t, best_gm = None, float("inf"), 0
         for gm in cand.get("grid_modes", [0]):
-            # the matrix gridding does not use the gridding tile
-            for tile in (cand["tiles"][:1] if gm == 3 else cand["tiles"]):
+            # the matrix and register-tile griddings do not use the gridding tile
+            for tile in (cand["tiles"][:1] if gm in (3, 4) else cand["tiles"]):
                 try:
                     _apply(plans, tile, 0, 0, gmode=gm)
                     b = BatchedNUFFT(plans, ncoils)
@@ -123,7 +125,8 @@ def tune_batch(plans, ncoils, candidates=None, reps=5, use_cache=True, verbose=F
         best_split = 0
         # (for the matrix gridding: the row length above which a whole
         # workgroup sums one cell)
-        for sp in cand.get("matrix_splits" if best_gm == 3 else "grid_splits", []):
+        splits = "matrix_splits" if best_gm == 3 else "tiles_splits" if best_gm == 4 else "grid_splits"
+        for sp in cand.get(splits, []):
             _apply(plans, best_tile, 0, best_gc, split=sp, gmode=best_gm)
             b = BatchedNUFFT(plans, ncoils)
             t = _time(lambda: b.gridding(b.grid), reps, dev)

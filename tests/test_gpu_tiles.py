@@ -137,3 +137,25 @@ def test_tiles_refused_where_they_do_not_fit():
         tilenufft.fft.plan_nufft(testing.kooshball(8, 16, 16), 16, 6, 2.0, device=DEV, grid_mode=4)
     with pytest.raises(RuntimeError, match="register tiles"):
         tilenufft.fft.plan_nufft(testing.kooshball(8, 16, 14), 14, 4, 1.5, device=DEV, grid_mode=4)  # 21^3
+
+
+def test_autotune_3d_considers_register_tiles():
+    """The 3D autotuner's gridding candidates include the register tiles
+    (tile-independent, with their own split candidates); it leaves the plan
+    on the fastest variant, whose adjoint matches the oracle."""
+    from tilenufft.autotune import default_candidates, tune_batch
+    from tilenufft.batch import BatchedNUFFT
+
+    assert 4 in default_candidates(3)["grid_modes"] and 4 not in default_candidates(2)["grid_modes"]
+
+    n = 16
+    coord = testing.kooshball(40, 32, n)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV)
+    best = tune_batch([plan], 3, reps=2, use_cache=False)
+    assert best["grid_mode"] in (1, 2, 3, 4)
+    assert plan.interpolator.grid_mode_effective() == best["grid_mode"]
+    k = cnormal((3,) + coord.shape[:-1], 76)
+    a = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    assert rel(a, orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0))) < RTOL
+    b = BatchedNUFFT([plan], 3)
+    assert b.grid.shape[-3:] == (2 * n,) * 3
