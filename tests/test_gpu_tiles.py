@@ -159,3 +159,36 @@ def test_autotune_3d_considers_register_tiles():
     assert rel(a, orc.apply_nufft_adj(k, orc.plan_nufft(coord, n, 4, 2.0))) < RTOL
     b = BatchedNUFFT([plan], 3)
     assert b.grid.shape[-3:] == (2 * n,) * 3
+
+
+def test_tiles_batched_plans_and_frames_equal_single_plans():
+    """Two register-tile plans (one of two frames, one of one) with 5 and 8
+    coils in one batched gridding launch: each plan's grid is the same bits
+    as its own launch (per-job taps, bins and coil-minor copies), and the
+    two-frame plan's adjoint matches the oracle frame by frame."""
+    from tilenufft.batch import BatchedNUFFT
+
+    n = 16
+    c1 = np.stack([testing.kooshball(30, 32, n), uniform3d(30, 32, n, 11)])  # (2 frames, views, samples, 3)
+    c2 = testing.kooshball(45, 32, n)
+    p1 = tilenufft.fft.plan_nufft(c1, n, 4, 2.0, device=DEV, grid_mode=4, grid_split=48)
+    p2 = tilenufft.fft.plan_nufft(c2, n, 4, 2.0, device=DEV, grid_mode=4)
+    assert p1.nframes == 2
+    b = BatchedNUFFT([p1, p2], [5, 8])
+    gen = torch.Generator().manual_seed(77)
+    b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(DEV))
+    b.gridding(b.grid)
+    torch.cuda.synchronize()
+    for k, (p, c) in enumerate([(p1, 5), (p2, 8)]):
+        print(f"tiles batch plan {k}", flush=True)
+        single = BatchedNUFFT([p], c)
+        single.kspace.copy_(b.kspace_view(k).reshape(-1))
+        single.gridding(single.grid)
+        torch.cuda.synchronize()
+        g0 = b.grid_off[k] // int(np.prod(b.os_shape))
+        mine = b.grid[g0:g0 + c * p.nframes]
+        assert torch.equal(mine.view(torch.float32), single.grid.view(torch.float32))
+    ks = b.kspace_view(0).cpu().numpy()  # (coils, frames, npts)
+    img = tilenufft.fft.apply_nufft_adj(torch.from_numpy(ks.reshape(5, 2, 30, 32)).to(DEV), p1).cpu().numpy()
+    ref = orc.apply_nufft_adj(ks.reshape(5, 2, 30, 32), orc.plan_nufft(c1, n, 4, 2.0))
+    assert rel(img, ref) < RTOL
