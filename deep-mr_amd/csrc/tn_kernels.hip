@@ -1220,7 +1220,7 @@ __global__ __launch_bounds__(kWave) void matrix_fold_kernel(const Job *__restric
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
-constexpr int kFoldBatch = 8;
+constexpr int kFoldBatch = 16;
 __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jobs,
                                                       const Item *__restrict__ items,
                                                       const float *__restrict__ scratch,
@@ -1241,8 +1241,11 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jo
     const float *src = scratch + slot_base[it.e0];
     const int64_t pstride = 2 * (int64_t)ncoil * cells;
     const int Gx = J.G[0], Gy = J.G[1];
-    Idx4 c(threadIdx.x, kBlock, Tx, Ty, Tz);
-    for (int i = threadIdx.x; i < ncoil * cells; i += kBlock, c.next()) {
+    // workgroup blockIdx.y of gridDim.y takes elements blockIdx.y * kBlock + t,
+    // then every kBlock * gridDim.y-th
+    const int start = blockIdx.y * kBlock + threadIdx.x, stride = kBlock * gridDim.y;
+    Idx4 c(start, stride, Tx, Ty, Tz);
+    for (int i = start; i < ncoil * cells; i += stride, c.next()) {
         if (c.x < ex && c.y < ey && c.z < ez) {
             const float2 *s2 = (const float2 *)(src + 2 * (int64_t)i);
             float2 a = s2[0];
@@ -1700,7 +1703,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                            (const float2 *)t.d_st, grid, scratch, t.d_slot_base);
         TN_HIP(hipGetLastError());
         if (t.n_fold > 0) {
-            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
+            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold, (unsigned)t.fold_chunks), dim3(kBlock), 0, stream, t.d_jobs,
                                t.d_fold, scratch, t.d_slot_base, grid);
             TN_HIP(hipGetLastError());
         }
@@ -1725,7 +1728,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                        t.d_grid, t.n_grid, samp, grid, weight, scratch, t.d_slot_base, lr);
     TN_HIP(hipGetLastError());
     if (t.n_fold > 0) {
-        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
+        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold, (unsigned)t.fold_chunks), dim3(kBlock), 0, stream, t.d_jobs,
                            t.d_fold, scratch, t.d_slot_base, grid);
         TN_HIP(hipGetLastError());
     }
