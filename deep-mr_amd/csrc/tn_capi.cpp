@@ -459,7 +459,7 @@ void destroy(tn_plan *pl) {
     if (pl->d_mrow) (void)hipFree(pl->d_mrow);
     if (pl->d_mnz) (void)hipFree(pl->d_mnz);
     if (pl->d_taps) (void)hipFree(pl->d_taps);
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 3; ++a)
         if (pl->d_fft_tw[a]) (void)hipFree(pl->d_fft_tw[a]);
     for (int d = 0; d < 3; ++d)
         if (pl->d_apod[d]) (void)hipFree(pl->d_apod[d]);
@@ -808,6 +808,26 @@ int tn_fft_adjoint(tn_plan *plan, void *grid, void *image, int64_t nimg, tn_stre
     int rc = check_device(plan->device);
     if (rc) return rc;
     return launch_fft_adjoint(plan, (float2 *)grid, (float2 *)image, nimg, (hipStream_t)stream);
+}
+
+int tn_fft_z_forward(tn_plan *plan, const void *image, void *zout, int64_t nimg, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nimg < 0) return fail(TN_E_ARG, "nimg must be >= 0");
+    if (nimg > 0 && (!image || !zout)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    return launch_fft_z_forward(plan, (const float2 *)image, (float2 *)zout, nimg, (hipStream_t)stream);
+}
+
+int tn_fft_z_adjoint(tn_plan *plan, const void *zin, void *image, int64_t nimg, tn_stream stream) {
+    clear_error();
+    if (!plan) return fail(TN_E_ARG, "plan is NULL");
+    if (nimg < 0) return fail(TN_E_ARG, "nimg must be >= 0");
+    if (nimg > 0 && (!image || !zin)) return fail(TN_E_ARG, "NULL data pointer");
+    int rc = check_device(plan->device);
+    if (rc) return rc;
+    return launch_fft_z_adjoint(plan, (const float2 *)zin, (float2 *)image, nimg, (hipStream_t)stream);
 }
 
 int tn_batch_create(tn_plan *const *plans, const int64_t *nbatch, const int64_t *grid_offset,

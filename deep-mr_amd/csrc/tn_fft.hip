@@ -279,11 +279,13 @@ __device__ __forceinline__ bool column_block(int64_t nimg, int Nx, int C, int64_
 // Forward column stage: image columns mx0 .. mx0 + C - 1 of image b are
 // apodized (x then y, as the separate pre-pass does), placed at their grid
 // rows in LDS, transformed along y (length L = Gy; rows without data are
-// zeros) and written to grid columns (mx - Nx/2) mod Gx.
+// zeros) and written to grid columns (mx - hx) mod Gx (hx = Nx/2 for the x
+// axis of a grid; 0 where the columns are a flattened (y, x) plane that a
+// later stage places).
 template <int L, int C>
 __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__restrict__ img,
                                                               float2 *__restrict__ grid, int64_t nimg, int Nx,
-                                                              int Ny, int Gx, const float *__restrict__ apx,
+                                                              int Ny, int Gx, int hx, const float *__restrict__ apx,
                                                               const float *__restrict__ apy,
                                                               const float2 *__restrict__ tw) {
     constexpr int LP = fft_line(L) + fft_pad(C);
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__re
     int64_t b;
     int mx0;
     if (!column_block(nimg, Nx, C, b, mx0)) return;
-    const int hy = Ny / 2, hx = Nx / 2;
+    const int hy = Ny / 2;
     for (int i0 = threadIdx.x; i0 < Ny * C; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
         float2 v[kLoadBatch];
         float ax[kLoadBatch], ay[kLoadBatch];
@@ -342,13 +344,13 @@ __global__ __launch_bounds__(kBlock) void fft_cols_fwd_kernel(const float2 *__re
     }
 }
 
-// Adjoint column stage: grid columns (mx - Nx/2) mod Gx for image columns
+// Adjoint column stage: grid columns (mx - hx) mod Gx for image columns
 // mx0 .. mx0 + C - 1, inverse-transformed along y (L = Gy); the Ny image rows
 // are scaled by 1 / (Gx Gy), apodized (x then y) and written to the image.
 template <int L, int C>
 __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__restrict__ grid,
                                                               float2 *__restrict__ img, int64_t nimg, int Nx,
-                                                              int Ny, int Gx, const float *__restrict__ apx,
+                                                              int Ny, int Gx, int hx, const float *__restrict__ apx,
                                                               const float *__restrict__ apy,
                                                               const float2 *__restrict__ tw, float scale) {
     constexpr int LP = fft_line(L) + fft_pad(C);
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void fft_cols_inv_kernel(const float2 *__re
     int64_t b;
     int mx0;
     if (!column_block(nimg, Nx, C, b, mx0)) return;
-    const int hy = Ny / 2, hx = Nx / 2;
+    const int hy = Ny / 2;
     for (int i0 = threadIdx.x; i0 < L * C; i0 += kLoadBatch * kBlock) {  // loads first, then the LDS writes
         float2 v[kLoadBatch];
 #pragma unroll
@@ -424,9 +426,9 @@ constexpr size_t col_lds_bytes(int L) {
 }
 
 using RowFn = void (*)(float2 *, int64_t, int, const float2 *);
-using ColFwdFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, const float *, const float *,
+using ColFwdFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, int, const float *, const float *,
                           const float2 *);
-using ColInvFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, const float *, const float *,
+using ColInvFn = void (*)(const float2 *, float2 *, int64_t, int, int, int, int, const float *, const float *,
                           const float2 *, float);
 
 struct FftFns {
@@ -485,20 +487,24 @@ unsigned col_blocks(int64_t nimg, int Nx, int C) {
 
 }  // namespace
 
+// 2D and 3D plans whose transformed grid lengths have kernels
 bool fft_supported(const tn_plan *plan) {
     const tn_plan_params &p = plan->p;
-    if (p.ndim != 2) return false;
+    if (p.ndim != 2 && p.ndim != 3) return false;
+    if (p.ndim == 3 && fft_fns(p.grid[0]).L == 0) return false;
     return fft_fns(p.grid[2]).L != 0 && fft_fns(p.grid[1]).L != 0;
 }
 
 int build_fft(tn_plan *plan, hipStream_t stream) {
     if (!fft_supported(plan)) return TN_OK;
-    {   // dynamic LDS of the column tiles (up to 64 KiB with these column counts)
-        const FftFns fy = fft_fns(plan->p.grid[1]);
-        TN_HIP(hipFuncSetAttribute((const void *)fy.col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fy.lds));
-        TN_HIP(hipFuncSetAttribute((const void *)fy.col_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fy.lds));
+    // dynamic LDS of the column tiles (up to 64 KiB with these column counts),
+    // for the y columns and, in 3D, the z columns
+    for (int a = 1; a < plan->p.ndim; ++a) {
+        const FftFns fc = fft_fns(plan->p.grid[2 - a]);
+        TN_HIP(hipFuncSetAttribute((const void *)fc.col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fc.lds));
+        TN_HIP(hipFuncSetAttribute((const void *)fc.col_inv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fc.lds));
     }
-    for (int a = 0; a < 2; ++a) {  // 0: x (grid[2]), 1: y (grid[1])
+    for (int a = 0; a < plan->p.ndim; ++a) {  // 0: x (grid[2]), 1: y (grid[1]), 2: z (grid[0])
         const int L = plan->p.grid[2 - a];
         std::vector<float2> tw(L);
         const double pi = 3.14159265358979323846;
@@ -521,7 +527,7 @@ int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, i
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
     const FftFns fy = fft_fns(Gy), fx = fft_fns(Gx);
     hipLaunchKernelGGL(fy.col_fwd, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), fy.lds, stream, image, grid, nimg, Nx,
-                       Ny, Gx, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
+                       Ny, Gx, Nx / 2, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
                        (const float2 *)plan->d_fft_tw[1]);
     TN_HIP(hipGetLastError());
     const int64_t nlines = nimg * Gy;
@@ -547,8 +553,38 @@ int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t
     TN_HIP(hipGetLastError());
     const float scale = (float)(1.0 / ((double)Gx * (double)Gy));
     hipLaunchKernelGGL(fy.col_inv, dim3(col_blocks(nimg, Nx, fy.C)), dim3(kBlock), fy.lds, stream, grid, image, nimg, Nx,
-                       Ny, Gx, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
+                       Ny, Gx, Nx / 2, (const float *)plan->d_apod[2], (const float *)plan->d_apod[1],
                        (const float2 *)plan->d_fft_tw[1], scale);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+// z stage of the 3D plans: each image volume (Nz, Ny, Nx) is a 2D image of
+// Nz rows and Ny*Nx columns; the column kernels transform it along z (length
+// Gz) with the z apodization, the z placement (mz - Nz/2) mod Gz (forward) or
+// crop (adjoint) and, in the adjoint, the 1/Gz scale; the columns keep their
+// flattened (y, x) order (hx = 0) for the plane stages.
+int launch_fft_z_forward(const tn_plan *plan, const float2 *image, float2 *zout, int64_t nimg, hipStream_t stream) {
+    if (plan->p.ndim != 3 || !plan->d_fft_tw[2]) return fail(TN_E_ARG, "the plan has no z stage (3D plans with pruned FFT stages)");
+    if (nimg == 0) return TN_OK;
+    const int cols = plan->p.shape[1] * plan->p.shape[2], Nz = plan->p.shape[0], Gz = plan->p.grid[0];
+    const FftFns fz = fft_fns(Gz);
+    hipLaunchKernelGGL(fz.col_fwd, dim3(col_blocks(nimg, cols, fz.C)), dim3(kBlock), fz.lds, stream, image, zout, nimg, cols,
+                       Nz, cols, 0, (const float *)nullptr, (const float *)plan->d_apod[0],
+                       (const float2 *)plan->d_fft_tw[2]);
+    TN_HIP(hipGetLastError());
+    return TN_OK;
+}
+
+int launch_fft_z_adjoint(const tn_plan *plan, const float2 *zin, float2 *image, int64_t nimg, hipStream_t stream) {
+    if (plan->p.ndim != 3 || !plan->d_fft_tw[2]) return fail(TN_E_ARG, "the plan has no z stage (3D plans with pruned FFT stages)");
+    if (nimg == 0) return TN_OK;
+    const int cols = plan->p.shape[1] * plan->p.shape[2], Nz = plan->p.shape[0], Gz = plan->p.grid[0];
+    const FftFns fz = fft_fns(Gz);
+    const float scale = (float)(1.0 / (double)Gz);
+    hipLaunchKernelGGL(fz.col_inv, dim3(col_blocks(nimg, cols, fz.C)), dim3(kBlock), fz.lds, stream, zin, image, nimg, cols,
+                       Nz, cols, 0, (const float *)nullptr, (const float *)plan->d_apod[0],
+                       (const float2 *)plan->d_fft_tw[2], scale);
     TN_HIP(hipGetLastError());
     return TN_OK;
 }

@@ -191,7 +191,7 @@ struct tn_plan {
     int grid_mode = tn::kGridSpread;  // effective gridding variant (entry order)
     int32_t *d_mrow = nullptr;  // matrix gridding: row offsets, (nframes * cells + 1)
     int2 *d_mnz = nullptr;      // matrix gridding: (sample, weight bits) sorted by (frame, cell)
-    float2 *d_fft_tw[2] = {nullptr, nullptr};  // pruned FFT twiddles exp(-2 pi i m / G), (x, y); null: no pruned FFT
+    float2 *d_fft_tw[3] = {nullptr, nullptr, nullptr};  // pruned FFT twiddles exp(-2 pi i m / G), (x, y, z); null: none
     int64_t fft_tw_bytes = 0;
     int64_t mnnz = 0;
     float4 *d_taps = nullptr;   // tile gridding: 4 float4 per sample (taps per axis, origin)
@@ -251,6 +251,8 @@ bool fft_supported(const tn_plan *plan);
 int build_fft(tn_plan *plan, hipStream_t stream);
 int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, int64_t nimg, hipStream_t stream);
 int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t nimg, hipStream_t stream);
+int launch_fft_z_forward(const tn_plan *plan, const float2 *image, float2 *zout, int64_t nimg, hipStream_t stream);
+int launch_fft_z_adjoint(const tn_plan *plan, const float2 *zin, float2 *image, int64_t nimg, hipStream_t stream);
 int count_footprint(tn_plan *plan, hipStream_t stream);
 
 // Variant selection shared by the table builder and the launchers.

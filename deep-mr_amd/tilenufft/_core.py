@@ -288,21 +288,50 @@ class NativePlan:
 
     @property
     def has_fft(self):
-        """True when the plan has the pruned, fused FFT stages (2D, grid
-        lengths among 64, 80, 96, 128, 160, 192, 240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024)."""
+        """True when the plan has the pruned, fused FFT stages (2D and 3D,
+        the y and x grid lengths among 64, 80, 96, 128, 160, 192, 240, 256,
+        288, 320, 384, 400, 480, 512, 576, 640, 768, 1024)."""
         return bool(_lib.load().tn_plan_has_fft(self.handle))
 
     def fft_forward(self, image, grid, nimg):
-        """image (nimg, *N) -> grid (nimg, *G): apodize, pad and forward DFT."""
-        _lib.check(_lib.load().tn_fft_forward(self.handle, ptr(image), ptr(grid), int(nimg),
+        """image (nimg, *N) -> grid (nimg, *G): apodize, pad and forward DFT.
+
+        3D: the z stage (apodization, placement and DFT along z, into a
+        (nimg, Gz, Ny, Nx) workspace), then the (y, x) plane stages on its
+        nimg * Gz planes."""
+        lib = _lib.load()
+        if self.ndim == 3:
+            gz, ny, nx = self.grid[0], self.shape[1], self.shape[2]
+            with torch.cuda.device(self.device):
+                zout = torch.empty((nimg, gz, ny, nx), dtype=torch.complex64, device=self.device)
+                _lib.check(lib.tn_fft_z_forward(self.handle, ptr(image), ptr(zout), int(nimg),
+                                                stream_ptr(self.device)), "tn_fft_z_forward")
+                _lib.check(lib.tn_fft_forward(self.handle, ptr(zout), ptr(grid), int(nimg * gz),
                                               stream_ptr(self.device)), "tn_fft_forward")
+            return grid
+        _lib.check(lib.tn_fft_forward(self.handle, ptr(image), ptr(grid), int(nimg),
+                                      stream_ptr(self.device)), "tn_fft_forward")
         return grid
 
     def fft_adjoint(self, grid, image, nimg):
         """grid (nimg, *G) -> image (nimg, *N): inverse DFT (1 / prod(G)),
-        crop and apodize; ``grid`` is overwritten."""
-        _lib.check(_lib.load().tn_fft_adjoint(self.handle, ptr(grid), ptr(image), int(nimg),
+        crop and apodize; ``grid`` is overwritten.
+
+        3D: the (y, x) plane stages on the nimg * Gz grid planes, into a
+        (nimg, Gz, Ny, Nx) workspace, then the z stage (inverse DFT along z,
+        1 / Gz, crop and apodization)."""
+        lib = _lib.load()
+        if self.ndim == 3:
+            gz, ny, nx = self.grid[0], self.shape[1], self.shape[2]
+            with torch.cuda.device(self.device):
+                planes = torch.empty((nimg, gz, ny, nx), dtype=torch.complex64, device=self.device)
+                _lib.check(lib.tn_fft_adjoint(self.handle, ptr(grid), ptr(planes), int(nimg * gz),
                                               stream_ptr(self.device)), "tn_fft_adjoint")
+                _lib.check(lib.tn_fft_z_adjoint(self.handle, ptr(planes), ptr(image), int(nimg),
+                                                stream_ptr(self.device)), "tn_fft_z_adjoint")
+            return image
+        _lib.check(lib.tn_fft_adjoint(self.handle, ptr(grid), ptr(image), int(nimg),
+                                      stream_ptr(self.device)), "tn_fft_adjoint")
         return image
 
 

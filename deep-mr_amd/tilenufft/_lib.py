@@ -67,6 +67,8 @@ _SIGNATURES = {
     "tn_plan_has_fft": (c_int32, [c_void_p]),
     "tn_fft_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "tn_fft_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "tn_fft_z_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "tn_fft_z_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "tn_batch_create": (c_int32, [ctypes.POINTER(c_void_p), ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
                                   ctypes.POINTER(c_int64), ctypes.POINTER(c_int64), c_int32,
                                   ctypes.POINTER(c_void_p)]),

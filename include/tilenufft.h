@@ -196,8 +196,14 @@ int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg,
                  tn_stream stream);
 
 /* 1 when the plan has the pruned, fused oversampled FFT stages below, else
- * 0: 2D plans whose grid lengths are among the oversampled sizes ceil(osf N)
- * of N = 64..512 at osf 1.25, 1.5 and 2 (64, 80, 96, 128, 160, 192, 240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024). */
+ * 0: 2D and 3D plans whose grid lengths are among the oversampled sizes
+ * ceil(osf N) of N = 64..512 at osf 1.25, 1.5 and 2 (64, 80, 96, 128, 160,
+ * 192, 240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024).
+ * For 3D plans tn_fft_forward / tn_fft_adjoint transform (y, x) planes
+ * (nimg counts planes) and tn_fft_z_forward / tn_fft_z_adjoint the z axis:
+ * forward = z stage into a (nimg, Gz, Ny, Nx) workspace, then the plane
+ * stages on its nimg * Gz planes; adjoint = the plane stages into that
+ * workspace, then the z stage. */
 int tn_plan_has_fft(const tn_plan *plan);
 
 /* Whole forward FFT stage of the NUFFT on the device: apodization, centred
@@ -217,6 +223,21 @@ int tn_fft_forward(tn_plan *plan, const void *image, void *grid, int64_t nimg,
  * (src/deepmr/fft/nufft.py:587-593). TN_E_ARG when tn_plan_has_fft is 0. */
 int tn_fft_adjoint(tn_plan *plan, void *grid, void *image, int64_t nimg,
                    tn_stream stream);
+
+/* z stage of a 3D plan's forward FFT: apodization along z, centred zero
+ * padding and the unnormalised DFT along z, image (nimg, Nz, Ny, Nx) ->
+ * zout (nimg, Gz, Ny, Nx) (every cell written). With the plane stages it
+ * replaces _apodize + resize + fft.fft(norm=None) of a 3D plan
+ * (src/deepmr/fft/nufft.py:499-505). TN_E_ARG without a z stage. */
+int tn_fft_z_forward(tn_plan *plan, const void *image, void *zout, int64_t nimg,
+                     tn_stream stream);
+
+/* z stage of a 3D plan's adjoint FFT: inverse DFT along z scaled by 1 / Gz,
+ * centred crop and apodization along z, zin (nimg, Gz, Ny, Nx) -> image
+ * (nimg, Nz, Ny, Nx) (src/deepmr/fft/nufft.py:587-593, with the plane
+ * stages). TN_E_ARG without a z stage. */
+int tn_fft_z_adjoint(tn_plan *plan, const void *zin, void *image, int64_t nimg,
+                     tn_stream stream);
 
 /* ---- batched launches over several plans (ragged work queue) ----------
  * A batch binds `njobs` (plan, nbatch, grid offset, sample offset) tuples to

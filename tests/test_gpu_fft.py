@@ -110,3 +110,30 @@ def test_fft_stages_unsupported_lengths_fall_back():
     x = testing.complex_normal((1, 28, 28), seed=5).numpy()
     y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
     assert rel(y, orc.apply_nufft(x, orc.plan_nufft(coord, 28, 4, 2.0))) < RTOL
+
+
+@pytest.mark.parametrize("n, osf", [(32, 2.0), (64, 1.25), (64, 1.5), (51, 1.25)])
+def test_fft_stages_3d_match_library_fft(n, osf):
+    """3D plans: the z axis through torch.fft around the library's (y, x)
+    plane stages against tn_apod_pad + torch.fft.fftn (forward) and
+    torch.fft.ifftn + tn_crop_apod (adjoint), for even and odd image sizes
+    (N = 51: grid 64, the centring of an odd axis)."""
+    coord = testing.kooshball(20, 16, n)
+    plan = tilenufft.fft.plan_nufft(coord, n, 4, osf, device=DEV)
+    nat = plan.interpolator
+    assert nat.has_fft
+    nimg = 3
+    g = plan.os_shape
+    x = testing.complex_normal((nimg, n, n, n), seed=n).to(DEV)
+    grid = torch.empty((nimg, *g), dtype=torch.complex64, device=DEV)
+    got = nat.fft_forward(x, grid, nimg).clone()
+    pad = torch.empty_like(grid)
+    nat.apod_pad(x, pad, nimg)
+    ref = torch.fft.fftn(pad, dim=(-3, -2, -1))
+    assert rel(got.cpu().numpy(), ref.cpu().numpy()) < 1e-5
+    k = testing.complex_normal((nimg, *g), seed=n + 1).to(DEV)
+    img = torch.empty((nimg, n, n, n), dtype=torch.complex64, device=DEV)
+    ref_img = torch.empty_like(img)
+    nat.crop_apod(torch.fft.ifftn(k, dim=(-3, -2, -1)), ref_img, nimg)
+    nat.fft_adjoint(k.clone(), img, nimg)
+    assert rel(img.cpu().numpy(), ref_img.cpu().numpy()) < 1e-5
