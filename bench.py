@@ -358,7 +358,8 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
 def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, compare_tiles=True, reps=3, grid_mode=0):
     """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
     256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3): the
-    automatic plan's gridding (the gridding matrix) and staged interpolation, with
+    automatic plan's gridding (the gridding matrix) and staged interpolation,
+    the whole forward and adjoint transforms, with
     ``compare_tiles`` the register-tile gridding and with ``compare_runs`` the
     register-run gridding too. Durations from HIP events
     on the launch stream; algorithmic bytes as for the 2D workload. Returns a
@@ -388,6 +389,11 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, compare_tiles=True
         ti = time_kernel(lambda: b.interp(b.grid), reps, dev)
         case["interp_ms"] = ti
         case["interp_frac"] = b_interp / (ti * 1e-3) / 1e9 / HBM_PEAK_GBS
+        # whole transforms: forward (FFT stages + interpolation) and adjoint
+        # (gridding + FFT stages) of the 8 coil images
+        case["fft_stages"] = bool(plan.interpolator.has_fft)
+        case["nufft_fwd_ms"] = time_kernel(lambda: b.forward(), reps, dev)
+        case["nufft_adj_ms"] = time_kernel(lambda: b.adjoint(), reps, dev)
         b = None
         if compare_tiles:
             retune(plan, grid_mode=4)  # register tiles
