@@ -455,9 +455,15 @@ FftFns fns_for() {
 
 // Transform lengths with kernels: the oversampled sizes ceil(osf N) of
 // N = 64, 128, 192, 256, 320, 384, 512 at osf 1.25, 1.5 and 2 (all of the
-// form 2^a 3^b 5^c), 64 .. 1024.
+// form 2^a 3^b 5^c), 64 .. 1024, and the short lengths 16 .. 48 of small
+// grids and of Cartesian slice axes (osf 1).
 FftFns fft_fns(int L) {
     switch (L) {
+        case 16: return fns_for<16>();
+        case 24: return fns_for<24>();
+        case 32: return fns_for<32>();
+        case 40: return fns_for<40>();
+        case 48: return fns_for<48>();
         case 64: return fns_for<64>();
         case 80: return fns_for<80>();
         case 96: return fns_for<96>();

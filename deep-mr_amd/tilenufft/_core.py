@@ -289,8 +289,8 @@ class NativePlan:
     @property
     def has_fft(self):
         """True when the plan has the pruned, fused FFT stages (2D and 3D,
-        the y and x grid lengths among 64, 80, 96, 128, 160, 192, 240, 256,
-        288, 320, 384, 400, 480, 512, 576, 640, 768, 1024)."""
+        every grid length among 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192,
+        240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024)."""
         return bool(_lib.load().tn_plan_has_fft(self.handle))
 
     def fft_forward(self, image, grid, nimg):

@@ -198,7 +198,8 @@ int tn_crop_apod(tn_plan *plan, const void *grid, void *image, int64_t nimg,
 /* 1 when the plan has the pruned, fused oversampled FFT stages below, else
  * 0: 2D and 3D plans whose grid lengths are among the oversampled sizes
  * ceil(osf N) of N = 64..512 at osf 1.25, 1.5 and 2 (64, 80, 96, 128, 160,
- * 192, 240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024).
+ * 192, 240, 256, 288, 320, 384, 400, 480, 512, 576, 640, 768, 1024) and the
+ * short lengths 16, 24, 32, 40, 48 (small grids, Cartesian slice axes).
  * For 3D plans tn_fft_forward / tn_fft_adjoint transform (y, x) planes
  * (nimg counts planes) and tn_fft_z_forward / tn_fft_z_adjoint the z axis:
  * forward = z stage into a (nimg, Gz, Ny, Nx) workspace, then the plane

@@ -137,3 +137,27 @@ def test_fft_stages_3d_match_library_fft(n, osf):
     nat.crop_apod(torch.fft.ifftn(k, dim=(-3, -2, -1)), ref_img, nimg)
     nat.fft_adjoint(k.clone(), img, nimg)
     assert rel(img.cpu().numpy(), ref_img.cpu().numpy()) < 1e-5
+
+
+def test_stack_of_stars_short_z_axis_matches_oracle():
+    """Stack of stars: golden-angle radial in-plane, integer z (a Cartesian
+    slice axis: width 1, osf 1, so Gz = Nz = 32, a short transform length).
+    The forward and adjoint run through the 3D FFT stages and match the
+    oracle."""
+    n, nz = 32, 32
+    print(f"stack of stars {nz} slices {n}^2", flush=True)
+    rad = testing.golden_angle_radial(12, 48, n)  # (views, samples, 2)
+    z = (np.arange(nz) - nz // 2).astype(np.float32)
+    coord = np.concatenate([np.broadcast_to(rad[None], (nz, *rad.shape)),
+                            np.broadcast_to(z[:, None, None, None], (nz, *rad.shape[:2], 1))], -1)
+    coord = np.ascontiguousarray(coord.reshape(nz * rad.shape[0], rad.shape[1], 3))
+    plan = tilenufft.fft.plan_nufft(coord, (nz, n, n), 4, 2.0, device=DEV)
+    assert plan.os_shape == (nz, 2 * n, 2 * n) and plan.width[0] == 1
+    assert plan.interpolator.has_fft
+    x = testing.complex_normal((2, nz, n, n), seed=40).numpy()
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    p_ref = orc.plan_nufft(coord, (nz, n, n), 4, 2.0)
+    assert rel(y, orc.apply_nufft(x, p_ref)) < 1e-5
+    k = testing.complex_normal(y.shape, seed=41).numpy()
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    assert rel(xa, orc.apply_nufft_adj(k, p_ref)) < 1e-5
