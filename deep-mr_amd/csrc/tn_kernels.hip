@@ -261,6 +261,44 @@ __device__ __forceinline__ float2 staged_taps(const float2 *r, const float (&w)[
     return make_float2(re, im);
 }
 
+// The same for two coils of the staged region (planes r0 and r1): each tap's
+// weight is formed once for both, and each coil's sum is the same sequence
+// of operations as staged_taps gives it.
+template <int NDIM, int WC, bool FULL>
+__device__ __forceinline__ void staged_taps2(const float2 *r0, const float2 *r1, const float (&w)[3][WC], int nx,
+                                             int ny, int nz, int Rx, int Ry, float2 &t0, float2 &t1) {
+    float re0 = 0.0f, im0 = 0.0f, re1 = 0.0f, im1 = 0.0f;
+    auto plane = [&](int kz, float wz) {
+#pragma unroll
+        for (int ky = 0; ky < WC; ++ky) {
+            if (FULL || ky < ny) {
+                const float vzy = wz * w[1][ky];
+                const int off = (kz * Ry + ky) * Rx;
+#pragma unroll
+                for (int kx = 0; kx < WC; ++kx) {
+                    if (FULL || kx < nx) {
+                        const float v = vzy * w[0][kx];
+                        const float2 a = r0[off + kx], b = r1[off + kx];
+                        re0 += v * a.x;
+                        im0 += v * a.y;
+                        re1 += v * b.x;
+                        im1 += v * b.y;
+                    }
+                }
+            }
+        }
+    };
+    if constexpr (NDIM > 2) {
+#pragma unroll 1
+        for (int kz = 0; kz < WC; ++kz)
+            if (FULL || kz < nz) plane(kz, w[2][kz]);
+    } else {
+        plane(0, w[2][0]);
+    }
+    t0 = make_float2(re0, im0);
+    t1 = make_float2(re1, im1);
+}
+
 // Staged interpolation: a workgroup takes one segment (<= kBlock * staged_ept(NDIM)
 // samples) of one interpolation tile. Each lane evaluates the taps of its
 // samples once and keeps them in registers; then, coil chunk by coil chunk,
@@ -365,13 +403,25 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
 #pragma unroll
         for (int q = 0; q < kStagedEpt; ++q) {
             if (loc[q] < 0) continue;
-            for (int b = 0; b < ncoil; ++b) {
+            int b = 0;
+            for (; b + 2 <= ncoil; b += 2) {  // coil pairs share each tap's weight
+                const float2 *r = region + (size_t)b * cells + loc[q];
+                float2 t0, t1;
+                if (full)
+                    staged_taps2<NDIM, WC, true>(r, r + cells, w[q], nx, ny, nz, Rx, Ry, t0, t1);
+                else
+                    staged_taps2<NDIM, WC, false>(r, r + cells, w[q], nx, ny, nz, Rx, Ry, t0, t1);
+                samp[J.samp_off + (int64_t)(cc + b) * J.samp_bstride + pnt[q]] =
+                    make_float2(t0.x * scl[q], t0.y * scl[q]);
+                samp[J.samp_off + (int64_t)(cc + b + 1) * J.samp_bstride + pnt[q]] =
+                    make_float2(t1.x * scl[q], t1.y * scl[q]);
+            }
+            if (b < ncoil) {
                 const float2 *r = region + (size_t)b * cells + loc[q];
                 const float2 t = full ? staged_taps<NDIM, WC, true>(r, w[q], nx, ny, nz, Rx, Ry)
                                       : staged_taps<NDIM, WC, false>(r, w[q], nx, ny, nz, Rx, Ry);
-                const float re = t.x, im = t.y;
                 samp[J.samp_off + (int64_t)(cc + b) * J.samp_bstride + pnt[q]] =
-                    make_float2(re * scl[q], im * scl[q]);
+                    make_float2(t.x * scl[q], t.y * scl[q]);
             }
         }
     }
