@@ -527,7 +527,7 @@ int build_fft(tn_plan *plan, hipStream_t stream) {
 }
 
 int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, int64_t nimg, hipStream_t stream) {
-    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D plans with common oversampled grid lengths, 64..1024)");
+    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D and 3D plans with common oversampled grid lengths, 16..1024)");
     if (nimg == 0) return TN_OK;
     const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
@@ -546,7 +546,7 @@ int launch_fft_forward(const tn_plan *plan, const float2 *image, float2 *grid, i
 }
 
 int launch_fft_adjoint(const tn_plan *plan, float2 *grid, float2 *image, int64_t nimg, hipStream_t stream) {
-    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D plans with common oversampled grid lengths, 64..1024)");
+    if (!plan->d_fft_tw[0]) return fail(TN_E_ARG, "the plan has no pruned FFT (2D and 3D plans with common oversampled grid lengths, 16..1024)");
     if (nimg == 0) return TN_OK;
     const int Nx = plan->p.shape[2], Ny = plan->p.shape[1];
     const int Gx = plan->p.grid[2], Gy = plan->p.grid[1];
