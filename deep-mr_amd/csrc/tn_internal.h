@@ -106,7 +106,8 @@ struct alignas(16) Job {
     const float4 *entries;    // (x, y, z, bits(sample)) sorted by tile
     const int32_t *bin_off;   // (ntiles + 1) offsets into entries
     const float4 *pentries;   // each sample once, sorted by the interpolation
-                              // tile holding its footprint origin
+                              // tile holding its footprint origin (3D: then
+                              // by the origin cell)
     const int32_t *pbin_off;  // (interp tiles + 1) offsets into pentries
     const int32_t *mrow;      // matrix gridding: (cells + 1) row offsets of this frame
     const int2 *mnz;          // matrix gridding: (sample, weight bits) per entry

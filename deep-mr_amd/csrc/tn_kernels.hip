@@ -304,7 +304,8 @@ __device__ __forceinline__ void staged_taps2(const float2 *r0, const float2 *r1,
 // samples once and keeps them in registers; then, coil chunk by coil chunk,
 // the workgroup loads the (tile + W-1 halo) grid region of the chunk into
 // LDS with coalesced row reads and every lane gathers its taps from LDS.
-// Samples come from the plan's primary bins (each sample exactly once).
+// Samples come from the plan's primary bins (each sample exactly once; in
+// 3D ordered by footprint origin inside the tile).
 template <int NDIM, int WC>
 __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__restrict__ jobs,
                                                                const Item *__restrict__ items, int64_t nitems,

@@ -511,8 +511,12 @@ int build_bins(tn_plan *plan, hipStream_t stream) {
         hipLaunchKernelGGL(bin_primary_kernel, dim3(grid_blocks(total)), dim3(kBlock), 0, stream,
                            pgeo, plan->d_coord, (uint32_t *)keys.p, (int32_t *)vals.p);
         TN_HIP(hipGetLastError());
+        // 3D: footprint-origin order inside each tile, so that the lanes of
+        // a wave read neighbouring cells of the staged region (fewer LDS bank
+        // conflicts; every sample's sum is the same whichever lane forms it);
+        // 2D measures the same either way and keeps sample order
         int rc = sort_entries(pgeo, plan->d_coord, keys.p, vals.p, total, plan->d_pbin_off,
-                              &plan->d_pentries, stream);
+                              &plan->d_pentries, stream, pgeo.ndim == 3 ? kOrderOrigin : kOrderSample);
         if (rc) return rc;
         TN_HIP(hipMemcpyAsync(plan->h_pbin_off.data(), plan->d_pbin_off, sizeof(int32_t) * (npbins + 1),
                               hipMemcpyDeviceToHost, stream));

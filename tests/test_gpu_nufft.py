@@ -248,6 +248,30 @@ def test_interp_variants_match_oracle(ndim, mode, itile):
     assert rel(y, orc.apply_nufft(x, orc.plan_nufft(coord, n, 4, 1.5))) < RTOL
 
 
+def test_staged_interp_3d_same_bits_for_every_tiling():
+    """3D staged interpolation takes each tile's samples in footprint-origin
+    order, a segment of them per workgroup: whichever tile, segment, lane or
+    coil chunk forms a sample's value, its taps are summed in the same order,
+    so the k-space is the same bits for every tiling and chunk, and matches
+    the oracle. A dense kooshball centre gives origin cells shared by many
+    samples, segments splitting a tile and wrapped regions at the faces."""
+    n = 16
+    coord = np.concatenate([testing.kooshball(120, 32, n), np.random.default_rng(5).uniform(
+        -n / 2, n / 2, (20, 32, 3)).astype(np.float32)])
+    x = cnormal((5, n, n, n), 33)
+    xt = torch.from_numpy(x).to(DEV)
+    outs = []
+    for itile, chunk in [((8, 8, 16), 2), ((8, 8, 8), 4), ((4, 8, 8), 1), ((4, 4, 4), 3)]:
+        print(f"staged 3D interp itile={itile} chunk={chunk}", flush=True)
+        plan = tilenufft.fft.plan_nufft(coord, n, 4, 2.0, device=DEV, interp_mode=2, itile=itile,
+                                        interp_chunk=chunk)
+        outs.append(tilenufft.fft.apply_nufft(xt, plan))
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(o.view(torch.float32), outs[0].view(torch.float32))
+    assert rel(outs[0].cpu().numpy(), orc.apply_nufft(x, orc.plan_nufft(coord, n, 4, 2.0))) < RTOL
+
+
 @pytest.mark.parametrize("split", [-1, 3, 64, 500])
 def test_grid_split_matches_oracle_and_is_deterministic(split):
     # hot tiles split across workgroups, partials folded in a fixed order
