@@ -247,7 +247,7 @@ def cpu_baseline(wl, batch, frame=0, reps=5, warm=1, budget_s=30.0):
         orc.set_fft_backend("numpy")
 
 
-def config1_bench(dev, reps=10, warm=2):
+def plain_radial_bench(dev, reps=10, warm=2):
     """Plain 2D single-coil NUFFT forward + adjoint: 128x128 image (complex
     N(0,1)), golden-angle radial 128 spokes x 256 samples in [-64, 64), osf
     2.0, KB width 4. Times the oracle's -ffast-math C/OpenMP restatement of
@@ -481,7 +481,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
-    ap.add_argument("--no-config1", action="store_true", help="skip the plain single-coil 2D leg")
+    ap.add_argument("--no-plain-radial", action="store_true", help="skip the plain single-coil 2D radial leg")
     ap.add_argument("--no-kooshball", action="store_true", help="skip the 3D kooshball kernel leg")
     ap.add_argument("--no-cgsense", action="store_true", help="skip the unrolled CG-SENSE (autograd) leg")
     ap.add_argument("--kernels-only", action="store_true",
@@ -640,10 +640,10 @@ def main():
     if args.calibrate_copy > 0:
         calibration_copy(dev, args.calibrate_copy)
 
-    config1 = None
-    if not args.no_config1 and not args.kernels_only:
+    plain_radial = None
+    if not args.no_plain_radial and not args.kernels_only:
         log("bench: plain single-coil 2D radial (CPU oracle on all threads / one thread, GPU)")
-        config1 = config1_bench(dev)
+        plain_radial = plain_radial_bench(dev)
     kooshball = None
     if not args.no_kooshball and not args.kernels_only:
         kooshball = kooshball_bench(dev)
@@ -709,7 +709,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "parity_rel_l2_frame0_vs_oracle": parity,
-        "config1_plain_2d": config1,
+        "plain_radial_2d": plain_radial,
         "kooshball_3d": kooshball,
         "cgsense_2d": cgsense,
         "serving": serving,
