@@ -1,5 +1,6 @@
-"""GPU parity of the pruned, fused FFT stages of the 2D NUFFT (tn_fft_forward /
-tn_fft_adjoint) against the oracle's own steps (src/deepmr/fft/nufft.py:499-505
+"""GPU parity of the pruned, fused FFT stages of the NUFFT (tn_fft_forward /
+tn_fft_adjoint; in 3D with the z stage tn_fft_z_forward / tn_fft_z_adjoint
+around them) against the oracle's own steps (src/deepmr/fft/nufft.py:499-505
 and :587-593): apodize -> centred zero pad -> centred FFT, and centred
 inverse FFT -> centred crop -> apodize. The device grid holds the centred
 spectrum with the shifts folded in (ifftshift of the oracle's), so the
@@ -112,27 +113,30 @@ def test_fft_stages_unsupported_lengths_fall_back():
     assert rel(y, orc.apply_nufft(x, orc.plan_nufft(coord, 28, 4, 2.0))) < RTOL
 
 
-@pytest.mark.parametrize("n, osf", [(32, 2.0), (64, 1.25), (64, 1.5), (51, 1.25)])
-def test_fft_stages_3d_match_library_fft(n, osf):
-    """3D plans: the z axis through torch.fft around the library's (y, x)
-    plane stages against tn_apod_pad + torch.fft.fftn (forward) and
-    torch.fft.ifftn + tn_crop_apod (adjoint), for even and odd image sizes
-    (N = 51: grid 64, the centring of an odd axis)."""
-    coord = testing.kooshball(20, 16, n)
-    plan = tilenufft.fft.plan_nufft(coord, n, 4, osf, device=DEV)
+@pytest.mark.parametrize("shape, osf", [((32, 32, 32), 2.0), ((64, 64, 64), 1.25), ((64, 64, 64), 1.5),
+                                         ((51, 51, 51), 1.25), ((16, 32, 48), 2.0), ((40, 24, 32), 2.0)])
+def test_fft_stages_3d_match_library_fft(shape, osf):
+    """3D plans: the native z stage around the (y, x) plane stages against
+    tn_apod_pad + torch.fft.fftn (forward) and torch.fft.ifftn +
+    tn_crop_apod (adjoint), for even and odd image sizes (N = 51: grid 64,
+    the centring of an odd axis) and for three different axis lengths
+    (z, y, x), so that an exchanged axis or pitch shows."""
+    nz, ny, nx = shape
+    coord = testing.kooshball(20, 16, 2) * (np.array([nx, ny, nz], np.float32) / 2)
+    plan = tilenufft.fft.plan_nufft(coord, shape, 4, osf, device=DEV)
     nat = plan.interpolator
     assert nat.has_fft
     nimg = 3
     g = plan.os_shape
-    x = testing.complex_normal((nimg, n, n, n), seed=n).to(DEV)
+    x = testing.complex_normal((nimg, *shape), seed=nx).to(DEV)
     grid = torch.empty((nimg, *g), dtype=torch.complex64, device=DEV)
     got = nat.fft_forward(x, grid, nimg).clone()
     pad = torch.empty_like(grid)
     nat.apod_pad(x, pad, nimg)
     ref = torch.fft.fftn(pad, dim=(-3, -2, -1))
     assert rel(got.cpu().numpy(), ref.cpu().numpy()) < 1e-5
-    k = testing.complex_normal((nimg, *g), seed=n + 1).to(DEV)
-    img = torch.empty((nimg, n, n, n), dtype=torch.complex64, device=DEV)
+    k = testing.complex_normal((nimg, *g), seed=nx + 1).to(DEV)
+    img = torch.empty((nimg, *shape), dtype=torch.complex64, device=DEV)
     ref_img = torch.empty_like(img)
     nat.crop_apod(torch.fft.ifftn(k, dim=(-3, -2, -1)), ref_img, nimg)
     nat.fft_adjoint(k.clone(), img, nimg)
@@ -161,3 +165,21 @@ def test_stack_of_stars_short_z_axis_matches_oracle():
     k = testing.complex_normal(y.shape, seed=41).numpy()
     xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
     assert rel(xa, orc.apply_nufft_adj(k, p_ref)) < 1e-5
+
+
+def test_anisotropic_3d_nufft_through_fft_stages_matches_oracle():
+    """A 3D image with three different axis lengths (z, y, x) = (8, 16, 24)
+    at osf 2: grid (16, 32, 48), every axis through the native stages. The
+    forward and adjoint NUFFT match the oracle."""
+    shape = (8, 16, 24)
+    print(f"anisotropic 3D NUFFT {shape}", flush=True)
+    coord = testing.kooshball(40, 24, 2) * (np.array([24, 16, 8], np.float32) / 2)
+    plan = tilenufft.fft.plan_nufft(coord, shape, 4, 2.0, device=DEV)
+    assert plan.os_shape == (16, 32, 48) and plan.interpolator.has_fft
+    x = testing.complex_normal((2, *shape), seed=42).numpy()
+    y = tilenufft.fft.apply_nufft(torch.from_numpy(x).to(DEV), plan).cpu().numpy()
+    p_ref = orc.plan_nufft(coord, shape, 4, 2.0)
+    assert rel(y, orc.apply_nufft(x, p_ref)) < RTOL
+    k = testing.complex_normal(y.shape, seed=43).numpy()
+    xa = tilenufft.fft.apply_nufft_adj(torch.from_numpy(k).to(DEV), plan).cpu().numpy()
+    assert rel(xa, orc.apply_nufft_adj(k, p_ref)) < RTOL
