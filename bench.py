@@ -355,15 +355,18 @@ def serving_bench(dev, nreq=48, rate_hz=400.0, seed=7, warm=6):
     )
 
 
-def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, compare_tiles=True, reps=3, grid_mode=0):
+def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, compare_tiles=True, reps=3, grid_mode=0,
+                    tune=False):
     """3D kooshball kernels: 128^3 image, 8 coils, 20000 golden-means spokes x
     256 samples, KB width 4, osf 2.0 (256^3 grid) and 1.5 (192^3): the
     automatic plan's gridding (the gridding matrix) and staged interpolation,
     the whole forward and adjoint transforms, with
     ``compare_tiles`` the register-tile gridding and with ``compare_runs`` the
-    register-run gridding too. Durations from HIP events
-    on the launch stream; algorithmic bytes as for the 2D workload. Returns a
-    dict."""
+    register-run gridding too; with ``tune`` the autotuner's choice among the
+    matrix and register-tile griddings (with their split thresholds) and the
+    staged interpolation's 3D tiles, timed the same way. Durations from HIP
+    events on the launch stream; algorithmic bytes as for the 2D workload.
+    Returns a dict."""
     import tilenufft
     from tilenufft import testing
     from tilenufft.batch import BatchedNUFFT
@@ -407,6 +410,31 @@ def kooshball_bench(dev, osfs=(2.0, 1.5), compare_runs=False, compare_tiles=True
             retune(plan, grid_mode=2, tile=(16, 16, 16))
             b = BatchedNUFFT([plan], 8)
             case["grid_runs_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)
+            b = None
+        if tune:
+            from tilenufft.autotune import default_candidates, tune_batch
+
+            # the fast 3D candidates only: the register-run and LDS-atomic
+            # griddings take 25-300 ms here and are never chosen
+            cand = dict(default_candidates(3), grid_modes=[3, 4], grid_chunks=[0], matrix_splits=[128, 256],
+                        tiles_splits=[256, 384], interp_chunks=[], itiles=[(8, 8, 16), (8, 8, 8)],
+                        staged_chunks=[2])
+            log(f"bench: kooshball osf {osf} autotune")
+            t0 = time.perf_counter()
+            best = tune_batch([plan], 8, candidates=cand, reps=reps, use_cache=False)
+            torch.cuda.synchronize(dev)
+            tuned = {"tune_s": time.perf_counter() - t0, "grid_mode": best["grid_mode"],
+                     "grid_split": best["grid_split"], "interp_mode": best["interp_mode"], "itile": best["itile"],
+                     "interp_chunk": best["interp_chunk"], "plan_bytes": plan.interpolator.device_bytes()}
+            b = BatchedNUFFT([plan], 8)
+            b.kspace.copy_(testing.complex_normal(tuple(b.kspace.shape), generator=gen).to(dev))
+            tuned["grid_ms"] = time_kernel(lambda: b.gridding(b.grid), reps, dev)
+            tuned["grid_frac"] = b_grid / (tuned["grid_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+            tuned["interp_ms"] = time_kernel(lambda: b.interp(b.grid), reps, dev)
+            tuned["interp_frac"] = b_interp / (tuned["interp_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+            tuned["nufft_fwd_ms"] = time_kernel(lambda: b.forward(), reps, dev)
+            tuned["nufft_adj_ms"] = time_kernel(lambda: b.adjoint(), reps, dev)
+            case["tuned"] = tuned
             b = None
         out["cases"].append(case)
         plan = None
@@ -529,7 +557,7 @@ def main():
         torch.cuda.set_device(dev)
         # --kernels-only: the automatic plan only
         res = kooshball_bench(dev, compare_runs=not args.kernels_only, compare_tiles=not args.kernels_only,
-                              grid_mode=args.grid_mode)
+                              grid_mode=args.grid_mode, tune=not args.kernels_only)
         if args.calibrate_copy > 0:
             calibration_copy(dev, args.calibrate_copy)
         print(json.dumps(res), flush=True)
@@ -646,7 +674,7 @@ def main():
         plain_radial = plain_radial_bench(dev)
     kooshball = None
     if not args.no_kooshball and not args.kernels_only:
-        kooshball = kooshball_bench(dev)
+        kooshball = kooshball_bench(dev, tune=True)
 
     cgsense = None
     if not args.no_cgsense and not args.kernels_only:
