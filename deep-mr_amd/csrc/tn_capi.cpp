@@ -168,11 +168,13 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                             LaunchTables &t, bool force_gather = false, bool no_copy = false) {
     t.kind_interp = interp_kind(jobs);
     t.kind_grid = t.kind_interp;
-    int64_t max_plane = 1, max_region = 1;
+    int64_t max_plane = 1, max_region = 1, max_rows = 1, max_cells = 1;
     for (const Job &j : jobs) {
         max_plane = std::max<int64_t>(max_plane, (int64_t)j.T[2] * j.T[1] * (j.T[0] + kGridPad) + kWave);
         max_region = std::max<int64_t>(max_region, (int64_t)(j.IT[0] + j.W[0] - 1) *
                                                        (j.IT[1] + j.W[1] - 1) * (j.IT[2] + j.W[2] - 1));
+        max_rows = std::max<int64_t>(max_rows, (int64_t)(j.IT[1] + j.W[1] - 1) * (j.IT[2] + j.W[2] - 1));
+        max_cells = std::max<int64_t>(max_cells, (int64_t)j.G[0] * j.G[1] * j.G[2]);
     }
     const int wc = kind_width(t.kind_grid);
     const int gmode = plan_of.empty() ? kGridSpread : plan_of[0]->grid_mode;
@@ -191,11 +193,16 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
            grid_lds_bytes(lds_coils + 1, max_plane, wc, lmode, gndim) <= kGridLdsBudget)
         ++lds_coils;
     const size_t region_coil = (size_t)max_region * 2 * sizeof(float);
-    const int region_coils = (int)std::max<size_t>(1, kInterpLdsBudget / region_coil);
+    // the staged interpolation's row table (one int per (z, y) row of a
+    // region) follows the chunk's regions in LDS
+    const size_t rows_bytes = ((size_t)max_rows * sizeof(int) + 15) / 16 * 16;
+    const int region_coils =
+        (int)std::max<size_t>(1, (kInterpLdsBudget - std::min(rows_bytes, kInterpLdsBudget)) / region_coil);
 
     int mode = plan_of.empty() ? kInterpGather : plan_of[0]->p.interp_mode;
     if (mode == kInterpAuto) mode = kInterpStaged;
-    if (mode == kInterpStaged && region_coil > kInterpLdsBudget) mode = kInterpGather;
+    if (mode == kInterpStaged && region_coil + rows_bytes > kInterpLdsBudget) mode = kInterpGather;
+    if (mode == kInterpStaged && max_cells >= INT32_MAX) mode = kInterpGather;  // 32-bit row offsets
     if (kind_width(t.kind_interp) > kStagedMaxWidth) mode = kInterpGather;  // wide kernels gather
     if (force_gather) mode = kInterpGather;
     t.interp_mode = mode;
@@ -373,7 +380,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                                                   sizeof(float2) * std::max<int64_t>(t.st_elems, 1))
                                       : 0;
     t.grid_lds = gmode == kGridMatrix ? 0 : grid_lds_bytes(max_gc, max_plane, wc, lmode, gndim);
-    t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic : 0;
+    t.interp_lds = mode == kInterpStaged ? region_coil * (size_t)max_ic + rows_bytes : 0;
     t.interp_coils = max_ic;
     const int rc = upload_tables(jobs_st, interp, grid, fold, slot_base, t);
     t.device_bytes += copy_bytes;

@@ -367,38 +367,70 @@ __global__ __launch_bounds__(kBlock) void interp_staged_kernel(const Job *__rest
         }
     }
 
-    // regions no larger than the grid wrap at most once per axis, and the
-    // offsets of the item's coils fit 32 bits: the cheap load path
-    const bool near = Rx <= Gx && Ry <= Gy && Rz <= Gz && (int64_t)it.c1 * J.grid_bstride < INT32_MAX;
+    // wrapped grid offset of the first cell of each (z, y) row of the region,
+    // once per item, in the LDS words after the chunk's region
+    int *rowtab = reinterpret_cast<int *>(region + (size_t)chunk * cells);
+    const int RZY = Ry * Rz;
+    for (int i = threadIdx.x; i < RZY; i += kBlock) {
+        const int z = i / Ry, y = i - z * Ry;
+        rowtab[i] = (((oz + z) % Gz) * Gy + (oy + y) % Gy) * Gx;
+    }
+    // lane (xl, rr) loads column xl of region rows rr, rr + step, ... (rows
+    // of Rx cells; a row per lane group, no per-cell index arithmetic)
+    const bool rowwise = Rx <= kBlock;
+    const int step = rowwise ? kBlock / Rx : 1;
+    const int rr = rowwise ? (int)threadIdx.x / Rx : 0;
+    const int xl = (int)threadIdx.x - rr * Rx;
+    const int xoff = (ox + xl) % Gx;
     // every axis at the full compiled width: branch-free tap sums
     const bool full = nx == WC && (NDIM < 2 || ny == WC) && (NDIM < 3 || nz == WC);
     for (int cc = it.c0; cc < it.c1; cc += chunk) {
         const int ncoil = min(chunk, it.c1 - cc);
-        __syncthreads();  // previous chunk consumed
-        // kLoadBatch independent loads per lane in flight before their LDS writes
-        Idx4 q(threadIdx.x, kBlock, Rx, Ry, Rz);
-        const int total = ncoil * cells;
-        for (int i0 = threadIdx.x; i0 < total; i0 += kLoadBatch * kBlock) {
-            float2 v[kLoadBatch];
+        __syncthreads();  // previous chunk consumed; the row table written
+        const int rows = ncoil * RZY;
+        if (rowwise) {
+            if (rr < step) {
+                // kLoadBatch independent loads per lane in flight before their LDS writes
+                int r = rr, yz = rr;
+                int64_t coff = J.grid_off + (int64_t)cc * J.grid_bstride;
+                while (yz >= RZY) {
+                    yz -= RZY;
+                    coff += J.grid_bstride;
+                }
+                while (r < rows) {
+                    // the batch's rows first, then their table reads, loads
+                    // and LDS writes, each kind issued together
+                    int dst[kLoadBatch], row[kLoadBatch];
+                    int64_t co[kLoadBatch];
 #pragma unroll
-            for (int u = 0; u < kLoadBatch; ++u, q.next()) {
-                if (i0 + u * kBlock >= total) continue;
-                int gx = ox + q.x, gy = oy + q.y, gz = oz + q.z;
-                if (near) {  // one wrap per axis at most, 32-bit offsets from the job's grid
-                    gx -= gx >= Gx ? Gx : 0;
-                    gy -= gy >= Gy ? Gy : 0;
-                    gz -= gz >= Gz ? Gz : 0;
-                    v[u] = grid[J.grid_off + ((cc + (int)q.b) * (int)J.grid_bstride + (gz * Gy + gy) * Gx + gx)];
-                } else {
-                    while (gx >= Gx) gx -= Gx;
-                    while (gy >= Gy) gy -= Gy;
-                    while (gz >= Gz) gz -= Gz;
-                    v[u] = grid[J.grid_off + (int64_t)(cc + q.b) * J.grid_bstride + ((int64_t)gz * Gy + gy) * Gx + gx];
+                    for (int u = 0; u < kLoadBatch; ++u) {
+                        dst[u] = r < rows ? r * Rx + xl : -1;
+                        row[u] = yz;
+                        co[u] = coff;
+                        r += step;
+                        yz += step;
+                        while (yz >= RZY) {
+                            yz -= RZY;
+                            coff += J.grid_bstride;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLoadBatch; ++u) row[u] = rowtab[row[u]];
+                    float2 v[kLoadBatch];
+#pragma unroll
+                    for (int u = 0; u < kLoadBatch; ++u)
+                        if (dst[u] >= 0) v[u] = grid[co[u] + row[u] + xoff];
+#pragma unroll
+                    for (int u = 0; u < kLoadBatch; ++u)
+                        if (dst[u] >= 0) region[dst[u]] = v[u];
                 }
             }
-#pragma unroll
-            for (int u = 0; u < kLoadBatch; ++u)
-                if (i0 + u * kBlock < total) region[i0 + u * kBlock] = v[u];
+        } else {  // rows longer than the workgroup: cell by cell
+            for (int i = threadIdx.x; i < rows * Rx; i += kBlock) {
+                const int r = i / Rx, x = i - r * Rx;
+                const int b = r / RZY, yz = r - b * RZY;
+                region[i] = grid[J.grid_off + (int64_t)(cc + b) * J.grid_bstride + rowtab[yz] + (ox + x) % Gx];
+            }
         }
         __syncthreads();
 #pragma unroll
