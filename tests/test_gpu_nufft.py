@@ -272,6 +272,26 @@ def test_staged_interp_3d_same_bits_for_every_tiling():
     assert rel(outs[0].cpu().numpy(), orc.apply_nufft(x, orc.plan_nufft(coord, n, 4, 2.0))) < RTOL
 
 
+def test_staged_interp_rows_longer_than_workgroup():
+    """Interpolation tiles whose region rows (tile + W - 1 cells) are longer
+    than the 256 lanes of a workgroup take the cell-by-cell region loads:
+    the k-space matches the oracle and is the same bits as with the default
+    16 x 16 tiles (a sample's sum does not depend on the tiling)."""
+    n = 192
+    print("staged interp rows longer than the workgroup", flush=True)
+    coord = testing.golden_angle_radial(16, 128, n)
+    x = cnormal((2, n, n), 34)
+    xt = torch.from_numpy(x).to(DEV)
+    wide = tilenufft.fft.plan_nufft(coord, n, 4, 1.5, device=DEV, interp_mode=2, itile=(4, 288))
+    assert wide.os_shape == (288, 288)
+    base = tilenufft.fft.plan_nufft(coord, n, 4, 1.5, device=DEV, interp_mode=2, itile=(16, 16))
+    a = tilenufft.fft.apply_nufft(xt, wide)
+    b = tilenufft.fft.apply_nufft(xt, base)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+    assert rel(a.cpu().numpy(), orc.apply_nufft(x, orc.plan_nufft(coord, n, 4, 1.5))) < RTOL
+
+
 @pytest.mark.parametrize("split", [-1, 3, 64, 500])
 def test_grid_split_matches_oracle_and_is_deterministic(split):
     # hot tiles split across workgroups, partials folded in a fixed order
