@@ -230,7 +230,6 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
     std::vector<Item> interp, grid, fold, tr;
     std::vector<int64_t> slot_base;
     int64_t scratch = 0;
-    int64_t max_fold = 0;  // largest partial tile (coils x cells) of the tile folds
     for (size_t ji = 0; ji < jobs.size(); ++ji) {
         const Job &j = jobs[ji];
         const tn_plan *pl = plan_of[ji];
@@ -337,15 +336,22 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                     const int32_t e0 = b0 + q * per, e1 = std::min(b1, e0 + per);
                     grid.push_back(Item{(int)ji, (int)tl, c0, c1, e0, e1, q, slot});
                 }
-                fold.push_back(Item{(int)ji, (int)tl, c0, c1, slot, nparts, 0, 0});
-                max_fold = std::max<int64_t>(max_fold, (int64_t)(c1 - c0) * cells);
+                // fold items: element ranges [part, slot) of the partial
+                // tile; one workgroup walks a tile of few parts, while a tile
+                // of many parts (a dependent chain per element) spreads its
+                // elements over up to kFoldChunks workgroups
+                const int64_t elems = (int64_t)(c1 - c0) * cells;
+                const int64_t nsub = std::max<int64_t>(
+                    1, std::min<int64_t>({(int64_t)kFoldChunks, (elems + kBlock - 1) / kBlock,
+                                          ((int64_t)nparts + kFoldChainParts - 1) / kFoldChainParts}));
+                const int64_t fper = (elems + nsub - 1) / nsub;
+                for (int64_t k = 0; k < nsub; ++k)
+                    fold.push_back(Item{(int)ji, (int)tl, c0, c1, slot, nparts, (int)(k * fper),
+                                        (int)std::min(elems, (k + 1) * fper)});
             }
         }
     }
     t.scratch_floats = scratch;
-    // one thread per partial-tile element where that is at most kFoldChunks
-    // workgroups per fold item (the part sums are dependent chains)
-    t.fold_chunks = (int)std::min<int64_t>(kFoldChunks, std::max<int64_t>(1, (max_fold + kBlock - 1) / kBlock));
     t.grid_coils = max_gc;
     std::vector<Job> jobs_st(jobs);
     if ((gmode == kGridMatrix || gmode == kGridTiles) && st_cw > 0 && !st_none) {

@@ -38,6 +38,7 @@ constexpr int kInterpChunk3d = 2;
 constexpr int interp_segment(int ndim) { return 256 * staged_ept(ndim); }  // samples per staged work item
 constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
 constexpr int kFoldChunks = 32;          // most workgroups per tile fold item
+constexpr int kFoldChainParts = 16;      // parts per element chain above which a tile fold is spread wider
 constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch
 constexpr int kGridPad = 4;      // accumulator row padding (floats)
 
@@ -142,7 +143,6 @@ struct LaunchTables {
     int64_t n_interp = 0;
     int64_t n_grid = 0;
     int64_t n_fold = 0;
-    int fold_chunks = 1;       // workgroups per tile fold item (kBlock elements of its partial tile each)
     int64_t scratch_floats = 0;
     int grid_coils = 1;        // coils per gridding workgroup (LDS bound)
     size_t grid_lds = 0;       // dynamic LDS bytes for the gridding kernel

@@ -1303,6 +1303,8 @@ __global__ __launch_bounds__(kWave) void matrix_fold_kernel(const Job *__restric
 }
 
 // Fold of split gridding tiles: out = part 0 + part 1 + ... in that order.
+// Item: unit = tile, coils c0 .. c1 - 1, e0 = slot, e1 = parts, elements
+// [part, slot) of the partial tile ((coil, z, y, x) order), lane by lane.
 constexpr int kFoldBatch = 16;
 __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jobs,
                                                       const Item *__restrict__ items,
@@ -1322,32 +1324,28 @@ __global__ __launch_bounds__(kBlock) void fold_kernel(const Job *__restrict__ jo
     const int ncoil = it.c1 - it.c0;
     const int nparts = it.e1;
     const float *src = scratch + slot_base[it.e0];
-    const int64_t pstride = 2 * (int64_t)ncoil * cells;
+    const int64_t ps = (int64_t)ncoil * cells;  // float2 elements per part
     const int Gx = J.G[0], Gy = J.G[1];
-    // workgroup blockIdx.y of gridDim.y takes elements blockIdx.y * kBlock + t,
-    // then every kBlock * gridDim.y-th
-    const int start = blockIdx.y * kBlock + threadIdx.x, stride = kBlock * gridDim.y;
-    Idx4 c(start, stride, Tx, Ty, Tz);
-    for (int i = start; i < ncoil * cells; i += stride, c.next()) {
+    const int start = it.part + (int)threadIdx.x;
+    Idx4 c(start, kBlock, Tx, Ty, Tz);
+    for (int i = start; i < it.slot; i += kBlock, c.next()) {
         if (c.x < ex && c.y < ey && c.z < ez) {
             const float2 *s2 = (const float2 *)(src + 2 * (int64_t)i);
             float2 a = s2[0];
-            int q = 1;
-            // kFoldBatch parts' loads in flight, added in part order
-            for (; q + kFoldBatch <= nparts; q += kFoldBatch) {
+            // up to kFoldBatch parts' loads in flight, added in part order
+            for (int q = 1; q < nparts; q += kFoldBatch) {
+                const int nb = min(kFoldBatch, nparts - q);
                 float2 v[kFoldBatch];
 #pragma unroll
-                for (int u = 0; u < kFoldBatch; ++u) v[u] = s2[(int64_t)(q + u) * (pstride / 2)];
+                for (int u = 0; u < kFoldBatch; ++u)
+                    if (u < nb) v[u] = s2[(int64_t)(q + u) * ps];
 #pragma unroll
                 for (int u = 0; u < kFoldBatch; ++u) {
-                    a.x += v[u].x;
-                    a.y += v[u].y;
+                    if (u < nb) {
+                        a.x += v[u].x;
+                        a.y += v[u].y;
+                    }
                 }
-            }
-            for (; q < nparts; ++q) {
-                const float2 v = s2[(int64_t)q * (pstride / 2)];
-                a.x += v.x;
-                a.y += v.y;
             }
             grid[J.grid_off + (int64_t)(it.c0 + c.b) * J.grid_bstride +
                  ((int64_t)(oz + c.z) * Gy + (oy + c.y)) * Gx + (ox + c.x)] = a;
@@ -1786,7 +1784,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                            (const float2 *)t.d_st, grid, scratch, t.d_slot_base);
         TN_HIP(hipGetLastError());
         if (t.n_fold > 0) {
-            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold, (unsigned)t.fold_chunks), dim3(kBlock), 0, stream, t.d_jobs,
+            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
                                t.d_fold, scratch, t.d_slot_base, grid);
             TN_HIP(hipGetLastError());
         }
@@ -1811,7 +1809,7 @@ int launch_grid(const LaunchTables &t, const float2 *samp, float2 *grid, const f
                        t.d_grid, t.n_grid, samp, grid, weight, scratch, t.d_slot_base, lr);
     TN_HIP(hipGetLastError());
     if (t.n_fold > 0) {
-        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold, (unsigned)t.fold_chunks), dim3(kBlock), 0, stream, t.d_jobs,
+        hipLaunchKernelGGL(fold_kernel, dim3((unsigned)t.n_fold), dim3(kBlock), 0, stream, t.d_jobs,
                            t.d_fold, scratch, t.d_slot_base, grid);
         TN_HIP(hipGetLastError());
     }
