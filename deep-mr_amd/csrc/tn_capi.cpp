@@ -336,18 +336,7 @@ static int build_tables_for(const std::vector<Job> &jobs, const std::vector<cons
                     const int32_t e0 = b0 + q * per, e1 = std::min(b1, e0 + per);
                     grid.push_back(Item{(int)ji, (int)tl, c0, c1, e0, e1, q, slot});
                 }
-                // fold items: element ranges [part, slot) of the partial
-                // tile; one workgroup walks a tile of few parts, while a tile
-                // of many parts (a dependent chain per element) spreads its
-                // elements over up to kFoldChunks workgroups
-                const int64_t elems = (int64_t)(c1 - c0) * cells;
-                const int64_t nsub = std::max<int64_t>(
-                    1, std::min<int64_t>({(int64_t)kFoldChunks, (elems + kBlock - 1) / kBlock,
-                                          ((int64_t)nparts + kFoldChainParts - 1) / kFoldChainParts}));
-                const int64_t fper = (elems + nsub - 1) / nsub;
-                for (int64_t k = 0; k < nsub; ++k)
-                    fold.push_back(Item{(int)ji, (int)tl, c0, c1, slot, nparts, (int)(k * fper),
-                                        (int)std::min(elems, (k + 1) * fper)});
+                tile_fold_items((int)ji, (int)tl, c0, c1, slot, nparts, (int64_t)(c1 - c0) * cells, kBlock, fold);
             }
         }
     }

@@ -37,8 +37,6 @@ constexpr int kInterpChunk2d = 8;
 constexpr int kInterpChunk3d = 2;
 constexpr int interp_segment(int ndim) { return 256 * staged_ept(ndim); }  // samples per staged work item
 constexpr int kGridSplitDefault = 2048;  // max tile entries per gridding workgroup
-constexpr int kFoldChunks = 32;          // most workgroups per tile fold item
-constexpr int kFoldChainParts = 16;      // parts per element chain above which a tile fold is spread wider
 constexpr int kGridBatch = 256;  // tile entries staged in LDS per gridding batch
 constexpr int kGridPad = 4;      // accumulator row padding (floats)
 

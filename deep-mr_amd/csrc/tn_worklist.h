@@ -23,12 +23,32 @@ inline int matrix_stage(int ndim) { return ndim == 3 ? kMatrixStage : kMatrixSta
 // interpolation: unit = tile, [e0, e1) = slice of its entries. Gridding:
 // unit = tile, [e0, e1) = slice of its entries; part / slot >= 0 when the
 // tile is split (partial `part` of fold slot `slot`), slot = -1 otherwise.
-// Fold items: unit = tile, e0 = slot, e1 = number of parts. Matrix gridding
-// items: see matrix_items.
+// Tile fold items: unit = tile, e0 = slot, e1 = number of parts, elements
+// [part, slot) of the partial tile. Matrix gridding items: see matrix_items.
 struct alignas(16) Item {
     int32_t job, unit, c0, c1;
     int32_t e0, e1, part, slot;
 };
+
+constexpr int kFoldChunks = 32;      // most workgroups per tile fold item
+constexpr int kFoldChainParts = 16;  // parts per element chain above which a tile fold is spread wider
+
+// Fold items of one split gridding tile (coils c0 .. c1 - 1, `nparts`
+// partial tiles of `elems` elements in fold slot `slot`): one workgroup of
+// `block` lanes walks a tile of at most kFoldChainParts parts; a tile of more
+// parts (a longer dependent chain per element) spreads its elements over
+// ceil(parts / kFoldChainParts) contiguous ranges, at most kFoldChunks and at
+// most one per `block` elements (one element per lane at the limit).
+inline void tile_fold_items(int job, int tile, int c0, int c1, int slot, int nparts, int64_t elems, int block,
+                            std::vector<Item> &fold) {
+    const int64_t nsub = std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)kFoldChunks, (elems + block - 1) / block,
+                              ((int64_t)nparts + kFoldChainParts - 1) / kFoldChainParts}));
+    const int64_t per = (elems + nsub - 1) / nsub;
+    for (int64_t k = 0; k < nsub && k * per < elems; ++k)
+        fold.push_back(Item{job, tile, c0, c1, slot, nparts, (int32_t)(k * per),
+                            (int32_t)std::min(elems, (k + 1) * per)});
+}
 
 // Matrix-gridding work of one frame, independent of the coils: a run of
 // ncells consecutive cells from `cell` whose entries are [e0, e0 + ne), or,

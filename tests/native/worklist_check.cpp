@@ -3,8 +3,9 @@
 // first), short runs stay within the kernel's cell and LDS-stage limits and
 // carry the entry range their cells' rows span, and the items cover every
 // (run, coil record) with long rows split into in-order parts whose partial
-// records are disjoint and inside the scratch size. Prints "ok <cases>" or
-// the first failure.
+// records are disjoint and inside the scratch size; and that the fold items
+// of split gridding tiles cover each partial tile's elements once in
+// contiguous ranges. Prints "ok <cases>" or the first failure.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -117,6 +118,32 @@ int main() {
         if (gi != grid.size()) return fail("extra grid items", cs, -1);
         if (empty && (!fold.empty() || scratch != 0)) return fail("an empty matrix needs no parts", cs, -1);
         if (fi != fold.size()) return fail("extra fold items", cs, -1);
+        // tile folds: the element ranges of a split tile are contiguous,
+        // cover [0, elems) once, carry the tile's fields, number at most
+        // kFoldChunks (one for a tile of at most kFoldChainParts parts), and
+        // hold at most one element per lane when the chain is long enough
+        {
+            const int block = 256;
+            const int64_t elems = 1 + rng() % (16 * 4096);
+            const int nparts = 2 + (int)(rng() % (cs % 3 ? 40 : 3000));
+            std::vector<tn::Item> tf;
+            tn::tile_fold_items(3, 17, 2, 9, 5, nparts, elems, block, tf);
+            if (tf.empty() || (int)tf.size() > tn::kFoldChunks) return fail("tile fold item count", cs, elems);
+            if (nparts <= tn::kFoldChainParts && tf.size() != 1) return fail("light tile split", cs, elems);
+            int64_t next = 0;
+            for (const tn::Item &f : tf) {
+                if (f.job != 3 || f.unit != 17 || f.c0 != 2 || f.c1 != 9 || f.e0 != 5 || f.e1 != nparts)
+                    return fail("tile fold item fields", cs, elems);
+                if (f.part != next || f.slot <= f.part) return fail("tile fold ranges", cs, elems);
+                const int64_t lanes_needed = (elems + block - 1) / block;
+                if (lanes_needed <= tn::kFoldChunks &&
+                    (nparts + tn::kFoldChainParts - 1) / tn::kFoldChainParts >= lanes_needed &&
+                    f.slot - f.part > block)
+                    return fail("tile fold range wider than a workgroup", cs, elems);
+                next = f.slot;
+            }
+            if (next != elems) return fail("tile fold ranges do not cover the tile", cs, elems);
+        }
         ++cases;
     }
     std::printf("ok %d\n", cases);
